@@ -1,0 +1,256 @@
+// ec_problems.cpp -- host-side initial data of the euler_compare test
+// problems, written into any ec_layout.  Host code on purpose: the reference
+// keeps problem setup on the host and both sides then see bit-identical
+// initial data.  Citations are relative to
+// amrex/apps/euler_compare/src/euler_compare/.
+#include <cmath>
+
+#include "../../include/ec_amd.h"
+
+namespace {
+
+struct P4 { double r, u, v, p; };
+
+void to_cons(const P4& q, double gamma, double* u)
+{
+  // core/state_primitives.hpp:113-117
+  const double kinetic = 0.5 * q.r * (q.u * q.u + q.v * q.v);
+  u[0] = q.r;
+  u[1] = q.r * q.u;
+  u[2] = q.r * q.v;
+  u[3] = q.p / (gamma - 1.0) + kinetic;
+}
+
+// problems/exact_riemann.hpp:2-35
+bool toro_state(int id, P4& l, P4& r)
+{
+  switch (id) {
+    case 1: l = {1.0, 0.0, 0.0, 1.0}; r = {0.125, 0.0, 0.0, 0.1}; return true;
+    case 2: l = {1.0, -2.0, 0.0, 0.4}; r = {1.0, 2.0, 0.0, 0.4}; return true;
+    case 3: l = {1.0, 0.0, 0.0, 1000.0}; r = {1.0, 0.0, 0.0, 0.01}; return true;
+    case 4: l = {1.0, 0.0, 0.0, 0.01}; r = {1.0, 0.0, 0.0, 100.0}; return true;
+    case 5: l = {5.99924, 19.5975, 0.0, 460.894}; r = {5.99242, -6.19633, 0.0, 46.0950}; return true;
+    case 93: l = {0.445, 1.698, 0.0, 3.528}; r = {0.5, 0.0, 0.0, 0.571}; return true;
+    case 94: l = {1000.0, 1.0, 0.0, 1.0e5}; r = {0.01, 1.0, 0.0, 1.0e5}; return true;
+    case 95: l = {1.0, 0.0, 0.0, 1.01}; r = {1.0, 0.0, 0.0, 1.0}; return true;
+    case 96: l = {1.0, 0.0, 0.0, 1.0001}; r = {1.0, 0.0, 0.0, 1.0}; return true;
+    case 97: l = {1.0, 2.0, 0.0, 0.1}; r = {1.0, -2.0, 0.0, 0.1}; return true;
+    case 90: l = {1.0, 0.0, 0.0, 1.0}; r = {1.0, 0.0, 0.0, 1.0}; return true;
+    case 91: l = {2.0, 0.0, 0.0, 1.0}; r = {1.0, 0.0, 0.0, 1.0}; return true;
+    case 92: l = {2.0, 0.1, 0.0, 1.0}; r = {1.0, 0.1, 0.0, 1.0}; return true;
+  }
+  return false;
+}
+
+// problems/problem_state_functions.hpp:2-30 with the velocity scale
+// mach * sqrt(gamma p0 / rho_outer) (core/state_primitives.hpp:207-215).
+P4 gresho_state(double x, double y, const ec_problem_desc& d)
+{
+  const double dx = x - d.vortex_center[0];
+  const double dy = y - d.vortex_center[1];
+  const double r = std::sqrt(dx * dx + dy * dy);
+  const double rho = d.density_outer;
+  const double vs = d.mach * std::sqrt(d.gamma * d.pressure / d.density_outer);
+  double tf = 0.0;
+  double pf = -2.0 + 4.0 * std::log(2.0);
+  if (r < 0.2) {
+    tf = 5.0 * r;
+    pf = 12.5 * r * r;
+  } else if (r < 0.4) {
+    tf = 2.0 - 5.0 * r;
+    pf = 12.5 * r * r + 4.0 * (1.0 - 5.0 * r - std::log(0.2) + std::log(r));
+  }
+  const double vt = vs * tf;
+  const double p = d.pressure + rho * vs * vs * pf;
+  if (r < 1.0e-14) return P4{rho, 0.0, 0.0, p};
+  return P4{rho, -vt * dy / r, vt * dx / r, p};
+}
+
+// problems/problem_state_functions.hpp:32-71
+double periodic_nearest_delta(double x, double c, double lo, double hi)
+{
+  const double length = hi - lo;
+  double delta = x - c;
+  if (length > 0.0) delta -= length * std::floor(delta / length + 0.5);
+  return delta;
+}
+
+P4 isentropic_vortex_state(double x, double y, const ec_problem_desc& d)
+{
+  const double pi = 3.141592653589793238462643383279502884;
+  const double g = d.gamma;
+  const double s = d.isentropic_vortex_strength;
+  const double dx = periodic_nearest_delta(x, d.vortex_center[0], d.prob_lo[0], d.prob_hi[0]);
+  const double dy = periodic_nearest_delta(y, d.vortex_center[1], d.prob_lo[1], d.prob_hi[1]);
+  const double r2 = dx * dx + dy * dy;
+  const double exp_half = std::exp(0.5 * (1.0 - r2));
+  const double exp_full = std::exp(1.0 - r2);
+  const double temp = 1.0 - (g - 1.0) * s * s / (8.0 * g * pi * pi) * exp_full;
+  const double rho = d.density_outer * std::pow(temp, 1.0 / (g - 1.0));
+  const double p = d.pressure * std::pow(temp, g / (g - 1.0));
+  const double vf = s / (2.0 * pi) * exp_half;
+  return P4{rho, d.velocity[0] - vf * dy, d.velocity[1] + vf * dx, p};
+}
+
+// problems/problem_state_functions.hpp:73-98 (Liska-Wendroff quadrant)
+P4 quadrant_state(double x, double y, const ec_problem_desc& d)
+{
+  const bool right = x >= d.riemann_interface_x;
+  const bool top = y >= d.riemann_interface_y;
+  if (right && top) return P4{1.5, 0.0, 0.0, 1.5};
+  if (!right && top) return P4{0.532258064516129, 1.206045378311055, 0.0, 0.3};
+  if (!right && !top) {
+    return P4{0.137992831541219, 1.206045378311055, 1.206045378311055, 0.029032258064516};
+  }
+  return P4{0.532258064516129, 0.0, 1.206045378311055, 0.3};
+}
+
+// problems/problem_initialization.hpp:2-102 (fixed geometry of the
+// reference's shock-density bubble: shock x=0.2, bubble (0.5,0) r=0.2).
+const P4 kBubblePostShock{2.8181818181818183, 1.6064386578049976, 0.0, 5.0};
+
+double bubble_fraction(double xc, double yc, double dx, double dy, double shock_x, double bx,
+                       double by, double br)
+{
+  double inside = 0.0;
+  for (int sy = 0; sy < 4; ++sy) {
+    for (int sx = 0; sx < 4; ++sx) {
+      const double x = xc + (((double)sx + 0.5) / 4.0 - 0.5) * dx;
+      const double y = yc + (((double)sy + 0.5) / 4.0 - 0.5) * dy;
+      const double rx = x - bx, ry = y - by;
+      if (x >= shock_x && rx * rx + ry * ry <= br * br) inside += 1.0;
+    }
+  }
+  return inside / 16.0;
+}
+
+}  // namespace
+
+extern "C" ec_status ec_post_shock_state(double gamma, double mach, double rho1, double p1,
+                                         double* prim_out)
+{
+  if (!prim_out || !(gamma > 1.0) || !(mach >= 1.0) || !(rho1 > 0.0) || !(p1 > 0.0)) {
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  // Rankine-Hugoniot for a shock moving at mach * c1 into gas at rest.
+  const double m2 = mach * mach;
+  const double c1 = std::sqrt(gamma * p1 / rho1);
+  const double rho2 = rho1 * (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0);
+  const double p2 = p1 * (1.0 + 2.0 * gamma / (gamma + 1.0) * (m2 - 1.0));
+  const double u2 = mach * c1 * (1.0 - rho1 / rho2);
+  prim_out[0] = rho2;
+  prim_out[1] = u2;
+  prim_out[2] = 0.0;
+  prim_out[3] = p2;
+  return EC_OK;
+}
+
+extern "C" ec_status ec_init_state(const ec_problem_desc* d, double* U, const ec_layout* lay)
+{
+  if (!d || !U || !lay || d->nx < 1 || d->ny < 1 || lay->row_stride < d->nx ||
+      lay->comp_stride < lay->row_stride * d->ny) {
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  // AMReX Geometry: dx = (prob_hi - prob_lo) / n_cell.
+  const double dx = (d->prob_hi[0] - d->prob_lo[0]) / d->nx;
+  const double dy = (d->prob_hi[1] - d->prob_lo[1]) / d->ny;
+  P4 tl{}, tr{};
+  if (d->kind == EC_PROBLEM_TORO && !toro_state(d->toro_test, tl, tr)) return EC_ERR_INVALID_ARGUMENT;
+  P4 post = kBubblePostShock;
+  if (d->kind == EC_PROBLEM_SHOCK_BUBBLE_MACH) {
+    double q[4];
+    const ec_status st = ec_post_shock_state(d->gamma, d->shock_mach, d->density_outer, d->pressure, q);
+    if (st != EC_OK) return st;
+    post = P4{q[0], q[1], q[2], q[3]};
+  }
+  for (int j = 0; j < d->ny; ++j) {
+    for (int i = 0; i < d->nx; ++i) {
+      const double x = d->prob_lo[0] + ((double)i + 0.5) * dx;
+      const double y = d->prob_lo[1] + ((double)j + 0.5) * dy;
+      P4 q{};
+      switch (d->kind) {
+        case EC_PROBLEM_TORO:  // problem_initialization.hpp:265-284
+          q = x < d->riemann_interface_x ? tl : tr;
+          break;
+        case EC_PROBLEM_GRESHO:
+          q = gresho_state(x, y, *d);
+          break;
+        case EC_PROBLEM_ISENTROPIC_VORTEX:
+          q = isentropic_vortex_state(x, y, *d);
+          break;
+        case EC_PROBLEM_RIEMANN_QUADRANT:
+          q = quadrant_state(x, y, *d);
+          break;
+        case EC_PROBLEM_SHOCK_DENSITY_BUBBLE:  // problem_initialization.hpp:91-102
+          if (x < 0.2) {
+            q = kBubblePostShock;
+          } else {
+            const double f = bubble_fraction(x, y, dx, dy, 0.2, 0.5, 0.0, 0.2);
+            q = P4{f * 0.1 + (1.0 - f) * 1.0, 0.0, 0.0, 1.0};
+          }
+          break;
+        case EC_PROBLEM_SHOCK_BUBBLE_MACH:
+          if (x < d->shock_x) {
+            q = post;
+          } else {
+            const double f = bubble_fraction(x, y, dx, dy, d->shock_x, d->blob_center[0],
+                                             d->blob_center[1], d->blob_radius);
+            q = P4{f * d->density_inner + (1.0 - f) * d->density_outer, 0.0, 0.0, d->pressure};
+          }
+          break;
+        case EC_PROBLEM_ADVECTION_BLOB: {  // problem_initialization.hpp:127-155
+          const double cx = d->blob_center[0], cy = d->blob_center[1];
+          const double r2 = (x - cx) * (x - cx) + (y - cy) * (y - cy);
+          const double rho = r2 <= d->blob_radius * d->blob_radius ? d->density_inner : d->density_outer;
+          const double u = d->velocity[0], v = d->velocity[1];
+          const long long k = (long long)j * lay->row_stride + i;
+          U[k] = rho;
+          U[lay->comp_stride + k] = rho * u;
+          U[2 * lay->comp_stride + k] = rho * v;
+          U[3 * lay->comp_stride + k] = d->pressure / (d->gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+          continue;
+        }
+        default:
+          return EC_ERR_INVALID_ARGUMENT;
+      }
+      double u[4];
+      to_cons(q, d->gamma, u);
+      const long long k = (long long)j * lay->row_stride + i;
+      for (int c = 0; c < 4; ++c) U[c * lay->comp_stride + k] = u[c];
+    }
+  }
+  return EC_OK;
+}
+
+extern "C" ec_status ec_problem_defaults(int kind, ec_problem_desc* d)
+{
+  if (!d) return EC_ERR_INVALID_ARGUMENT;
+  // core/run_config.hpp:2-55 defaults.
+  *d = ec_problem_desc{};
+  d->kind = kind;
+  d->nx = 64;
+  d->ny = 64;
+  d->prob_lo[0] = 0.0;
+  d->prob_lo[1] = 0.0;
+  d->prob_hi[0] = 1.0;
+  d->prob_hi[1] = 1.0;
+  d->gamma = 1.4;
+  d->toro_test = 1;
+  d->riemann_interface_x = 0.5;
+  d->riemann_interface_y = 0.5;
+  d->velocity[0] = 1.0;
+  d->velocity[1] = 0.5;
+  d->density_inner = 2.0;
+  d->density_outer = 1.0;
+  d->pressure = 1.0;
+  d->mach = 0.1;
+  d->blob_center[0] = 0.35;
+  d->blob_center[1] = 0.35;
+  d->blob_radius = 0.15;
+  d->vortex_center[0] = 0.0;
+  d->vortex_center[1] = 0.0;
+  d->isentropic_vortex_strength = 5.0;
+  d->shock_mach = 1.22;
+  d->shock_x = 0.5;
+  return EC_OK;
+}

@@ -1,0 +1,192 @@
+"""Explicit O2 update on the device, through the C-ABI.
+
+Mirrors the reference's explicit calls: ``euler_fv_one_step``
+(numerics/explicit_update.hpp:58-102), ``compute_euler_dt`` and
+``compute_diagnostics`` (numerics/diagnostic_metrics_timestep.hpp:18-47,
+:227-260), and the explicit branch of the driver loop
+(app/driver_main.hpp:224-333).  Citations are relative to
+amrex/apps/euler_compare/src/euler_compare/.
+
+States are torch float64 CUDA tensors of shape (4, ny, nx) (comp-major,
+components rho, mx, my, E); torch is only the device allocator and stream
+provider here, all arithmetic runs in the HIP kernels of libec_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import torch
+
+from ._lib import (BC, LIMITER, RIEMANN, RUN_STATUS, GridDesc, Layout, RunState, StepDiag,
+                   check, lib)
+
+
+@dataclass
+class Grid:
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    gamma: float = 1.4
+    riemann: str = "hllc"
+    limiter: str = "minmod"
+    spatial_order: int = 2
+    bc: tuple = ("periodic", "periodic", "periodic", "periodic")  # xlo, xhi, ylo, yhi
+    fixed_state: tuple = field(default_factory=lambda: (None, None, None, None))
+
+    def desc(self) -> GridDesc:
+        d = GridDesc()
+        d.nx, d.ny, d.dx, d.dy, d.gamma = self.nx, self.ny, self.dx, self.dy, self.gamma
+        d.riemann = RIEMANN[self.riemann]
+        d.limiter = LIMITER[self.limiter]
+        d.spatial_order = self.spatial_order
+        for s in range(4):
+            d.bc[s] = BC[self.bc[s]] if isinstance(self.bc[s], str) else int(self.bc[s])
+            fs = self.fixed_state[s]
+            if fs is not None:
+                for c in range(4):
+                    d.fixed_state[s][c] = float(fs[c])
+        return d
+
+
+def _ptr(t: torch.Tensor) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+@dataclass
+class StepResult:
+    rho_min: float
+    pressure_min: float
+    nonfinite_count: int
+    degenerate_count: int
+    max_rate: float
+
+
+class ExplicitSolver:
+    """Device explicit O2 solver bound to one grid and one HIP stream."""
+
+    def __init__(self, grid: Grid, device: torch.device | str = "cuda", stream=None):
+        self.grid = grid
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("ExplicitSolver runs on the GPU only")
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self._ctx = C.c_void_p()
+        with torch.cuda.device(self.device):
+            d = grid.desc()
+            check(None, lib().ec_create(C.byref(d), C.c_void_p(self.stream.cuda_stream),
+                                        C.byref(self._ctx)), "ec_create")
+        self.layout = Layout(grid.nx, grid.nx * grid.ny)
+
+    def close(self) -> None:
+        if self._ctx:
+            lib().ec_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def empty_state(self) -> torch.Tensor:
+        return torch.empty((4, self.grid.ny, self.grid.nx), dtype=torch.float64, device=self.device)
+
+    def _check_state(self, U: torch.Tensor) -> None:
+        if (U.dtype != torch.float64 or U.device != self.device or not U.is_contiguous()
+                or tuple(U.shape) != (4, self.grid.ny, self.grid.nx)):
+            raise ValueError(f"state must be contiguous float64 (4,{self.grid.ny},{self.grid.nx}) "
+                             f"on {self.device}")
+
+    def set_tile(self, tile_id: int) -> None:
+        check(self._ctx, lib().ec_set_explicit_tile(self._ctx, tile_id), "ec_set_explicit_tile")
+
+    @staticmethod
+    def tile_shapes() -> list[tuple[int, int, int]]:
+        out = []
+        for t in range(lib().ec_explicit_tile_count()):
+            a, b, c = C.c_int(), C.c_int(), C.c_int()
+            lib().ec_explicit_tile_shape(t, C.byref(a), C.byref(b), C.byref(c))
+            out.append((a.value, b.value, c.value))
+        return out
+
+    def step(self, U_in: torch.Tensor, U_out: torch.Tensor, dt: float,
+             diag: bool = True) -> StepResult | None:
+        """One explicit step U_out = U_in - dt div F(U_in)."""
+        self._check_state(U_in)
+        self._check_state(U_out)
+        d = StepDiag()
+        check(self._ctx, lib().ec_explicit_step(self._ctx, _ptr(U_in), _ptr(U_out),
+                                                 C.byref(self.layout), float(dt),
+                                                 C.byref(d) if diag else None),
+              "ec_explicit_step")
+        if not diag:
+            return None
+        return StepResult(d.rho_min, d.pressure_min, d.nonfinite_count, d.degenerate_count,
+                          d.max_rate)
+
+    def step_inplace(self, U: torch.Tensor, dt: float) -> StepResult:
+        self._check_state(U)
+        d = StepDiag()
+        check(self._ctx, lib().ec_explicit_step_inplace(self._ctx, _ptr(U), C.byref(self.layout),
+                                                         float(dt), C.byref(d)),
+              "ec_explicit_step_inplace")
+        return StepResult(d.rho_min, d.pressure_min, d.nonfinite_count, d.degenerate_count,
+                          d.max_rate)
+
+    def diagnostics(self, U: torch.Tensor) -> StepResult:
+        self._check_state(U)
+        d = StepDiag()
+        check(self._ctx, lib().ec_state_diagnostics(self._ctx, _ptr(U), C.byref(self.layout),
+                                                     C.byref(d)), "ec_state_diagnostics")
+        return StepResult(d.rho_min, d.pressure_min, d.nonfinite_count, d.degenerate_count,
+                          d.max_rate)
+
+    @staticmethod
+    def euler_dt(max_rate: float, cfl: float, time: float, stop_time: float) -> float:
+        return lib().ec_euler_dt(max_rate, cfl, time, stop_time)
+
+    def new_run(self, cfl: float, stop_time: float, max_step: int) -> RunState:
+        rs = RunState()
+        rs.cfl, rs.stop_time, rs.max_step = cfl, stop_time, max_step
+        return rs
+
+    def advance(self, U_a: torch.Tensor, U_b: torch.Tensor, nsteps: int,
+                run: RunState) -> torch.Tensor:
+        """Device-resident driver loop for up to nsteps; returns the buffer holding the state."""
+        self._check_state(U_a)
+        self._check_state(U_b)
+        in_b = C.c_int()
+        check(self._ctx, lib().ec_explicit_advance(self._ctx, _ptr(U_a), _ptr(U_b),
+                                                    C.byref(self.layout), int(nsteps),
+                                                    C.byref(run), C.byref(in_b)),
+              "ec_explicit_advance")
+        return U_b if in_b.value else U_a
+
+    @staticmethod
+    def status_name(run: RunState) -> str:
+        return RUN_STATUS.get(run.status, str(run.status))
+
+
+def run_explicit(solver: ExplicitSolver, U0: torch.Tensor, cfl: float, stop_time: float,
+                 max_step: int, chunk: int = 256):
+    """Run the explicit driver loop from U0 to stop_time / max_step on the device.
+
+    Returns (U_final, RunState)."""
+    A = U0.clone()
+    B = torch.empty_like(A)
+    run = solver.new_run(cfl, stop_time, max_step)
+    while True:
+        before = run.step
+        res = solver.advance(A, B, chunk, run)
+        if res is B:
+            A, B = B, A
+        done = (run.status != 0 or run.step >= max_step or
+                run.time >= stop_time - 1.0e-14 or run.step == before)
+        if done:
+            return A, run

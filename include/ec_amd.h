@@ -1,0 +1,207 @@
+/*
+ * ec_amd.h -- C-ABI of the MI355X-native euler_compare update path.
+ *
+ * Plain C: pointers, sizes and POD structs only.  Every entry point returns an
+ * ec_status code and never aborts.  Device pointers are hipMalloc'd memory on
+ * the context's device; the caller owns them.  One context per host thread;
+ * a context is not re-entrant.  All work is enqueued on the context's HIP
+ * stream; entry points that return host values synchronise that stream.
+ *
+ * State layout (any AMReX FArrayBox-compatible strides): a state pointer
+ * addresses component 0 of valid cell (0,0); cell (i,j) of component c is at
+ * ptr[c*comp_stride + j*row_stride + i], components (rho, mx, my, E)
+ * (amrex/apps/euler_compare/src/euler_compare_app.cpp:47).  Ghost cells, if
+ * the caller's layout has them, are neither read nor written: the boundary
+ * rule of each side is applied on read (wrap, clamp, mirror or fixed state).
+ *
+ * Reference citations below are relative to
+ * amrex/apps/euler_compare/src/euler_compare/.
+ */
+#ifndef EC_AMD_H
+#define EC_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  EC_OK = 0,
+  EC_ERR_INVALID_ARGUMENT = 1,
+  EC_ERR_HIP = 2,
+  EC_ERR_OUT_OF_MEMORY = 3,
+  EC_ERR_UNSUPPORTED = 4
+} ec_status;
+
+/* Riemann solver selector: the euler.riemann values of core/solver_types.hpp:11
+ * (parsed at core/config_parse_helpers.hpp:44-61). */
+typedef enum {
+  EC_RIEMANN_RUSANOV = 0,
+  EC_RIEMANN_HLLC = 1,
+  EC_RIEMANN_LOW_MACH_HLLC = 2,
+  EC_RIEMANN_XIE_AM_HLLC_P = 3
+} ec_riemann;
+
+/* euler.slope_limiter (core/solver_types.hpp:14). */
+typedef enum { EC_LIMITER_MINMOD = 0, EC_LIMITER_MC = 1, EC_LIMITER_VAN_LEER = 2 } ec_limiter;
+
+/* Per-side ghost rule.  PERIODIC and OUTFLOW reproduce
+ * problems/boundary_conditions.hpp:2-25; FIXED (inflow state, priority over
+ * the other axis) and REFLECT (mirror, normal momentum negated) reproduce the
+ * shock-bubble fill at problems/boundary_conditions.hpp:27-73. */
+typedef enum { EC_BC_PERIODIC = 0, EC_BC_OUTFLOW = 1, EC_BC_REFLECT = 2, EC_BC_FIXED = 3 } ec_bc;
+
+typedef struct {
+  int nx, ny;               /* valid cells */
+  double dx, dy;            /* cell sizes */
+  double gamma;
+  int riemann;              /* ec_riemann */
+  int limiter;              /* ec_limiter */
+  int spatial_order;        /* 1 or 2 */
+  int bc[4];                /* ec_bc for x-low, x-high, y-low, y-high */
+  double fixed_state[4][4]; /* conserved state of an EC_BC_FIXED side */
+} ec_grid_desc;
+
+typedef struct {
+  long long row_stride;  /* doubles between (i,j) and (i,j+1) */
+  long long comp_stride; /* doubles between component planes */
+} ec_layout;
+
+typedef struct ec_ctx ec_ctx;
+
+/* Diagnostics of one state (core/basic_diagnostics.hpp: Diagnostics) plus the
+ * CFL spectral-rate maximum and the step's degenerate-star count. */
+typedef struct {
+  double rho_min;            /* min rho over valid cells, -inf where non-finite */
+  double pressure_min;       /* min p, -inf where non-finite */
+  int nonfinite_count;
+  int degenerate_count;      /* faces counted per adjacent valid cell */
+  double max_rate;           /* max (|u|+c)/dx + (|v|+c)/dy */
+} ec_step_diag;
+
+/* Device-resident run state of ec_explicit_advance, mirroring the explicit
+ * branch of the driver loop (app/driver_main.hpp:224-333). */
+typedef struct {
+  double time;
+  double stop_time;
+  double cfl;
+  double dt_last, dt_min, dt_max, dt_sum;
+  double rho_min, pressure_min, max_rate;
+  long long degenerate_total;
+  int step;
+  int max_step;
+  int nonfinite_count;
+  /* 0 running/ok, 1 initial_nonphysical_state, 2 invalid_time_step,
+   * 3 nonfinite_state, 4 nonpositive_density, 5 nonpositive_pressure */
+  int status;
+} ec_run_state;
+
+/* ---------------------------------------------------------------- context */
+/* Creates a context on the current HIP device.  stream: a hipStream_t, or
+ * NULL for the default stream.  No reference counterpart: it replaces the
+ * MultiFab/Geometry/RunConfig triple the reference threads through every call
+ * (app/driver_main.hpp:17-23). */
+ec_status ec_create(const ec_grid_desc* desc, void* stream, ec_ctx** out);
+ec_status ec_destroy(ec_ctx* ctx);
+const char* ec_status_string(ec_status s);
+/* Text of the last HIP error seen by this context (empty if none). */
+const char* ec_last_error(const ec_ctx* ctx);
+/* Device buffer helpers (optional; any hipMalloc'd memory works). */
+ec_status ec_alloc(ec_ctx* ctx, size_t bytes, void** dev_ptr);
+ec_status ec_free(ec_ctx* ctx, void* dev_ptr);
+ec_status ec_synchronize(ec_ctx* ctx);
+
+/* ----------------------------------------------------- explicit O2 step */
+/* One explicit step, out of place: U_out = U_in - dt*div F(U_in).
+ * Replaces int euler_fv_one_step(MultiFab&, const Geometry&, const RunConfig&,
+ * Real dt, Real time) (numerics/explicit_update.hpp:58-102): fused ghost rule,
+ * MUSCL reconstruction, every face flux once, the degenerate-star count
+ * (:2-56) and, as an epilogue on U_out, compute_diagnostics
+ * (numerics/diagnostic_metrics_timestep.hpp:18-47) and the max-rate reduction
+ * of compute_euler_dt (:227-260).  diag may be NULL (no synchronisation);
+ * otherwise the stream is synchronised and diag filled. */
+ec_status ec_explicit_step(ec_ctx* ctx, const double* U_in, double* U_out,
+                           const ec_layout* layout, double dt, ec_step_diag* diag);
+
+/* The same step in place, as the reference mutates its MultiFab: the result
+ * goes through a context-owned buffer and is copied back (the reference's
+ * MultiFab::Copy, numerics/explicit_update.hpp:100). */
+ec_status ec_explicit_step_inplace(ec_ctx* ctx, double* U, const ec_layout* layout,
+                                   double dt, ec_step_diag* diag);
+
+/* ---------------------------------------------------- CFL / diagnostics */
+/* The standalone max-rate + diagnostics reduction (wavefront shuffle).
+ * Replaces the reductions of compute_euler_dt and compute_diagnostics
+ * (numerics/diagnostic_metrics_timestep.hpp:18-47, :227-246). */
+ec_status ec_state_diagnostics(ec_ctx* ctx, const double* U, const ec_layout* layout,
+                               ec_step_diag* diag);
+/* dt from a max rate: compute_euler_dt's closing arithmetic
+ * (numerics/diagnostic_metrics_timestep.hpp:252-259).  Host-only, no context. */
+double ec_euler_dt(double max_rate, double cfl, double time, double stop_time);
+
+/* ------------------------------------------------- device-resident loop */
+/* Runs up to nsteps iterations of the explicit driver loop entirely on the
+ * device (dt from the previous step's fused max-rate, no host round trip),
+ * ping-ponging between U_a and U_b.  The loop stops stepping, exactly as
+ * app/driver_main.hpp:224-310 does, on a failure status, at max_step or at
+ * stop_time.  On return *result_in_b says which buffer holds the final state
+ * and io holds the updated run state.  io->status must be 0 and the initial
+ * diagnostics are evaluated on the first call (io->step == 0). */
+ec_status ec_explicit_advance(ec_ctx* ctx, double* U_a, double* U_b, const ec_layout* layout,
+                              int nsteps, ec_run_state* io, int* result_in_b);
+
+/* Kernel tile shape of the explicit step (autotuner hook).  tile_id indexes
+ * the compiled shapes; ec_explicit_tile_count() says how many exist and
+ * ec_explicit_tile_shape() describes one. */
+int ec_explicit_tile_count(void);
+ec_status ec_explicit_tile_shape(int tile_id, int* tx, int* ty, int* threads);
+ec_status ec_set_explicit_tile(ec_ctx* ctx, int tile_id);
+
+/* ---------------------------------------------------- host initial data */
+/* Test problems of problems/problem_initialization.hpp:420-435 (selector
+ * euler.problem, core/config_parse_helpers.hpp:11-42), plus
+ * EC_PROBLEM_SHOCK_BUBBLE_MACH: a shock of Mach number shock_mach at
+ * x = shock_x running into gas at rest (density_outer, pressure) with a
+ * circular bubble of density density_inner at blob_center, radius
+ * blob_radius (the configurable generalisation of the reference's fixed
+ * shock-density bubble, problems/problem_initialization.hpp:2-102). */
+typedef enum {
+  EC_PROBLEM_ADVECTION_BLOB = 0,
+  EC_PROBLEM_TORO = 1,
+  EC_PROBLEM_GRESHO = 2,
+  EC_PROBLEM_RIEMANN_QUADRANT = 3,
+  EC_PROBLEM_ISENTROPIC_VORTEX = 4,
+  EC_PROBLEM_SHOCK_DENSITY_BUBBLE = 5,
+  EC_PROBLEM_SHOCK_BUBBLE_MACH = 6
+} ec_problem_kind;
+
+typedef struct {
+  int kind;                 /* ec_problem_kind */
+  int nx, ny;
+  double prob_lo[2], prob_hi[2];
+  double gamma;
+  int toro_test;            /* prob.test */
+  double riemann_interface_x, riemann_interface_y;
+  double velocity[2];
+  double density_inner, density_outer, pressure, mach;
+  double blob_center[2], blob_radius;
+  double vortex_center[2];
+  double isentropic_vortex_strength;
+  double shock_mach, shock_x;
+} ec_problem_desc;
+
+/* Defaults of core/run_config.hpp:2-55 for the given problem kind. */
+ec_status ec_problem_defaults(int kind, ec_problem_desc* desc);
+/* Writes the initial conserved state into HOST memory with the given layout
+ * (initialize_state, problems/problem_initialization.hpp:420-435; cell
+ * centres prob_lo + (i+1/2) dx with dx = (prob_hi - prob_lo)/n). */
+ec_status ec_init_state(const ec_problem_desc* desc, double* host_U, const ec_layout* layout);
+/* Post-shock primitive state (rho, u, v, p) behind a shock of the given Mach
+ * number moving into gas at rest (Rankine-Hugoniot). */
+ec_status ec_post_shock_state(double gamma, double mach, double rho1, double p1, double* prim_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

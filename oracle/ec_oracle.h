@@ -1,0 +1,89 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * A serial CPU restatement of the euler_compare update path, written in plain
+ * C from the reference's formulas and operation order.  It exists to check the
+ * HIP product path (tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg are its only callers).  Nothing in the product links it.
+ *
+ * Layout: every field is compact SoA, comp-major: U[c*ny*nx + j*nx + i],
+ * c in {0:rho, 1:mx, 2:my, 3:E} (amrex/apps/euler_compare/src/
+ * euler_compare_app.cpp:47).  Ghost cells are never stored: they are produced
+ * on read by the boundary rule of each side (see ecor_load).
+ */
+#ifndef EC_ORACLE_H
+#define EC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ECOR_RUSANOV = 0, ECOR_HLLC = 1, ECOR_LOWMACH_HLLC = 2, ECOR_XIE_HLLCP = 3 };
+enum { ECOR_MINMOD = 0, ECOR_MC = 1, ECOR_VANLEER = 2 };
+enum { ECOR_BC_PERIODIC = 0, ECOR_BC_OUTFLOW = 1, ECOR_BC_REFLECT = 2, ECOR_BC_FIXED = 3 };
+
+typedef struct {
+  int nx, ny;
+  double dx, dy;
+  double gamma;
+  int riemann;   /* ECOR_RUSANOV .. ECOR_XIE_HLLCP */
+  int limiter;   /* ECOR_MINMOD .. ECOR_VANLEER */
+  int order;     /* 1 or 2 */
+  int bc[4];     /* xlo, xhi, ylo, yhi */
+  double fixed[4][4]; /* conserved state used by an ECOR_BC_FIXED side */
+} ecor_grid;
+
+/* ---- pointwise algebra (state_primitives.hpp / reconstruction_fluxes.hpp) */
+void ecor_to_primitive(const double* u, double gamma, double* q);
+void ecor_to_conserved(const double* q, double gamma, double* u);
+double ecor_limited_slope(double backward, double forward, int limiter);
+void ecor_physical_flux(const double* q, int axis, double gamma, double* f);
+void ecor_interface_flux(const double* qL, const double* qR, int axis, double gamma,
+                         int riemann, double sensor_f, double* f);
+int ecor_hllc_star_degenerate(const double* qL, const double* qR, int axis, double gamma);
+double ecor_pressure_ratio(double pL, double pR);
+void ecor_exact_riemann_star(const double* qL, const double* qR, double gamma,
+                             double* p_star, double* u_star);
+void ecor_exact_riemann_sample(double x, double t, const double* qL, const double* qR,
+                               double gamma, double x0, double* q);
+
+/* ---- grid operations (explicit_update.hpp, diagnostic_metrics_timestep.hpp) */
+void ecor_load(const ecor_grid* g, const double* U, int i, int j, double* u);
+void ecor_reconstruct(const ecor_grid* g, const double* U, int i, int j, int axis,
+                      int side, double* q);
+void ecor_face_flux(const ecor_grid* g, const double* U, int face_i, int face_j,
+                    int axis, double* f);
+int ecor_fv_step(const ecor_grid* g, const double* U, double* Unew, double dt);
+double ecor_max_rate(const ecor_grid* g, const double* U);
+double ecor_euler_dt(double max_rate, double cfl, double time, double stop_time);
+void ecor_diagnostics(const ecor_grid* g, const double* U, double* rho_min,
+                      double* p_min, int* nonfinite);
+
+typedef struct {
+  int steps;
+  double time;
+  long long degenerate_total;
+  int status; /* 0 ok/capped, 1 initial_nonphysical_state, 2 invalid_time_step,
+                 3 nonfinite_state, 4 nonpositive_density, 5 nonpositive_pressure */
+  double rho_min, p_min;
+  int nonfinite;
+  double dt_last;
+} ecor_run_result;
+
+/* The explicit branch of the driver loop (driver_main.hpp:216-333), run on
+ * U in place for at most max_step steps or until stop_time. dts (optional,
+ * length max_step) receives the dt of every step. */
+void ecor_run_explicit(const ecor_grid* g, double* U, double cfl, double stop_time,
+                       int max_step, double* dts, ecor_run_result* out);
+
+/* ---- initial data (problem_initialization.hpp, problem_state_functions.hpp) */
+void ecor_init_gresho(int nx, int ny, double xlo, double ylo, double dx, double dy,
+                      double gamma, double cx, double cy, double rho, double mach,
+                      double p0, double* U);
+void ecor_init_toro(int nx, int ny, double xlo, double dx, double gamma, const double* qL,
+                    const double* qR, double x0, double* U);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes binding of the CPU restatement in ``oracle/`` (``libec_oracle.so``,
+built by ``oracle/Makefile``).  Only ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py`` import this module, and only as the
+checker.  Fields are numpy float64 arrays of shape (4, ny, nx), comp-major,
+components (rho, mx, my, E).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libec_oracle.so"
+
+RUSANOV, HLLC, LOWMACH_HLLC, XIE_HLLCP = 0, 1, 2, 3
+MINMOD, MC, VANLEER = 0, 1, 2
+BC_PERIODIC, BC_OUTFLOW, BC_REFLECT, BC_FIXED = 0, 1, 2, 3
+
+RIEMANN_NAMES = {"rusanov": RUSANOV, "hllc": HLLC, "low_mach_hllc": LOWMACH_HLLC,
+                 "xie_am_hllc_p": XIE_HLLCP}
+
+_dp = C.POINTER(C.c_double)
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("gamma", C.c_double), ("riemann", C.c_int), ("limiter", C.c_int),
+                ("order", C.c_int), ("bc", C.c_int * 4), ("fixed", (C.c_double * 4) * 4)]
+
+
+class RunResult(C.Structure):
+    _fields_ = [("steps", C.c_int), ("time", C.c_double), ("degenerate_total", C.c_longlong),
+                ("status", C.c_int), ("rho_min", C.c_double), ("p_min", C.c_double),
+                ("nonfinite", C.c_int), ("dt_last", C.c_double)]
+
+
+class ImexResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("residual_linf", C.c_double),
+                ("relative_residual_linf", C.c_double), ("rhs_norm", C.c_double),
+                ("picard_solves", C.c_int)]
+
+
+def build() -> None:
+    """Compile libec_oracle.so with make (no-op when up to date)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        G = C.POINTER(Grid)
+        L.ecor_to_primitive.argtypes = [_dp, C.c_double, _dp]
+        L.ecor_to_conserved.argtypes = [_dp, C.c_double, _dp]
+        L.ecor_limited_slope.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.ecor_limited_slope.restype = C.c_double
+        L.ecor_physical_flux.argtypes = [_dp, C.c_int, C.c_double, _dp]
+        L.ecor_interface_flux.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_int, C.c_double, _dp]
+        L.ecor_hllc_star_degenerate.argtypes = [_dp, _dp, C.c_int, C.c_double]
+        L.ecor_hllc_star_degenerate.restype = C.c_int
+        L.ecor_pressure_ratio.argtypes = [C.c_double, C.c_double]
+        L.ecor_pressure_ratio.restype = C.c_double
+        L.ecor_exact_riemann_star.argtypes = [_dp, _dp, C.c_double, _dp, _dp]
+        L.ecor_exact_riemann_sample.argtypes = [C.c_double, C.c_double, _dp, _dp, C.c_double,
+                                                C.c_double, _dp]
+        L.ecor_fv_step.argtypes = [G, _dp, _dp, C.c_double]
+        L.ecor_fv_step.restype = C.c_int
+        L.ecor_face_flux.argtypes = [G, _dp, C.c_int, C.c_int, C.c_int, _dp]
+        L.ecor_reconstruct.argtypes = [G, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.ecor_load.argtypes = [G, _dp, C.c_int, C.c_int, _dp]
+        L.ecor_max_rate.argtypes = [G, _dp]
+        L.ecor_max_rate.restype = C.c_double
+        L.ecor_euler_dt.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
+        L.ecor_euler_dt.restype = C.c_double
+        L.ecor_diagnostics.argtypes = [G, _dp, _dp, _dp, C.POINTER(C.c_int)]
+        L.ecor_run_explicit.argtypes = [G, _dp, C.c_double, C.c_double, C.c_int, _dp,
+                                        C.POINTER(RunResult)]
+        L.ecor_init_gresho.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, _dp]
+        L.ecor_init_toro.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
+                                     _dp, C.c_double, _dp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _vec(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def make_grid(nx, ny, dx, dy, gamma=1.4, riemann=HLLC, limiter=MINMOD, order=2,
+              bc=(BC_PERIODIC,) * 4, fixed=None) -> Grid:
+    g = Grid()
+    g.nx, g.ny, g.dx, g.dy, g.gamma = nx, ny, dx, dy, gamma
+    g.riemann, g.limiter, g.order = riemann, limiter, order
+    for s in range(4):
+        g.bc[s] = bc[s]
+        if fixed is not None and fixed[s] is not None:
+            for c in range(4):
+                g.fixed[s][c] = float(fixed[s][c])
+    return g
+
+
+# ---- pointwise -------------------------------------------------------------
+def to_primitive(u, gamma=1.4):
+    u = _vec(u); q = np.zeros(4)
+    lib().ecor_to_primitive(_ptr(u), gamma, _ptr(q)); return q
+
+
+def to_conserved(q, gamma=1.4):
+    q = _vec(q); u = np.zeros(4)
+    lib().ecor_to_conserved(_ptr(q), gamma, _ptr(u)); return u
+
+
+def limited_slope(b, f, limiter=MINMOD):
+    return lib().ecor_limited_slope(float(b), float(f), int(limiter))
+
+
+def physical_flux(q, axis, gamma=1.4):
+    q = _vec(q); f = np.zeros(4)
+    lib().ecor_physical_flux(_ptr(q), axis, gamma, _ptr(f)); return f
+
+
+def interface_flux(qL, qR, axis, gamma=1.4, riemann=HLLC, sensor_f=1.0):
+    qL = _vec(qL); qR = _vec(qR); f = np.zeros(4)
+    lib().ecor_interface_flux(_ptr(qL), _ptr(qR), axis, gamma, riemann, sensor_f, _ptr(f))
+    return f
+
+
+def hllc_star_degenerate(qL, qR, axis, gamma=1.4):
+    qL = _vec(qL); qR = _vec(qR)
+    return bool(lib().ecor_hllc_star_degenerate(_ptr(qL), _ptr(qR), axis, gamma))
+
+
+def pressure_ratio(pl, pr):
+    return lib().ecor_pressure_ratio(float(pl), float(pr))
+
+
+def exact_riemann_star(qL, qR, gamma=1.4):
+    qL = _vec(qL); qR = _vec(qR); p = C.c_double(); u = C.c_double()
+    lib().ecor_exact_riemann_star(_ptr(qL), _ptr(qR), gamma, C.byref(p), C.byref(u))
+    return p.value, u.value
+
+
+def exact_riemann_sample(x, t, qL, qR, gamma=1.4, x0=0.5):
+    qL = _vec(qL); qR = _vec(qR); q = np.zeros(4)
+    lib().ecor_exact_riemann_sample(float(x), float(t), _ptr(qL), _ptr(qR), gamma, x0, _ptr(q))
+    return q
+
+
+# ---- grid ------------------------------------------------------------------
+def fv_step(g: Grid, U: np.ndarray, dt: float):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    out = np.empty_like(U)
+    deg = lib().ecor_fv_step(C.byref(g), _ptr(U), _ptr(out), float(dt))
+    return out, deg
+
+
+def face_flux(g: Grid, U, fi, fj, axis):
+    U = np.ascontiguousarray(U, dtype=np.float64); f = np.zeros(4)
+    lib().ecor_face_flux(C.byref(g), _ptr(U), fi, fj, axis, _ptr(f)); return f
+
+
+def load(g: Grid, U, i, j):
+    U = np.ascontiguousarray(U, dtype=np.float64); u = np.zeros(4)
+    lib().ecor_load(C.byref(g), _ptr(U), i, j, _ptr(u)); return u
+
+
+def max_rate(g: Grid, U):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    return lib().ecor_max_rate(C.byref(g), _ptr(U))
+
+
+def euler_dt(max_rate_, cfl, time, stop_time):
+    return lib().ecor_euler_dt(max_rate_, cfl, time, stop_time)
+
+
+def diagnostics(g: Grid, U):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    r = C.c_double(); p = C.c_double(); n = C.c_int()
+    lib().ecor_diagnostics(C.byref(g), _ptr(U), C.byref(r), C.byref(p), C.byref(n))
+    return r.value, p.value, n.value
+
+
+def run_explicit(g: Grid, U: np.ndarray, cfl: float, stop_time: float, max_step: int):
+    """Run the explicit driver loop in place on a copy; returns (U, result, dts)."""
+    U = np.array(U, dtype=np.float64, order="C", copy=True)
+    dts = np.zeros(max(max_step, 1))
+    res = RunResult()
+    lib().ecor_run_explicit(C.byref(g), _ptr(U), cfl, stop_time, max_step, _ptr(dts),
+                            C.byref(res))
+    return U, res, dts[:res.steps].copy()
+
+
+def init_gresho(nx, ny, xlo=0.0, ylo=0.0, dx=None, dy=None, gamma=1.4, cx=0.5, cy=0.5,
+                rho=1.0, mach=0.01, p0=None):
+    dx = 1.0 / nx if dx is None else dx
+    dy = 1.0 / ny if dy is None else dy
+    p0 = 1.0 / (gamma * mach * mach) if p0 is None else p0
+    U = np.zeros((4, ny, nx))
+    lib().ecor_init_gresho(nx, ny, xlo, ylo, dx, dy, gamma, cx, cy, rho, mach, p0, _ptr(U))
+    return U
+
+
+def init_toro(nx, ny, qL, qR, xlo=0.0, dx=None, gamma=1.4, x0=0.5):
+    dx = 1.0 / nx if dx is None else dx
+    qL = _vec(qL); qR = _vec(qR)
+    U = np.zeros((4, ny, nx))
+    lib().ecor_init_toro(nx, ny, xlo, dx, gamma, _ptr(qL), _ptr(qR), x0, _ptr(U))
+    return U
+
+
+def rel_l1(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Relative L1 per conserved variable: sum|a-b| / sum|b| (mean|diff|/mean|ref|)."""
+    a = a.reshape(4, -1); b = b.reshape(4, -1)
+    den = np.abs(b).sum(axis=1)
+    den = np.where(den == 0.0, 1.0, den)
+    return np.abs(a - b).sum(axis=1) / den

@@ -1,0 +1,191 @@
+"""Parity of the HIP explicit O2 path (through the C-ABI) with the oracle.
+
+Bar (north star): relative L1 per conserved variable <= 1e-12 after N steps
+against the CPU restatement on identical initial data.  Single steps are also
+checked bit for bit: the kernels keep the reference's operation order with
+IEEE division and square root and no FMA contraction.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel_l1(a, b):
+    a = np.asarray(a).reshape(4, -1)
+    b = np.asarray(b).reshape(4, -1)
+    den = np.abs(b).sum(axis=1)
+    return np.abs(a - b).sum(axis=1) / np.where(den == 0, 1, den)
+
+
+def to_dev(U, dev):
+    return torch.from_numpy(np.ascontiguousarray(U)).to(dev)
+
+
+def solver_for(ecamd, g_or, **kw):
+    names = {0: "rusanov", 1: "hllc", 2: "low_mach_hllc", 3: "xie_am_hllc_p"}
+    bcn = {0: "periodic", 1: "outflow", 2: "reflect", 3: "fixed"}
+    fixed = tuple(tuple(g_or.fixed[s]) if g_or.bc[s] == 3 else None for s in range(4))
+    grid = ecamd.Grid(g_or.nx, g_or.ny, g_or.dx, g_or.dy, g_or.gamma, names[g_or.riemann],
+                      ["minmod", "mc", "van_leer"][g_or.limiter], g_or.order,
+                      tuple(bcn[g_or.bc[s]] for s in range(4)), fixed)
+    return ecamd.ExplicitSolver(grid, **kw)
+
+
+def perturbed_state(ecor, nx, ny, seed):
+    """Smooth field plus a jump, positive rho and p (exercises every branch of
+    the limiters and wave selections)."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) + 0.5 * (X > 0.55)
+    u = 0.4 * np.cos(2 * np.pi * Y) + 0.05 * rng.standard_normal((ny, nx))
+    v = -0.3 * np.sin(2 * np.pi * X) + 0.05 * rng.standard_normal((ny, nx))
+    p = 1.0 + 0.2 * np.cos(2 * np.pi * (X + Y)) + 2.0 * (Y < 0.3)
+    U = np.empty((4, ny, nx))
+    U[0] = rho
+    U[1] = rho * u
+    U[2] = rho * v
+    U[3] = p / 0.4 + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+@pytest.fixture(scope="module")
+def ecamd():
+    import ecamd as m
+    return m
+
+
+@pytest.mark.parametrize("riemann", [0, 1, 2, 3])
+@pytest.mark.parametrize("order,limiter", [(1, 0), (2, 0), (2, 1), (2, 2)])
+def test_single_step_bitwise(ecor, ecamd, gpu, riemann, order, limiter):
+    nx, ny = 37, 29   # partial tiles in both directions
+    g = ecor.make_grid(nx, ny, 1.0 / nx, 0.7 / ny, riemann=riemann, limiter=limiter, order=order,
+                       bc=(0, 0, 1, 1))
+    U = perturbed_state(ecor, nx, ny, seed=riemann * 10 + order + limiter)
+    dt = 0.3 * 0.8 / ecor.max_rate(g, U)
+    ref, deg = ecor.fv_step(g, U, dt)
+    s = solver_for(ecamd, g)
+    Ud = to_dev(U, gpu)
+    out = torch.empty_like(Ud)
+    d = s.step(Ud, out, dt)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (np.abs(got - ref).max(), rel_l1(got, ref))
+    assert d.degenerate_count == deg
+    r, p, n = ecor.diagnostics(g, ref)
+    assert (d.rho_min, d.pressure_min, d.nonfinite_count) == (r, p, n)
+    assert d.max_rate == ecor.max_rate(g, ref)
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2])
+def test_tile_shapes_agree(ecor, ecamd, gpu, tile):
+    nx, ny = 70, 45
+    g = ecor.make_grid(nx, ny, 1.0 / nx, 1.0 / ny, riemann=3)
+    U = perturbed_state(ecor, nx, ny, seed=7)
+    dt = 0.2 / ecor.max_rate(g, U)
+    ref, _ = ecor.fv_step(g, U, dt)
+    s = solver_for(ecamd, g)
+    if tile >= len(s.tile_shapes()):
+        pytest.skip("tile shape not compiled")
+    s.set_tile(tile)
+    Ud = to_dev(U, gpu)
+    out = torch.empty_like(Ud)
+    s.step(Ud, out, dt, diag=False)
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_sod_full_run_device_loop(ecor, ecamd, gpu):
+    """Sod 400x4 HLLC O2, CFL 0.8, t=0.2 -> 347 steps."""
+    nx = 400
+    g = ecor.make_grid(nx, 4, 1.0 / nx, 0.01 / 4, riemann=1, bc=(1, 1, 0, 0))
+    U = ecor.init_toro(nx, 4, [1, 0, 0, 1], [0.125, 0, 0, 0.1])
+    ref, res, _ = ecor.run_explicit(g, U, 0.8, 0.2, 100000)
+    s = solver_for(ecamd, g)
+    got, run = ecamd.run_explicit(s, to_dev(U, gpu), 0.8, 0.2, 100000, chunk=100)
+    assert run.step == res.steps == 347
+    assert run.status == 0 and run.time == res.time
+    assert run.degenerate_total == res.degenerate_total
+    assert np.array_equal(got.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("riemann", [1, 3])
+def test_gresho_device_loop_parity(ecor, ecamd, gpu, riemann):
+    """BASELINE config 2 at 64^2, M=0.01, CFL 0.4, N=100 steps."""
+    n = 64
+    g = ecor.make_grid(n, n, 1.0 / n, 1.0 / n, riemann=riemann)
+    U = ecor.init_gresho(n, n, mach=0.01)
+    ref, res, dts = ecor.run_explicit(g, U, 0.4, 1.0, 100)
+    s = solver_for(ecamd, g)
+    got, run = ecamd.run_explicit(s, to_dev(U, gpu), 0.4, 1.0, 100, chunk=37)
+    assert run.step == 100 and run.status == 0
+    assert run.time == res.time
+    assert np.all(rel_l1(got.cpu().numpy(), ref) <= TOL)
+    assert np.array_equal(got.cpu().numpy(), ref)
+
+
+def test_shock_bubble_boundaries(ecor, ecamd, gpu):
+    """Fixed inflow left, outflow right/top, reflecting bottom (the reference's
+    shock-bubble ghost rule), 30 steps through the device loop."""
+    nx, ny = 48, 20
+    post = ecor.to_conserved([2.8181818181818183, 1.6064386578049976, 0.0, 5.0])
+    g = ecor.make_grid(nx, ny, 2.0 / nx, 0.5 / ny, riemann=1, bc=(3, 1, 2, 1),
+                       fixed=(post, None, None, None))
+    U = perturbed_state(ecor, nx, ny, seed=3)
+    ref, res, _ = ecor.run_explicit(g, U, 0.4, 1.0, 30)
+    s = solver_for(ecamd, g)
+    got, run = ecamd.run_explicit(s, to_dev(U, gpu), 0.4, 1.0, 30)
+    assert run.step == res.steps
+    assert np.array_equal(got.cpu().numpy(), ref)
+
+
+def test_inplace_and_strided_layouts(ecor, ecamd, gpu):
+    """The in-place entry and an AMReX-style layout with 2 ghost cells give the
+    out-of-place result."""
+    import ctypes as C
+    from ecamd import _lib
+    nx, ny = 33, 18
+    g = ecor.make_grid(nx, ny, 1.0 / nx, 1.0 / ny, riemann=2)
+    U = perturbed_state(ecor, nx, ny, seed=11)
+    dt = 0.1 / ecor.max_rate(g, U)
+    ref, _ = ecor.fv_step(g, U, dt)
+    s = solver_for(ecamd, g)
+    Ud = to_dev(U, gpu)
+    s.step_inplace(Ud, dt)
+    assert np.array_equal(Ud.cpu().numpy(), ref)
+    fab = torch.full((4, ny + 4, nx + 4), float("nan"), dtype=torch.float64, device=gpu)
+    fab[:, 2:-2, 2:-2] = to_dev(U, gpu)
+    out = torch.full_like(fab, float("nan"))
+    lay = _lib.Layout(nx + 4, (nx + 4) * (ny + 4))
+    off = (2 * (nx + 4) + 2) * 8
+    st = _lib.lib().ec_explicit_step(s.ctx, C.c_void_p(fab.data_ptr() + off),
+                                     C.c_void_p(out.data_ptr() + off), C.byref(lay), dt, None)
+    assert st == 0
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    assert np.array_equal(o[:, 2:-2, 2:-2], ref)
+    assert np.isnan(o[:, :2, :]).all() and np.isnan(o[:, :, -2:]).all()  # ghosts untouched
+
+
+def test_run_status_paths(ecor, ecamd, gpu):
+    n = 16
+    g = ecor.make_grid(n, n, 1.0 / n, 1.0 / n, riemann=1)
+    U = ecor.init_gresho(n, n, mach=0.1)
+    s = solver_for(ecamd, g)
+    # capped at max_step before stop_time
+    got, run = ecamd.run_explicit(s, to_dev(U, gpu), 0.4, 1.0, 5)
+    assert run.step == 5 and run.status == 0 and run.time < 1.0
+    # initial non-physical state: negative pressure in one cell
+    bad = U.copy()
+    bad[3, 3, 3] = 0.0
+    got, run = ecamd.run_explicit(s, to_dev(bad, gpu), 0.4, 1.0, 5)
+    assert run.status == 1 and run.step == 0
+    assert np.array_equal(got.cpu().numpy(), bad)
+    # standalone diagnostics agree with the oracle
+    d = s.diagnostics(to_dev(bad, gpu))
+    r, p, nf = ecor.diagnostics(g, bad)
+    assert (d.rho_min, d.pressure_min, d.nonfinite_count) == (r, p, nf)
+    assert d.max_rate == ecor.max_rate(g, bad)
