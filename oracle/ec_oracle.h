@@ -83,6 +83,50 @@ void ecor_init_gresho(int nx, int ny, double xlo, double ylo, double dx, double 
 void ecor_init_toro(int nx, int ny, double xlo, double dx, double gamma, const double* qL,
                     const double* qR, double x0, double* U);
 
+/* ---- IMEX T1/S2 BDLTV20 source-map Picard step (ec_oracle_imex.c) */
+enum {
+  ECOR_IMEX_CONVERGED_ABSOLUTE = 0,
+  ECOR_IMEX_CONVERGED_RELATIVE = 1,
+  ECOR_IMEX_NOT_CONVERGED = 2,      /* linear_solve_not_converged */
+  ECOR_IMEX_BREAKDOWN = 3,          /* gmres_breakdown_not_converged */
+  ECOR_IMEX_DENSITY_FLOOR = 4,      /* density_denominator_floor_used */
+  ECOR_IMEX_ROW_NONFINITE = 5,      /* source_map_pressure_row_nonfinite */
+  ECOR_IMEX_RESIDUAL_NONFINITE = 6, /* linear_residual_nonfinite */
+  ECOR_IMEX_MOMENTUM_NONFINITE = 7  /* source_map_lagged_momentum_nonfinite */
+};
+
+typedef struct {
+  int iterations, converged, breakdown, nonfinite;
+  double residual_linf, relative_residual_linf;
+} ecor_gmres_result;
+
+typedef struct {
+  int status;
+  int iterations;        /* summed over Picard solves */
+  int last_iterations;
+  int picard_solves;
+  int density_floor_count;
+  int guard_count;
+  int nonfinite_count;
+  double residual_linf, relative_residual_linf, rhs_norm;
+  double coupled_residual_linf;
+} ecor_imex_result;
+
+void ecor_imex_star(const ecor_grid* g, const double* U, double dt, int acoustic, double* S);
+long ecor_imex_assemble(const ecor_grid* g, const double* star, const double* p,
+                        const double* mom, double dt, long* row_ptr, long* cols, double* vals,
+                        double* rhs);
+void ecor_gmres_full(long n, const long* rp, const long* cols, const double* vals,
+                     const double* b, const double* x0, int max_iterations, double tol,
+                     double* x, ecor_gmres_result* res);
+/* One IMEX step of U in place (on success); p_out (optional) receives the
+ * final pressure. */
+void ecor_imex_step(const ecor_grid* g, double* U, double dt, int picard, double tol, int max_iter,
+                    double* p_out, ecor_imex_result* out);
+double ecor_imex_dt(const ecor_grid* g, const double* U, double cfl, double time, double stop,
+                    int acoustic_startup, double acoustic_cap, double* material_max,
+                    double* acoustic_max);
+
 #ifdef __cplusplus
 }
 #endif

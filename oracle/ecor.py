@@ -40,10 +40,24 @@ class RunResult(C.Structure):
                 ("nonfinite", C.c_int), ("dt_last", C.c_double)]
 
 
+class GmresResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("breakdown", C.c_int),
+                ("nonfinite", C.c_int), ("residual_linf", C.c_double),
+                ("relative_residual_linf", C.c_double)]
+
+
 class ImexResult(C.Structure):
-    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("residual_linf", C.c_double),
-                ("relative_residual_linf", C.c_double), ("rhs_norm", C.c_double),
-                ("picard_solves", C.c_int)]
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("last_iterations", C.c_int),
+                ("picard_solves", C.c_int), ("density_floor_count", C.c_int),
+                ("guard_count", C.c_int), ("nonfinite_count", C.c_int),
+                ("residual_linf", C.c_double), ("relative_residual_linf", C.c_double),
+                ("rhs_norm", C.c_double), ("coupled_residual_linf", C.c_double)]
+
+
+IMEX_STATUS = {0: "converged_absolute", 1: "converged_relative", 2: "linear_solve_not_converged",
+               3: "gmres_breakdown_not_converged", 4: "density_denominator_floor_used",
+               5: "source_map_pressure_row_nonfinite", 6: "linear_residual_nonfinite",
+               7: "source_map_lagged_momentum_nonfinite"}
 
 
 def build() -> None:
@@ -91,6 +105,17 @@ def lib():
                                        C.c_double, C.c_double, C.c_double, _dp]
         L.ecor_init_toro.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
                                      _dp, C.c_double, _dp]
+        _lp = C.POINTER(C.c_long)
+        L.ecor_imex_star.argtypes = [G, _dp, C.c_double, C.c_int, _dp]
+        L.ecor_imex_assemble.argtypes = [G, _dp, _dp, _dp, C.c_double, _lp, _lp, _dp, _dp]
+        L.ecor_imex_assemble.restype = C.c_long
+        L.ecor_gmres_full.argtypes = [C.c_long, _lp, _lp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                      _dp, C.POINTER(GmresResult)]
+        L.ecor_imex_step.argtypes = [G, _dp, C.c_double, C.c_int, C.c_double, C.c_int, _dp,
+                                     C.POINTER(ImexResult)]
+        L.ecor_imex_dt.argtypes = [G, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                   C.c_double, _dp, _dp]
+        L.ecor_imex_dt.restype = C.c_double
         _lib = L
     return _lib
 
@@ -232,3 +257,53 @@ def rel_l1(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     den = np.abs(b).sum(axis=1)
     den = np.where(den == 0.0, 1.0, den)
     return np.abs(a - b).sum(axis=1) / den
+
+
+# ---- IMEX ------------------------------------------------------------------
+def imex_star(g: Grid, U, dt, acoustic=False):
+    U = np.ascontiguousarray(U, dtype=np.float64); S = np.empty_like(U)
+    lib().ecor_imex_star(C.byref(g), _ptr(U), float(dt), int(acoustic), _ptr(S)); return S
+
+
+def imex_assemble(g: Grid, star, p, mom, dt):
+    """CSR (indptr, indices, data) of the pressure rows and the right-hand side."""
+    n = g.nx * g.ny
+    star = np.ascontiguousarray(star, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+    mom = np.ascontiguousarray(mom, dtype=np.float64)
+    rp = np.zeros(n + 1, dtype=np.int64); cols = np.zeros(11 * n, dtype=np.int64)
+    vals = np.zeros(11 * n); rhs = np.zeros(n)
+    lp = C.POINTER(C.c_long)
+    nnz = lib().ecor_imex_assemble(C.byref(g), _ptr(star), _ptr(p), _ptr(mom), float(dt),
+                                   rp.ctypes.data_as(lp), cols.ctypes.data_as(lp), _ptr(vals),
+                                   _ptr(rhs))
+    return rp, cols[:nnz].copy(), vals[:nnz].copy(), rhs
+
+
+def gmres_full(indptr, indices, data, b, x0, max_iter, tol):
+    n = len(b)
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(indices, dtype=np.int64)
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    b = _vec(b); x0 = _vec(x0); x = np.zeros(n); r = GmresResult()
+    lp = C.POINTER(C.c_long)
+    lib().ecor_gmres_full(n, indptr.ctypes.data_as(lp), indices.ctypes.data_as(lp), _ptr(data),
+                          _ptr(b), _ptr(x0), int(max_iter), float(tol), _ptr(x), C.byref(r))
+    return x, r
+
+
+def imex_step(g: Grid, U, dt, picard=2, tol=1e-8, max_iter=1000):
+    """One IMEX step; returns (U_new, p, result).  U_new is U when the step fails."""
+    U = np.array(U, dtype=np.float64, order="C", copy=True)
+    p = np.zeros(g.nx * g.ny); r = ImexResult()
+    lib().ecor_imex_step(C.byref(g), _ptr(U), float(dt), int(picard), float(tol), int(max_iter),
+                         _ptr(p), C.byref(r))
+    return U, p.reshape(g.ny, g.nx), r
+
+
+def imex_dt(g: Grid, U, cfl, time, stop, acoustic_startup=0, acoustic_cap=0.0):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    m = C.c_double(); a = C.c_double()
+    dt = lib().ecor_imex_dt(C.byref(g), _ptr(U), cfl, time, stop, acoustic_startup,
+                            acoustic_cap, C.byref(m), C.byref(a))
+    return dt, m.value, a.value
