@@ -1,0 +1,609 @@
+// ec_adjoint.hip -- adjoint of the explicit O2 step (device) and host-callable
+// pointwise flux / VJP entry points used to verify the hand-written reverse
+// mode against finite differences.
+#include <cstring>
+
+#include "ec_adjoint.hpp"
+#include "ec_context.hpp"
+#include "ec_ghost.hpp"
+
+using namespace ec;
+
+extern "C" ec_status ec_flux_host(const double* qL, const double* qR, int axis, double gamma,
+                                  int riemann, double sensor_f, double* F)
+{
+  if (!qL || !qR || !F || axis < 0 || axis > 1 || riemann < 0 || riemann > 3) {
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  const Prim L{qL[0], qL[1], qL[2], qL[3]}, R{qR[0], qR[1], qR[2], qR[3]};
+  const Cons f = interface_flux(L, R, axis, gamma, riemann, sensor_f);
+  F[0] = f.r;
+  F[1] = f.mx;
+  F[2] = f.my;
+  F[3] = f.E;
+  return EC_OK;
+}
+
+extern "C" ec_status ec_flux_vjp_host(const double* qL, const double* qR, int axis, double gamma,
+                                      int riemann, double sensor_f, const double* Fbar,
+                                      double* qLbar, double* qRbar, double* sensor_bar)
+{
+  if (!qL || !qR || !Fbar || !qLbar || !qRbar || axis < 0 || axis > 1 || riemann < 0 ||
+      riemann > 3) {
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  const Prim L{qL[0], qL[1], qL[2], qL[3]}, R{qR[0], qR[1], qR[2], qR[3]};
+  Prim Lb{0.0, 0.0, 0.0, 0.0}, Rb{0.0, 0.0, 0.0, 0.0};
+  double sb = 0.0;
+  interface_flux_vjp(L, R, axis, gamma, riemann, sensor_f, Cons{Fbar[0], Fbar[1], Fbar[2], Fbar[3]},
+                     Lb, Rb, sb);
+  qLbar[0] = Lb.r; qLbar[1] = Lb.u; qLbar[2] = Lb.v; qLbar[3] = Lb.p;
+  qRbar[0] = Rb.r; qRbar[1] = Rb.u; qRbar[2] = Rb.v; qRbar[3] = Rb.p;
+  if (sensor_bar) *sensor_bar = sb;
+  return EC_OK;
+}
+
+// ===================================================================== step
+// Adjoint of one explicit step U' = U - dt div F(U) with dt and every branch
+// frozen at U:  lambda = lambda' + (d(-dt div F)/dU)^T lambda'.
+//   pass 1 (faces)   Fbar = dt/dh (lambda'_R - lambda'_L) (lambda' = 0 off the
+//                    valid box), VJP through the flux into the bars of the two
+//                    face states; for HLLC-P also the bar of the selected
+//                    sensor ratio
+//   pass 2 (ratios)  HLLC-P: each face ratio gathers its bar from the <= 5
+//                    sensors that selected it and passes it to the pressure of
+//                    its two face states
+//   pass 3 (cells)   on the grown box [-2,n+1]^2: face-state bars -> slope bars
+//                    -> primitive bars of the cell and its neighbours ->
+//                    conserved bars (to_prim VJP)
+//   pass 4 (fold)    each valid cell gathers the conserved bars of every
+//                    grown cell whose ghost rule reads it (wrap, clamp, mirror
+//                    with the momentum sign flipped; fixed states read none)
+// Each pass writes every output exactly once: no atomics, deterministic.
+namespace ec {
+
+struct AdjWork {
+  int nx = 0, ny = 0;
+  double* states = nullptr;  // segment of forward states (compact), seg_cap of them
+  int seg_cap = 0;
+  double* lam = nullptr;     // two compact adjoint buffers
+  double* dtlog = nullptr;   // device dt record
+  int dt_cap = 0;
+  double* qb = nullptr;    // 4 sides x 4 comps x GX*GY face-state bars
+  double* ub = nullptr;    // 4 comps x GX*GY conserved bars of grown cells
+  int* sel = nullptr;      // HLLC-P: sensor argmin per flux face
+  double* mbar = nullptr;  // HLLC-P: bar of the selected ratio per flux face
+};
+
+void adjoint_release(ec_ctx* ctx)
+{
+  AdjWork* w = static_cast<AdjWork*>(ctx->adjoint);
+  if (!w) return;
+  if (w->qb) (void)hipFree(w->qb);
+  if (w->ub) (void)hipFree(w->ub);
+  if (w->sel) (void)hipFree(w->sel);
+  if (w->mbar) (void)hipFree(w->mbar);
+  if (w->states) (void)hipFree(w->states);
+  if (w->lam) (void)hipFree(w->lam);
+  if (w->dtlog) (void)hipFree(w->dtlog);
+  delete w;
+  ctx->adjoint = nullptr;
+}
+
+static ec_status adjoint_workspace(ec_ctx* ctx, AdjWork** out)
+{
+  AdjWork* w = static_cast<AdjWork*>(ctx->adjoint);
+  if (!w) {
+    w = new AdjWork();
+    ctx->adjoint = w;
+    w->nx = ctx->grid.nx;
+    w->ny = ctx->grid.ny;
+    const size_t G = (size_t)(w->nx + 4) * (w->ny + 4);
+    const size_t nf = (size_t)(w->nx + 1) * w->ny + (size_t)w->nx * (w->ny + 1);
+    bool ok = hipMalloc((void**)&w->qb, 16 * G * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&w->ub, 4 * G * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&w->sel, nf * sizeof(int)) == hipSuccess &&
+              hipMalloc((void**)&w->mbar, nf * sizeof(double)) == hipSuccess;
+    if (!ok) {
+      adjoint_release(ctx);
+      return EC_ERR_OUT_OF_MEMORY;
+    }
+  }
+  *out = w;
+  return EC_OK;
+}
+
+struct AdjArgs {
+  const double* __restrict__ U;     // forward state (caller layout)
+  const double* __restrict__ lam1;  // lambda' (caller layout)
+  double* __restrict__ lam0;        // lambda (caller layout)
+  long long rs, cs;
+  GridParams g;
+  double dt;
+  double* qb;
+  double* ub;
+  int* sel;
+  double* mbar;
+};
+
+// Face-state bar storage: side 0 E (x,+), 1 W (x,-), 2 N (y,+), 3 S (y,-).
+__device__ __forceinline__ long long gidx(const GridParams& g, int i, int j)
+{
+  return (long long)(j + 2) * (g.nx + 4) + (i + 2);
+}
+
+__device__ __forceinline__ double* qb_at(const AdjArgs& A, int side, int i, int j)
+{
+  const long long G = (long long)(A.g.nx + 4) * (A.g.ny + 4);
+  return A.qb + (long long)side * 4 * G + gidx(A.g, i, j);
+}
+
+__device__ __forceinline__ Prim prim_at(const AdjArgs& A, int i, int j)
+{
+  return to_prim(load_cell(A.U, A.rs, A.cs, A.g, i, j), A.g.gamma);
+}
+
+__device__ __forceinline__ Prim slope_of(const AdjArgs& A, int i, int j, int axis)
+{
+  const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
+  const Prim m = prim_at(A, i - di, j - dj), c = prim_at(A, i, j), p = prim_at(A, i + di, j + dj);
+  const int lim = A.g.limiter;
+  return Prim{limited_slope(c.r - m.r, p.r - c.r, lim), limited_slope(c.u - m.u, p.u - c.u, lim),
+              limited_slope(c.v - m.v, p.v - c.v, lim), limited_slope(c.p - m.p, p.p - c.p, lim)};
+}
+
+__device__ __forceinline__ Prim face_state_at(const AdjArgs& A, int i, int j, int axis, double hs)
+{
+  Prim q = prim_at(A, i, j);
+  if (A.g.order == 2) {
+    const Prim s = slope_of(A, i, j, axis);
+    q.r = q.r + hs * s.r;
+    q.u = q.u + hs * s.u;
+    q.v = q.v + hs * s.v;
+    q.p = q.p + hs * s.p;
+  }
+  return q;
+}
+
+__device__ __forceinline__ Cons lam_at(const AdjArgs& A, int i, int j)
+{
+  if (i < 0 || j < 0 || i >= A.g.nx || j >= A.g.ny) return Cons{0.0, 0.0, 0.0, 0.0};
+  const long long k = (long long)j * A.rs + i;
+  return Cons{A.lam1[k], A.lam1[A.cs + k], A.lam1[2 * A.cs + k], A.lam1[3 * A.cs + k]};
+}
+
+__device__ __forceinline__ double ratio_of(const AdjArgs& A, int fi, int fj, int axis)
+{
+  const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
+  return pressure_ratio(face_state_at(A, fi - di, fj - dj, axis, 0.5).p,
+                        face_state_at(A, fi, fj, axis, -0.5).p);
+}
+
+__global__ void adj_faces_kernel(AdjArgs A)
+{
+  const GridParams& g = A.g;
+  const long long nxf = (long long)(g.nx + 1) * g.ny, nyf = (long long)g.nx * (g.ny + 1);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nxf + nyf;
+       t += (long long)gridDim.x * blockDim.x) {
+    int axis, fi, fj;
+    if (t < nxf) {
+      axis = 0;
+      fi = (int)(t % (g.nx + 1));
+      fj = (int)(t / (g.nx + 1));
+    } else {
+      axis = 1;
+      fi = (int)((t - nxf) % g.nx);
+      fj = (int)((t - nxf) / g.nx);
+    }
+    const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
+    const int li = fi - di, lj = fj - dj;
+    const double s = (axis == 0 ? g.inv_dx : g.inv_dy) * A.dt;
+    const Cons lr = lam_at(A, fi, fj), ll = lam_at(A, li, lj);
+    const Cons Fb{s * (lr.r - ll.r), s * (lr.mx - ll.mx), s * (lr.my - ll.my), s * (lr.E - ll.E)};
+    Prim Lb{0.0, 0.0, 0.0, 0.0}, Rb{0.0, 0.0, 0.0, 0.0};
+    int sel = -1;
+    double mb = 0.0;
+    if (Fb.r != 0.0 || Fb.mx != 0.0 || Fb.my != 0.0 || Fb.E != 0.0) {
+      const Prim L = face_state_at(A, li, lj, axis, 0.5);
+      const Prim R = face_state_at(A, fi, fj, axis, -0.5);
+      double sensor = 1.0, m = 1.0;
+      if (g.riemann == 3) {
+        // reconstruction_fluxes.hpp:421-440, the first minimal ratio wins
+        double r[5];
+        if (axis == 0) {
+          r[0] = ratio_of(A, fi, fj, 0);
+          r[1] = ratio_of(A, fi - 1, fj, 1);
+          r[2] = ratio_of(A, fi - 1, fj + 1, 1);
+          r[3] = ratio_of(A, fi, fj, 1);
+          r[4] = ratio_of(A, fi, fj + 1, 1);
+        } else {
+          r[0] = ratio_of(A, fi, fj, 1);
+          r[1] = ratio_of(A, fi, fj - 1, 0);
+          r[2] = ratio_of(A, fi + 1, fj - 1, 0);
+          r[3] = ratio_of(A, fi, fj, 0);
+          r[4] = ratio_of(A, fi + 1, fj, 0);
+        }
+        m = r[0];
+        sel = 0;
+        for (int k = 1; k < 5; ++k) {
+          if (r[k] < m) {
+            m = r[k];
+            sel = k;
+          }
+        }
+        const double c = clamp01(m);
+        sensor = c * c * c;
+      }
+      double sb = 0.0;
+      interface_flux_vjp(L, R, axis, g.gamma, g.riemann, sensor, Fb, Lb, Rb, sb);
+      if (g.riemann == 3) {
+        const double c = clamp01(m);
+        mb = clamp01_vjp(m, 3.0 * c * c * sb);
+      }
+    }
+    double* qL = qb_at(A, axis == 0 ? 0 : 2, li, lj);
+    double* qR = qb_at(A, axis == 0 ? 1 : 3, fi, fj);
+    const long long G = (long long)(g.nx + 4) * (g.ny + 4);
+    qL[0] = Lb.r; qL[G] = Lb.u; qL[2 * G] = Lb.v; qL[3 * G] = Lb.p;
+    qR[0] = Rb.r; qR[G] = Rb.u; qR[2 * G] = Rb.v; qR[3 * G] = Rb.p;
+    if (A.sel) {
+      A.sel[t] = sel;
+      A.mbar[t] = mb;
+    }
+  }
+}
+
+// HLLC-P: gather each face ratio's bar and pass it to its face pressures.
+__global__ void adj_ratios_kernel(AdjArgs A)
+{
+  const GridParams& g = A.g;
+  const long long nxf = (long long)(g.nx + 1) * g.ny;
+  const long long nxr = (long long)(g.nx + 1) * (g.ny + 2), nyr = (long long)(g.nx + 2) * (g.ny + 1);
+  auto xf = [&](int fi, int j, int want) -> double {  // x flux face (fi, j)
+    if (fi < 0 || fi > g.nx || j < 0 || j >= g.ny) return 0.0;
+    const long long t = (long long)j * (g.nx + 1) + fi;
+    return A.sel[t] == want ? A.mbar[t] : 0.0;
+  };
+  auto yf = [&](int i, int fj, int want) -> double {  // y flux face (i, fj)
+    if (i < 0 || i >= g.nx || fj < 0 || fj > g.ny) return 0.0;
+    const long long t = nxf + (long long)fj * g.nx + i;
+    return A.sel[t] == want ? A.mbar[t] : 0.0;
+  };
+  const long long G = (long long)(g.nx + 4) * (g.ny + 4);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nxr + nyr;
+       t += (long long)gridDim.x * blockDim.x) {
+    int axis, fi, fj;
+    double bar;
+    if (t < nxr) {
+      axis = 0;
+      fi = (int)(t % (g.nx + 1));
+      fj = (int)(t / (g.nx + 1)) - 1;
+      bar = xf(fi, fj, 0) + yf(fi, fj + 1, 1) + yf(fi - 1, fj + 1, 2) + yf(fi, fj, 3) +
+            yf(fi - 1, fj, 4);
+    } else {
+      axis = 1;
+      const long long s = t - nxr;
+      fi = (int)(s % (g.nx + 2)) - 1;
+      fj = (int)(s / (g.nx + 2));
+      bar = yf(fi, fj, 0) + xf(fi + 1, fj, 1) + xf(fi + 1, fj - 1, 2) + xf(fi, fj, 3) +
+            xf(fi, fj - 1, 4);
+    }
+    if (bar == 0.0) continue;
+    const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
+    const double pl = face_state_at(A, fi - di, fj - dj, axis, 0.5).p;
+    const double pr = face_state_at(A, fi, fj, axis, -0.5).p;
+    double dl, dr;
+    pressure_ratio_grad(pl, pr, dl, dr);
+    qb_at(A, axis == 0 ? 0 : 2, fi - di, fj - dj)[3 * G] += bar * dl;
+    qb_at(A, axis == 0 ? 1 : 3, fi, fj)[3 * G] += bar * dr;
+  }
+}
+
+__device__ __forceinline__ Prim qb_load(const AdjArgs& A, int side, int i, int j)
+{
+  if (i < -2 || j < -2 || i > A.g.nx + 1 || j > A.g.ny + 1) return Prim{0.0, 0.0, 0.0, 0.0};
+  const long long G = (long long)(A.g.nx + 4) * (A.g.ny + 4);
+  const double* p = qb_at(A, side, i, j);
+  return Prim{p[0], p[G], p[2 * G], p[3 * G]};
+}
+
+// Bar of the slope of cell (i,j) along an axis: 0.5 (bar_plus_state - bar_minus_state).
+__device__ __forceinline__ Prim slope_bar(const AdjArgs& A, int i, int j, int axis)
+{
+  const Prim a = qb_load(A, axis == 0 ? 0 : 2, i, j), b = qb_load(A, axis == 0 ? 1 : 3, i, j);
+  return Prim{0.5 * (a.r - b.r), 0.5 * (a.u - b.u), 0.5 * (a.v - b.v), 0.5 * (a.p - b.p)};
+}
+
+__global__ void adj_cells_kernel(AdjArgs A)
+{
+  const GridParams& g = A.g;
+  const int GX = g.nx + 4;
+  const long long G = (long long)GX * (g.ny + 4);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < G;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % GX) - 2, j = (int)(t / GX) - 2;
+    Prim qbar{0.0, 0.0, 0.0, 0.0};
+    for (int side = 0; side < 4; ++side) add(qbar, qb_load(A, side, i, j));
+    if (g.order == 2) {
+      for (int axis = 0; axis < 2; ++axis) {
+        const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
+        // slopes of (i,j) and of its two neighbours on this axis read q(i,j)
+        for (int o = -1; o <= 1; ++o) {
+          const int ci = i + o * di, cj = j + o * dj;
+          if (ci < -1 || cj < -1 || ci > g.nx || cj > g.ny) continue;  // no face states there
+          const Prim sb = slope_bar(A, ci, cj, axis);
+          if (sb.r == 0.0 && sb.u == 0.0 && sb.v == 0.0 && sb.p == 0.0) continue;
+          const Prim qm = prim_at(A, ci - di, cj - dj), qc = prim_at(A, ci, cj),
+                     qp = prim_at(A, ci + di, cj + dj);
+          const double bk[4] = {qc.r - qm.r, qc.u - qm.u, qc.v - qm.v, qc.p - qm.p};
+          const double fw[4] = {qp.r - qc.r, qp.u - qc.u, qp.v - qc.v, qp.p - qc.p};
+          const double sbv[4] = {sb.r, sb.u, sb.v, sb.p};
+          double contrib[4];
+          for (int c = 0; c < 4; ++c) {
+            double db, df;
+            limited_slope_grad(bk[c], fw[c], g.limiter, db, df);
+            // slope(ci) = lim(q_ci - q_{ci-1}, q_{ci+1} - q_ci)
+            double d = 0.0;
+            if (o == 0) d = db - df;       // (i,j) is the centre
+            else if (o == 1) d = -db;      // (i,j) = centre - 1
+            else d = df;                   // (i,j) = centre + 1
+            contrib[c] = sbv[c] * d;
+          }
+          qbar.r += contrib[0];
+          qbar.u += contrib[1];
+          qbar.v += contrib[2];
+          qbar.p += contrib[3];
+        }
+      }
+    }
+    Cons ubar{0.0, 0.0, 0.0, 0.0};
+    if (qbar.r != 0.0 || qbar.u != 0.0 || qbar.v != 0.0 || qbar.p != 0.0) {
+      to_prim_vjp(load_cell(A.U, A.rs, A.cs, g, i, j), g.gamma, qbar, ubar);
+    }
+    A.ub[t] = ubar.r;
+    A.ub[G + t] = ubar.mx;
+    A.ub[2 * G + t] = ubar.my;
+    A.ub[3 * G + t] = ubar.E;
+  }
+}
+
+// Where the explicit ghost rule reads grown cell (i,j) from: valid index and
+// momentum flips, or false for a fixed state.
+__device__ __forceinline__ bool ghost_source(const GridParams& g, int i, int j, int& ii, int& jj,
+                                             bool& rx, bool& ry)
+{
+  if (i < 0 && g.bc[0] == BC_FIXED) return false;
+  if (i >= g.nx && g.bc[1] == BC_FIXED) return false;
+  if (j < 0 && g.bc[2] == BC_FIXED) return false;
+  if (j >= g.ny && g.bc[3] == BC_FIXED) return false;
+  ii = map_axis(i, g.nx, g.bc[0], g.bc[1], rx);
+  jj = map_axis(j, g.ny, g.bc[2], g.bc[3], ry);
+  return true;
+}
+
+__global__ void adj_fold_kernel(AdjArgs A)
+{
+  const GridParams& g = A.g;
+  const int GX = g.nx + 4;
+  const long long G = (long long)GX * (g.ny + 4);
+  const long long n = (long long)g.nx * g.ny;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+    const int cx[5] = {i, -2, -1, g.nx, g.nx + 1};
+    const int cy[5] = {j, -2, -1, g.ny, g.ny + 1};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < 5; ++b) {
+      for (int a = 0; a < 5; ++a) {
+        const int gi = cx[a], gj = cy[b];
+        if ((a > 0 && gi == i) || (b > 0 && gj == j)) continue;  // duplicate candidate
+        int ii, jj;
+        bool rx, ry;
+        if (!ghost_source(g, gi, gj, ii, jj, rx, ry) || ii != i || jj != j) continue;
+        const long long k = gidx(g, gi, gj);
+        acc[0] += A.ub[k];
+        acc[1] += rx ? -A.ub[G + k] : A.ub[G + k];
+        acc[2] += ry ? -A.ub[2 * G + k] : A.ub[2 * G + k];
+        acc[3] += A.ub[3 * G + k];
+      }
+    }
+    const long long k = (long long)j * A.rs + i;
+    for (int c = 0; c < 4; ++c) A.lam0[c * A.cs + k] = A.lam1[c * A.cs + k] + acc[c];
+  }
+}
+
+static int adj_blocks(long long n)
+{
+  long long b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  return b < 1 ? 1 : (int)b;
+}
+
+ec_status launch_adjoint_step(ec_ctx* ctx, const double* U, const double* lam1, double* lam0,
+                              long long rs, long long cs, double dt)
+{
+  AdjWork* w = nullptr;
+  ec_status st = adjoint_workspace(ctx, &w);
+  if (st != EC_OK) return st;
+  AdjArgs A{U, lam1, lam0, rs, cs, ctx->grid, dt, w->qb, w->ub,
+            ctx->grid.riemann == 3 ? w->sel : nullptr, w->mbar};
+  const GridParams& g = ctx->grid;
+  const long long G = (long long)(g.nx + 4) * (g.ny + 4);
+  EC_HIP_TRY(ctx, hipMemsetAsync(w->qb, 0, 16 * G * sizeof(double), ctx->stream));
+  const long long nf = (long long)(g.nx + 1) * g.ny + (long long)g.nx * (g.ny + 1);
+  hipLaunchKernelGGL(adj_faces_kernel, dim3(adj_blocks(nf)), dim3(256), 0, ctx->stream, A);
+  if (g.riemann == 3) {
+    const long long nr = (long long)(g.nx + 1) * (g.ny + 2) + (long long)(g.nx + 2) * (g.ny + 1);
+    hipLaunchKernelGGL(adj_ratios_kernel, dim3(adj_blocks(nr)), dim3(256), 0, ctx->stream, A);
+  }
+  hipLaunchKernelGGL(adj_cells_kernel, dim3(adj_blocks(G)), dim3(256), 0, ctx->stream, A);
+  hipLaunchKernelGGL(adj_fold_kernel, dim3(adj_blocks((long long)g.nx * g.ny)), dim3(256), 0,
+                     ctx->stream, A);
+  EC_HIP_TRY(ctx, hipGetLastError());
+  return EC_OK;
+}
+
+}  // namespace ec
+
+extern "C" ec_status ec_adjoint_step(ec_ctx* ctx, const double* U_n, const double* lambda_next,
+                                     double* lambda_n, const ec_layout* lay, double dt)
+{
+  if (!ctx || !U_n || !lambda_next || !lambda_n || !lay || lambda_n == lambda_next ||
+      lay->row_stride < ctx->grid.nx || lay->comp_stride < lay->row_stride * ctx->grid.ny ||
+      !(dt >= 0.0)) {
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return launch_adjoint_step(ctx, U_n, lambda_next, lambda_n, lay->row_stride, lay->comp_stride, dt);
+}
+
+// ============================================================ checkpointing
+namespace ec {
+ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, long long rs,
+                               long long cs, double dt, int device_dt);
+ec_status launch_state_reduce(ec_ctx* ctx, const double* u, long long rs, long long cs, int mode);
+
+// Copy a state between layouts (4 planes of nx x ny).
+static ec_status copy_state(ec_ctx* ctx, double* dst, long long drs, long long dcs, const double* src,
+                            long long srs, long long scs)
+{
+  const size_t nx = (size_t)ctx->grid.nx, ny = (size_t)ctx->grid.ny;
+  for (int c = 0; c < 4; ++c) {
+    EC_HIP_TRY(ctx, hipMemcpy2DAsync(dst + c * dcs, drs * sizeof(double), src + c * scs,
+                                     srs * sizeof(double), nx * sizeof(double), ny,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return EC_OK;
+}
+
+static ec_status ensure_states(ec_ctx* ctx, AdjWork* w, int count)
+{
+  if (w->seg_cap >= count) return EC_OK;
+  if (w->states) (void)hipFree(w->states);
+  w->states = nullptr;
+  w->seg_cap = 0;
+  const size_t bytes = (size_t)count * 4 * (size_t)w->nx * w->ny * sizeof(double);
+  if (hipMalloc((void**)&w->states, bytes) != hipSuccess) return EC_ERR_OUT_OF_MEMORY;
+  w->seg_cap = count;
+  (void)ctx;
+  return EC_OK;
+}
+
+static ec_status ensure_lam(AdjWork* w)
+{
+  if (w->lam) return EC_OK;
+  const size_t bytes = 2 * 4 * (size_t)w->nx * w->ny * sizeof(double);
+  return hipMalloc((void**)&w->lam, bytes) == hipSuccess ? EC_OK : EC_ERR_OUT_OF_MEMORY;
+}
+
+}  // namespace ec
+
+static bool lay_ok(const ec_ctx* ctx, const ec_layout* l)
+{
+  return l && l->row_stride >= ctx->grid.nx && l->comp_stride >= l->row_stride * ctx->grid.ny;
+}
+
+extern "C" ec_status ec_forward_checkpointed(ec_ctx* ctx, const double* U0, double* U_final,
+                                             const ec_layout* lay, int nsteps, int checkpoint_every,
+                                             double cfl, double stop_time, double* checkpoints,
+                                             double* dts, int* steps_done)
+{
+  if (!ctx || !U0 || !U_final || !lay_ok(ctx, lay) || nsteps < 1 || checkpoint_every < 1 ||
+      !checkpoints || !dts || !steps_done) {
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  AdjWork* w = nullptr;
+  ec_status st = adjoint_workspace(ctx, &w);
+  if (st != EC_OK) return st;
+  if ((st = ensure_states(ctx, w, 2)) != EC_OK) return st;
+  if (w->dt_cap < nsteps) {
+    if (w->dtlog) (void)hipFree(w->dtlog);
+    w->dtlog = nullptr;
+    w->dt_cap = 0;
+    if (hipMalloc((void**)&w->dtlog, (size_t)nsteps * sizeof(double)) != hipSuccess) {
+      return EC_ERR_OUT_OF_MEMORY;
+    }
+    w->dt_cap = nsteps;
+  }
+  const long long nx = ctx->grid.nx, n = nx * ctx->grid.ny;
+  double* A = w->states;
+  double* B = w->states + 4 * n;
+  if ((st = copy_state(ctx, A, nx, n, U0, lay->row_stride, lay->comp_stride)) != EC_OK) return st;
+  DevCtrl& h = *ctx->ctrl_host;
+  std::memset(&h, 0, sizeof(DevCtrl));
+  h.stop_time = stop_time;
+  h.cfl = cfl;
+  h.dt_min = INFINITY;
+  h.max_step = nsteps;
+  EC_HIP_TRY(ctx, hipMemsetAsync(w->dtlog, 0, (size_t)nsteps * sizeof(double), ctx->stream));
+  EC_HIP_TRY(ctx, hipMemcpyAsync(ctx->ctrl, &h, sizeof(DevCtrl), hipMemcpyHostToDevice, ctx->stream));
+  if ((st = launch_state_reduce(ctx, A, nx, n, 3)) != EC_OK) return st;
+  ctx->dt_log = w->dtlog;
+  for (int s = 0; s < nsteps; ++s) {
+    if (s % checkpoint_every == 0) {
+      EC_HIP_TRY(ctx, hipMemcpyAsync(checkpoints + (size_t)(s / checkpoint_every) * 4 * n, A,
+                                     4 * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    st = launch_explicit_step(ctx, A, B, nx, n, 0.0, 1);
+    if (st != EC_OK) {
+      ctx->dt_log = nullptr;
+      return st;
+    }
+    double* t = A;
+    A = B;
+    B = t;
+  }
+  ctx->dt_log = nullptr;
+  if ((st = copy_state(ctx, U_final, lay->row_stride, lay->comp_stride, A, nx, n)) != EC_OK) return st;
+  EC_HIP_TRY(ctx, hipMemcpyAsync(dts, w->dtlog, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  EC_HIP_TRY(ctx, hipMemcpyAsync(&h, ctx->ctrl, sizeof(DevCtrl), hipMemcpyDeviceToHost, ctx->stream));
+  EC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *steps_done = h.step;
+  return EC_OK;
+}
+
+extern "C" ec_status ec_adjoint_checkpointed(ec_ctx* ctx, const double* checkpoints, const ec_layout* lay,
+                                             int nsteps, int checkpoint_every, const double* dts,
+                                             const double* lambda_N, double* lambda_0)
+{
+  if (!ctx || !checkpoints || !lay_ok(ctx, lay) || nsteps < 1 || checkpoint_every < 1 || !dts ||
+      !lambda_N || !lambda_0) {
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  AdjWork* w = nullptr;
+  ec_status st = adjoint_workspace(ctx, &w);
+  if (st != EC_OK) return st;
+  const int K = checkpoint_every;
+  if ((st = ensure_states(ctx, w, K < 2 ? 2 : K)) != EC_OK) return st;
+  if ((st = ensure_lam(w)) != EC_OK) return st;
+  const long long nx = ctx->grid.nx, n = nx * ctx->grid.ny;
+  double* la = w->lam;
+  double* lb = w->lam + 4 * n;
+  if ((st = copy_state(ctx, la, nx, n, lambda_N, lay->row_stride, lay->comp_stride)) != EC_OK) return st;
+  const int nseg = (nsteps + K - 1) / K;
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int start = seg * K;
+    const int len = nsteps - start < K ? nsteps - start : K;
+    // recompute the forward states of the segment from its checkpoint
+    EC_HIP_TRY(ctx, hipMemcpyAsync(w->states, checkpoints + (size_t)seg * 4 * n, 4 * n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    for (int s = 1; s < len; ++s) {
+      st = launch_explicit_step(ctx, w->states + (size_t)(s - 1) * 4 * n, w->states + (size_t)s * 4 * n,
+                                nx, n, dts[start + s - 1], 0);
+      if (st != EC_OK) return st;
+    }
+    for (int s = len - 1; s >= 0; --s) {
+      st = launch_adjoint_step(ctx, w->states + (size_t)s * 4 * n, la, lb, nx, n, dts[start + s]);
+      if (st != EC_OK) return st;
+      double* t = la;
+      la = lb;
+      lb = t;
+    }
+  }
+  if ((st = copy_state(ctx, lambda_0, lay->row_stride, lay->comp_stride, la, nx, n)) != EC_OK) return st;
+  EC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return EC_OK;
+}

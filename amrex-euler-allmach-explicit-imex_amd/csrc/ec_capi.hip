@@ -10,6 +10,7 @@ int num_tile_shapes();
 int step_partials_needed(const ec_ctx* ctx, int tile_id);
 int reduce_blocks(const ec_ctx* ctx);
 void imex_release(ec_ctx* ctx);
+void adjoint_release(ec_ctx* ctx);
 
 GridParams make_grid_params(const ec_grid_desc& d)
 {
@@ -113,6 +114,7 @@ extern "C" ec_status ec_destroy(ec_ctx* ctx)
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   else (void)hipDeviceSynchronize();
   imex_release(ctx);
+  adjoint_release(ctx);
   if (ctx->ctrl) (void)hipFree(ctx->ctrl);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
   if (ctx->partials) (void)hipFree(ctx->partials);

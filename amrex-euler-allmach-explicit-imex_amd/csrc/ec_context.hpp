@@ -53,8 +53,10 @@ struct ec_ctx {
   int max_partials = 0;
   double* scratch_state = nullptr;    // in-place step buffer (4*nx*ny)
   std::string last_error;
-  // IMEX and batch workspaces are attached lazily by their modules.
+  double* dt_log = nullptr;           // device-loop dt record (checkpointed runs)
+  // IMEX, adjoint and batch workspaces are attached lazily by their modules.
   void* imex = nullptr;
+  void* adjoint = nullptr;
 };
 
 namespace ec {

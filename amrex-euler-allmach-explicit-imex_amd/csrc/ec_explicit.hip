@@ -21,6 +21,7 @@
 // of tile rows: neighbouring tiles then share halo rows through one L2.
 
 #include "ec_context.hpp"
+#include "ec_ghost.hpp"
 #include "ec_state.hpp"
 
 namespace ec {
@@ -34,56 +35,9 @@ struct StepArgs {
   int device_dt;    // 1: dt from ctrl (device-resident loop)
   DevCtrl* ctrl;
   Partial* partials;
+  double* dt_log;   // optional: dt of every device-loop step, indexed by step
   int tiles_x, nblocks;
 };
-
-// Ghost rule of one axis; returns the valid index to read and whether the
-// normal momentum flips (boundary_conditions.hpp:2-25, :27-73).
-__device__ __forceinline__ int map_axis(int i, int n, int lo, int hi, bool& refl)
-{
-  refl = false;
-  if (i < 0) {
-    if (lo == BC_PERIODIC) {
-      const int m = i % n;
-      return m < 0 ? m + n : m;
-    }
-    if (lo == BC_REFLECT) {
-      refl = true;
-      i = -i - 1;
-    }
-  } else if (i >= n) {
-    if (hi == BC_PERIODIC) return i % n;
-    if (hi == BC_REFLECT) {
-      refl = true;
-      i = 2 * n - 1 - i;
-    }
-  } else {
-    return i;
-  }
-  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
-}
-
-__device__ __forceinline__ Cons fixed_state(const GridParams& g, int side)
-{
-  return Cons{g.fixed[side][0], g.fixed[side][1], g.fixed[side][2], g.fixed[side][3]};
-}
-
-__device__ __forceinline__ Cons load_cell(const double* __restrict__ u, long long rs, long long cs,
-                                          const GridParams& g, int i, int j)
-{
-  if (i < 0 && g.bc[0] == BC_FIXED) return fixed_state(g, 0);
-  if (i >= g.nx && g.bc[1] == BC_FIXED) return fixed_state(g, 1);
-  if (j < 0 && g.bc[2] == BC_FIXED) return fixed_state(g, 2);
-  if (j >= g.ny && g.bc[3] == BC_FIXED) return fixed_state(g, 3);
-  bool rx, ry;
-  const int ii = map_axis(i, g.nx, g.bc[0], g.bc[1], rx);
-  const int jj = map_axis(j, g.ny, g.bc[2], g.bc[3], ry);
-  const long long k = (long long)jj * rs + ii;
-  Cons c{u[k], u[cs + k], u[2 * cs + k], u[3 * cs + k]};
-  if (rx) c.mx = -c.mx;
-  if (ry) c.my = -c.my;
-  return c;
-}
 
 // compute_euler_dt's closing arithmetic (diagnostic_metrics_timestep.hpp:252-259).
 __host__ __device__ inline double euler_dt(double max_rate, double cfl, double time, double stop)
@@ -173,7 +127,7 @@ __device__ __forceinline__ Red block_reduce(Red r, Red* scratch)
 template <int NT>
 __device__ void finish_reduction(const Red& mine, Partial* partials, int slot, int nparts,
                                  DevCtrl* ctrl, int mode, bool active, bool dt_bad, double dt,
-                                 Red* scratch)
+                                 Red* scratch, double* dt_log = nullptr)
 {
   __shared__ int s_last;
   if (threadIdx.x == 0) {
@@ -196,6 +150,7 @@ __device__ void finish_reduction(const Red& mine, Partial* partials, int slot, i
   DevCtrl& c = *ctrl;
   if (mode == 1) {
     if (active) {
+      if (dt_log) dt_log[c.step] = dt;
       c.time = c.time + dt;
       c.step += 1;
       c.dt_last = dt;
@@ -450,7 +405,7 @@ __global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
   }
   mine = block_reduce<NT>(mine, red_scratch);
   finish_reduction<NT>(mine, A.partials, tile, A.nblocks, A.ctrl, A.device_dt ? 1 : 0, true, false,
-                       dt, red_scratch);
+                       dt, red_scratch, A.dt_log);
 }
 
 // Standalone max-rate + diagnostics reduction over a state (part (b) of the
@@ -535,6 +490,7 @@ ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, lon
   a.device_dt = device_dt;
   a.ctrl = ctx->ctrl;
   a.partials = ctx->partials;
+  a.dt_log = device_dt ? ctx->dt_log : nullptr;
   a.tiles_x = (ctx->grid.nx + s.tx - 1) / s.tx;
   const int tiles_y = (ctx->grid.ny + s.ty - 1) / s.ty;
   a.nblocks = a.tiles_x * tiles_y;

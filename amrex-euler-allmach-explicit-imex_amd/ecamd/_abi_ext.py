@@ -16,6 +16,27 @@ class ProblemDesc(C.Structure):
                 ("shock_x", C.c_double)]
 
 
+class ImexOpts(C.Structure):
+    _fields_ = [("picard_iterations", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int),
+                ("restart", C.c_int), ("precision", C.c_int), ("predictor_dissipation", C.c_int),
+                ("inner_reduction", C.c_double)]
+
+
+class ImexDiag(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("last_iterations", C.c_int),
+                ("picard_solves", C.c_int), ("refinements", C.c_int),
+                ("density_floor_count", C.c_int), ("guard_count", C.c_int),
+                ("nonfinite_count", C.c_int), ("residual_linf", C.c_double),
+                ("relative_residual_linf", C.c_double), ("rhs_norm", C.c_double),
+                ("coupled_residual_linf", C.c_double)]
+
+
+IMEX_STATUS = {0: "converged_absolute", 1: "converged_relative", 2: "linear_solve_not_converged",
+               3: "gmres_breakdown_not_converged", 4: "density_denominator_floor_used",
+               5: "source_map_pressure_row_nonfinite", 6: "linear_residual_nonfinite",
+               7: "source_map_lagged_momentum_nonfinite", 8: "not_run"}
+
+
 def declare(L) -> None:
     """Attach argtypes/restype for the entry points beyond the explicit step."""
     from ._lib import Layout
@@ -26,3 +47,15 @@ def declare(L) -> None:
     L.ec_init_state.restype = st
     L.ec_post_shock_state.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
     L.ec_post_shock_state.restype = st
+    L.ec_flux_host.argtypes = [dp, dp, C.c_int, C.c_double, C.c_int, C.c_double, dp]
+    L.ec_flux_host.restype = st
+    L.ec_flux_vjp_host.argtypes = [dp, dp, C.c_int, C.c_double, C.c_int, C.c_double, dp, dp, dp, dp]
+    L.ec_flux_vjp_host.restype = st
+    L.ec_imex_default_opts.argtypes = [C.POINTER(ImexOpts)]
+    L.ec_imex_default_opts.restype = st
+    L.ec_imex_step.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.POINTER(ImexOpts),
+                               C.POINTER(ImexDiag), vp]
+    L.ec_imex_step.restype = st
+    L.ec_imex_dt.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.c_double, C.c_double, C.c_int,
+                             C.c_double, dp, dp, dp]
+    L.ec_imex_dt.restype = st

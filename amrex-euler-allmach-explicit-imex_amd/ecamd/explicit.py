@@ -68,9 +68,12 @@ class ExplicitSolver:
 
     def __init__(self, grid: Grid, device: torch.device | str = "cuda", stream=None):
         self.grid = grid
-        self.device = torch.device(device)
-        if self.device.type != "cuda":
+        dev = torch.device(device)
+        if dev.type != "cuda":
             raise ValueError("ExplicitSolver runs on the GPU only")
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
         self._ctx = C.c_void_p()
         with torch.cuda.device(self.device):
@@ -171,6 +174,46 @@ class ExplicitSolver:
     @staticmethod
     def status_name(run: RunState) -> str:
         return RUN_STATUS.get(run.status, str(run.status))
+
+    # ---- IMEX T1/S2 BDLTV20 (imex/imex_source_map_step.hpp:300-616) --------
+    def imex_opts(self, **kw):
+        from ._abi_ext import ImexOpts
+        o = ImexOpts()
+        check(None, lib().ec_imex_default_opts(C.byref(o)), "ec_imex_default_opts")
+        prec = kw.pop("precision", None)
+        if prec is not None:
+            o.precision = {"fp64": 0, "mixed": 1}[prec] if isinstance(prec, str) else int(prec)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def imex_step(self, U: torch.Tensor, dt: float, opts=None, p_out: torch.Tensor | None = None):
+        """One IMEX step of U in place (only on success); returns the diagnostics."""
+        from ._abi_ext import ImexDiag
+        self._check_state(U)
+        o = opts if opts is not None else self.imex_opts()
+        d = ImexDiag()
+        pp = None
+        if p_out is not None:
+            if (p_out.dtype != torch.float64 or p_out.numel() != self.grid.nx * self.grid.ny
+                    or p_out.device != self.device or not p_out.is_contiguous()):
+                raise ValueError("p_out must be a contiguous float64 nx*ny device tensor")
+            pp = _ptr(p_out)
+        check(self._ctx, lib().ec_imex_step(self._ctx, _ptr(U), C.byref(self.layout), float(dt),
+                                             C.byref(o), C.byref(d), pp), "ec_imex_step")
+        return d
+
+    def imex_dt(self, U: torch.Tensor, cfl: float, time: float, stop_time: float,
+                acoustic_startup: int = 0, acoustic_cfl_cap: float = 0.0):
+        self._check_state(U)
+        dt, m, a = C.c_double(), C.c_double(), C.c_double()
+        check(self._ctx, lib().ec_imex_dt(self._ctx, _ptr(U), C.byref(self.layout), cfl, time,
+                                           stop_time, acoustic_startup, acoustic_cfl_cap,
+                                           C.byref(dt), C.byref(m), C.byref(a)), "ec_imex_dt")
+        return dt.value, m.value, a.value
+
+
+Solver = ExplicitSolver
 
 
 def run_explicit(solver: ExplicitSolver, U0: torch.Tensor, cfl: float, stop_time: float,

@@ -158,6 +158,105 @@ int ec_explicit_tile_count(void);
 ec_status ec_explicit_tile_shape(int tile_id, int* tx, int* ty, int* threads);
 ec_status ec_set_explicit_tile(ec_ctx* ctx, int tile_id);
 
+/* ------------------------------------------------ IMEX T1/S2 BDLTV20 step */
+/* solver_status of the reference (imex/imex_source_map_step.hpp:398-567). */
+typedef enum {
+  EC_IMEX_CONVERGED_ABSOLUTE = 0,
+  EC_IMEX_CONVERGED_RELATIVE = 1,
+  EC_IMEX_NOT_CONVERGED = 2,        /* linear_solve_not_converged */
+  EC_IMEX_BREAKDOWN = 3,            /* gmres_breakdown_not_converged */
+  EC_IMEX_DENSITY_FLOOR = 4,        /* density_denominator_floor_used */
+  EC_IMEX_ROW_NONFINITE = 5,        /* source_map_pressure_row_nonfinite */
+  EC_IMEX_RESIDUAL_NONFINITE = 6,   /* linear_residual_nonfinite */
+  EC_IMEX_MOMENTUM_NONFINITE = 7,   /* source_map_lagged_momentum_nonfinite */
+  EC_IMEX_NOT_RUN = 8               /* not_run */
+} ec_imex_status;
+
+typedef enum { EC_IMEX_PRECISION_FP64 = 0, EC_IMEX_PRECISION_MIXED = 1 } ec_imex_precision;
+
+typedef struct {
+  int picard_iterations;     /* euler.imex_picard_iterations */
+  double tol;                /* euler.imex_solver_tol (absolute or relative-to-max(|b|inf,1)) */
+  int max_iter;              /* euler.imex_solver_max_iter: Krylov iterations per solve */
+  int restart;               /* GMRES restart length, 1..64 */
+  int precision;             /* ec_imex_precision */
+  int predictor_dissipation; /* euler.imex_predictor_dissipation: 0 material, 1 acoustic */
+  double inner_reduction;    /* mixed: 2-norm reduction asked of each FP32 inner cycle */
+} ec_imex_opts;
+
+typedef struct {
+  int status;                /* ec_imex_status */
+  int iterations;            /* Krylov iterations summed over the Picard solves */
+  int last_iterations;
+  int picard_solves;
+  int refinements;           /* mixed: FP64 refinement steps of the last solve */
+  int density_floor_count;
+  int guard_count;           /* face-enthalpy denominator guards in the closeout */
+  int nonfinite_count;
+  double residual_linf;      /* |A p - b|_inf of the last solve */
+  double relative_residual_linf;
+  double rhs_norm;           /* max(|b|_inf, 1) */
+  double coupled_residual_linf; /* max |p/(g-1) - E + K| after the closeout */
+} ec_imex_diag;
+
+/* Defaults of core/run_config.hpp:2-55 (2 Picard solves, tol 1e-10,
+ * max_iter 100) with restart 30 and FP64 iterations. */
+ec_status ec_imex_default_opts(ec_imex_opts* opts);
+
+/* One IMEX step of U in place.  Replaces ImexStepDiagnostics
+ * euler_imex_bdltv20_o1_source_map_picard_one_step(MultiFab&, const
+ * Geometry&, const RunConfig&, Real dt, MultiFab*) (imex/
+ * imex_source_map_step.hpp:300-616).  As there, U changes only when the step
+ * succeeds (diag->status 0 or 1).  p_out (optional, device, nx*ny compact)
+ * receives the final pressure (the reference's solved_pressure_check). */
+ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* layout, double dt,
+                       const ec_imex_opts* opts, ec_imex_diag* diag, double* p_out);
+
+/* compute_imex_dt (numerics/diagnostic_metrics_timestep.hpp:262-326): the eq79
+ * material rate 2|u|/min(dx,dy) and the acoustic rate, reduced on the device. */
+ec_status ec_imex_dt(ec_ctx* ctx, const double* U, const ec_layout* layout, double cfl,
+                     double time, double stop_time, int acoustic_startup,
+                     double acoustic_cfl_cap, double* dt, double* material_rate_max,
+                     double* acoustic_rate_max);
+
+/* ------------------------------------------------------------- adjoint */
+/* Adjoint of one explicit O2 step (Rusanov, HLLC, low-Mach HLLC or HLLC-P;
+ * any limiter and ghost rule), with dt and every branch frozen at the
+ * forward state U_n:  lambda_n = (dU_{n+1}/dU_n)^T lambda_next.  No reference
+ * counterpart (the reference has no adjoint); it differentiates
+ * euler_fv_one_step (numerics/explicit_update.hpp:58-102). */
+ec_status ec_adjoint_step(ec_ctx* ctx, const double* U_n, const double* lambda_next,
+                          double* lambda_n, const ec_layout* layout, double dt);
+
+/* Forward driver loop (device dt, as ec_explicit_advance) for nsteps steps,
+ * storing the state every checkpoint_every steps into checkpoints (device,
+ * compact 4*nx*ny doubles each, ceil(nsteps/checkpoint_every) of them) and
+ * every step's dt into dts (host, nsteps). */
+ec_status ec_forward_checkpointed(ec_ctx* ctx, const double* U0, double* U_final,
+                                  const ec_layout* layout, int nsteps, int checkpoint_every,
+                                  double cfl, double stop_time, double* checkpoints, double* dts,
+                                  int* steps_done);
+
+/* Reverse sweep over nsteps steps from the checkpoints: each segment is
+ * recomputed once from its checkpoint with the recorded dts, then the
+ * adjoint steps run backwards.  lambda_N and lambda_0 use layout. */
+ec_status ec_adjoint_checkpointed(ec_ctx* ctx, const double* checkpoints, const ec_layout* layout,
+                                  int nsteps, int checkpoint_every, const double* dts,
+                                  const double* lambda_N, double* lambda_0);
+
+/* ------------------------------------------- pointwise verification hooks */
+/* Host-side evaluation of one face flux (the device code's arithmetic) and of
+ * its hand-written vector-Jacobian product: qL/qR primitive (rho,u,v,p),
+ * F = interface_flux (numerics/reconstruction_fluxes.hpp:373-388), and
+ * qLbar/qRbar/sensor_bar = (dF/d.)^T Fbar at the forward state's branches.
+ * No reference counterpart: they let a caller check the adjoint against
+ * finite differences without a GPU. */
+ec_status ec_flux_host(const double* qL, const double* qR, int axis, double gamma, int riemann,
+                       double sensor_f, double* F);
+ec_status ec_flux_vjp_host(const double* qL, const double* qR, int axis, double gamma,
+                           int riemann, double sensor_f, const double* Fbar, double* qLbar,
+                           double* qRbar, double* sensor_bar);
+
 /* ---------------------------------------------------- host initial data */
 /* Test problems of problems/problem_initialization.hpp:420-435 (selector
  * euler.problem, core/config_parse_helpers.hpp:11-42), plus

@@ -1,0 +1,102 @@
+"""Parity of the device IMEX T1/S2 BDLTV20 step with the oracle.
+
+IMEX parity is defined at solver-tolerance level (the reference's GMRES runs
+on the host with Eigen; the pressure is determined by the linear system only
+up to the tolerance): after a step both sides must meet the reference's
+convergence criterion, and the new states must agree to O(tol * |A^-1|).
+The star update, the face enthalpies and the closeout carry no solver error
+and are checked at round-off level through the uniform-flow and
+conservation properties.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ecamd():
+    import ecamd as m
+    return m
+
+
+def rel_l1(a, b):
+    a = np.asarray(a).reshape(4, -1)
+    b = np.asarray(b).reshape(4, -1)
+    return np.abs(a - b).sum(axis=1) / np.abs(b).sum(axis=1)
+
+
+def solver(ecamd, n, gpu):
+    return ecamd.Solver(ecamd.Grid(n, n, 1.0 / n, 1.0 / n, 1.4, "hllc", "minmod", 2), gpu)
+
+
+def test_imex_dt_matches_oracle(ecor, ecamd, gpu):
+    n = 64
+    U = ecor.init_gresho(n, n, mach=0.001)
+    g = ecor.make_grid(n, n, 1 / n, 1 / n)
+    s = solver(ecamd, n, gpu)
+    dt, m, a = s.imex_dt(torch.from_numpy(U).to(gpu), 0.8, 0.0, 1.0)
+    dto, mo, ao = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    assert (dt, m, a) == (dto, mo, ao)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gresho_imex_step_parity(ecor, ecamd, gpu, precision):
+    """BASELINE config 3 at 64^2 (M=0.001), one step.  The oracle's assembled
+    rows stagnate near 2.5e-10 relative residual at this size, so its tightest
+    solve (4e-10) is the reference state; the device solves to 1e-10.  Bar:
+    relative L1 <= 1e-8 per conserved variable (the oracle's own spread between
+    tol 1e-9 and 4e-10 is 3e-9) and pressure within 1e-11 of p_max."""
+    n = 64
+    U = ecor.init_gresho(n, n, mach=0.001)
+    g = ecor.make_grid(n, n, 1 / n, 1 / n)
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    ref, pref, r = ecor.imex_step(g, U, dt, picard=2, tol=4e-10, max_iter=2000)
+    assert r.status in (0, 1)
+    ref8, pref8, r8 = ecor.imex_step(g, U, dt, picard=2, tol=1e-8, max_iter=2000)
+    s = solver(ecamd, n, gpu)
+    for tol, R, P, bar in ((1e-10, ref, pref, 1e-8), (1e-8, ref8, pref8, 5e-7)):
+        Ud = torch.from_numpy(U).to(gpu)
+        p = torch.empty(n * n, dtype=torch.float64, device=gpu)
+        d = s.imex_step(Ud, dt, s.imex_opts(tol=tol, max_iter=2000, precision=precision), p)
+        assert ecamd.IMEX_STATUS[d.status] in ("converged_absolute", "converged_relative"), d.status
+        assert d.relative_residual_linf <= tol
+        got = Ud.cpu().numpy()
+        perr = np.abs(p.cpu().numpy() - P.reshape(-1)).max() / np.abs(P).max()
+        err = rel_l1(got, R)
+        print(precision, tol, d.iterations, d.refinements, perr, err)
+        assert perr < 100 * bar * 1e-4, perr
+        assert err[0] == 0.0                       # density is the star density
+        assert np.all(err <= bar), err
+    assert d.coupled_residual_linf < 1e-3 * d.rhs_norm
+
+
+def test_uniform_flow_is_exact_on_device(ecor, ecamd, gpu):
+    n = 32
+    q = np.array([1.3, 0.2, -0.1, 5.0])
+    U = np.broadcast_to(ecor.to_conserved(q)[:, None, None], (4, n, n)).copy()
+    s = solver(ecamd, n, gpu)
+    Ud = torch.from_numpy(U).to(gpu)
+    dt, _, _ = s.imex_dt(Ud, 0.8, 0.0, 1.0)
+    d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-12, max_iter=200))
+    assert d.status in (0, 1)
+    assert np.allclose(Ud.cpu().numpy(), U, rtol=1e-14, atol=1e-14)
+
+
+def test_periodic_conservation_and_failure_status(ecor, ecamd, gpu):
+    n = 48
+    U = ecor.init_gresho(n, n, mach=0.01)
+    s = solver(ecamd, n, gpu)
+    Ud = torch.from_numpy(U).to(gpu)
+    dt, _, _ = s.imex_dt(Ud, 0.8, 0.0, 1.0)
+    d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-10, max_iter=1000))
+    assert d.status in (0, 1)
+    V = Ud.cpu().numpy()
+    for c in (0, 3):
+        assert abs(V[c].sum() - U[c].sum()) <= 1e-13 * abs(U[c].sum())
+    # an iteration cap that cannot reach the tolerance leaves U untouched
+    Ud = torch.from_numpy(U).to(gpu)
+    d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-14, max_iter=3))
+    assert ecamd.IMEX_STATUS[d.status] == "linear_solve_not_converged"
+    assert np.array_equal(Ud.cpu().numpy(), U)
