@@ -51,6 +51,13 @@ def declare(L) -> None:
     L.ec_flux_host.restype = st
     L.ec_flux_vjp_host.argtypes = [dp, dp, C.c_int, C.c_double, C.c_int, C.c_double, dp, dp, dp, dp]
     L.ec_flux_vjp_host.restype = st
+    L.ec_adjoint_step.argtypes = [vp, vp, vp, vp, C.POINTER(Layout), C.c_double]
+    L.ec_adjoint_step.restype = st
+    L.ec_forward_checkpointed.argtypes = [vp, vp, vp, C.POINTER(Layout), C.c_int, C.c_int,
+                                          C.c_double, C.c_double, vp, dp, C.POINTER(C.c_int)]
+    L.ec_forward_checkpointed.restype = st
+    L.ec_adjoint_checkpointed.argtypes = [vp, vp, C.POINTER(Layout), C.c_int, C.c_int, dp, vp, vp]
+    L.ec_adjoint_checkpointed.restype = st
     L.ec_imex_default_opts.argtypes = [C.POINTER(ImexOpts)]
     L.ec_imex_default_opts.restype = st
     L.ec_imex_step.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.POINTER(ImexOpts),

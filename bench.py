@@ -56,6 +56,28 @@ def cpu_baseline(n: int, mach: float, cfl: float, steps: int) -> dict:
                       "wall_time_sec"}
 
 
+def copy_calibration(dev, n: int, steps: int) -> int:
+    """Device-to-device copies of one n x n state (4 planes of FP64): a plain
+    copy kernel of known size, so that FETCH_SIZE / WRITE_SIZE per dispatch can
+    be scaled to bytes."""
+    import torch
+    a = torch.ones((4, n, n), dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        b.copy_(a)
+    end.record()
+    torch.cuda.synchronize()
+    s = start.elapsed_time(end) / 1e3 / steps
+    nbytes = a.numel() * 8
+    print(json.dumps({"mode": "copy", "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
+                      "copies": steps, "us_per_copy": s * 1e6, "GB_per_s": 2 * nbytes / s / 1e9}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +85,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--riemann", default="xie_am_hllc_p")
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--mode", choices=("step", "copy"), default="step",
+                    help="copy: time a plain device copy of one state (HBM counter calibration)")
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--mach", type=float, default=0.01)
     ap.add_argument("--tile", type=int, default=-1, help="explicit tile shape id (-1: default)")
@@ -83,7 +108,9 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     n = args.n
-    grid = ecamd.Grid(n, n, 1.0 / n, 1.0 / n, 1.4, args.riemann, "minmod", 2)
+    if args.mode == "copy":
+        return copy_calibration(dev, n, args.steps)
+    grid = ecamd.Grid(n, n, 1.0 / n, 1.0 / n, 1.4, args.riemann, "minmod", args.order)
     stream = torch.cuda.current_stream(dev)
     solver = ecamd.ExplicitSolver(grid, dev, stream=stream)
     if args.tile >= 0:
@@ -118,7 +145,7 @@ def main() -> int:
     value = cells * args.steps / gpu_s
     achieved = BYTES_PER_CELL_UPDATE * cells / per_step / 1e9
     traffic = None
-    if TRAFFIC_FILE.exists() and n == 2048 and args.riemann == "xie_am_hllc_p":
+    if TRAFFIC_FILE.exists() and n == 2048 and args.riemann == "xie_am_hllc_p" and args.order == 2:
         try:
             traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
@@ -136,12 +163,12 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (Gresho vortex initial data generated on the host)",
-        "config": {"workload": f"gresho_vortex_{n}x{n}_periodic_M{args.mach:g}_explicit_o2_"
-                               f"minmod_{args.riemann}_cfl{args.cfl:g}",
+        "config": {"workload": f"gresho_vortex_{n}x{n}_periodic_M{args.mach:g}_explicit_"
+                               f"o{args.order}_minmod_{args.riemann}_cfl{args.cfl:g}",
                    "nx": n, "ny": n, "parallelism": "single_gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"explicit_step_kernel ({args.riemann}, O2)",
+                     "kernel": f"explicit_step_kernel ({args.riemann}, O{args.order})",
                      "bytes_per_cell_update": BYTES_PER_CELL_UPDATE},
         "wall_s": wall,
         "sim_time_advanced": run.time,
