@@ -48,6 +48,15 @@ def build(jobs: int | None = None) -> Path:
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # drop-in driver executable, linked against the library next to it
+    exe = PKG / "euler_compare_amd"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", str(PKG / "host" / "euler_compare_amd.cpp"),
+           "-o", str(exe) + ".tmp", f"-L{PKG}", "-lec_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed on the host driver")
+    os.replace(str(exe) + ".tmp", exe)
     return LIB
 
 

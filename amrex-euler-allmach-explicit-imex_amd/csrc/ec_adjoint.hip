@@ -2,6 +2,7 @@
 // pointwise flux / VJP entry points used to verify the hand-written reverse
 // mode against finite differences.
 #include <cstring>
+#include <vector>
 
 #include "ec_adjoint.hpp"
 #include "ec_context.hpp"
@@ -127,23 +128,23 @@ struct AdjArgs {
 };
 
 // Face-state bar storage: side 0 E (x,+), 1 W (x,-), 2 N (y,+), 3 S (y,-).
-__device__ __forceinline__ long long gidx(const GridParams& g, int i, int j)
+EC_HD long long gidx(const GridParams& g, int i, int j)
 {
   return (long long)(j + 2) * (g.nx + 4) + (i + 2);
 }
 
-__device__ __forceinline__ double* qb_at(const AdjArgs& A, int side, int i, int j)
+EC_HD double* qb_at(const AdjArgs& A, int side, int i, int j)
 {
   const long long G = (long long)(A.g.nx + 4) * (A.g.ny + 4);
   return A.qb + (long long)side * 4 * G + gidx(A.g, i, j);
 }
 
-__device__ __forceinline__ Prim prim_at(const AdjArgs& A, int i, int j)
+EC_HD Prim prim_at(const AdjArgs& A, int i, int j)
 {
   return to_prim(load_cell(A.U, A.rs, A.cs, A.g, i, j), A.g.gamma);
 }
 
-__device__ __forceinline__ Prim slope_of(const AdjArgs& A, int i, int j, int axis)
+EC_HD Prim slope_of(const AdjArgs& A, int i, int j, int axis)
 {
   const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
   const Prim m = prim_at(A, i - di, j - dj), c = prim_at(A, i, j), p = prim_at(A, i + di, j + dj);
@@ -152,7 +153,7 @@ __device__ __forceinline__ Prim slope_of(const AdjArgs& A, int i, int j, int axi
               limited_slope(c.v - m.v, p.v - c.v, lim), limited_slope(c.p - m.p, p.p - c.p, lim)};
 }
 
-__device__ __forceinline__ Prim face_state_at(const AdjArgs& A, int i, int j, int axis, double hs)
+EC_HD Prim face_state_at(const AdjArgs& A, int i, int j, int axis, double hs)
 {
   Prim q = prim_at(A, i, j);
   if (A.g.order == 2) {
@@ -165,26 +166,25 @@ __device__ __forceinline__ Prim face_state_at(const AdjArgs& A, int i, int j, in
   return q;
 }
 
-__device__ __forceinline__ Cons lam_at(const AdjArgs& A, int i, int j)
+EC_HD Cons lam_at(const AdjArgs& A, int i, int j)
 {
   if (i < 0 || j < 0 || i >= A.g.nx || j >= A.g.ny) return Cons{0.0, 0.0, 0.0, 0.0};
   const long long k = (long long)j * A.rs + i;
   return Cons{A.lam1[k], A.lam1[A.cs + k], A.lam1[2 * A.cs + k], A.lam1[3 * A.cs + k]};
 }
 
-__device__ __forceinline__ double ratio_of(const AdjArgs& A, int fi, int fj, int axis)
+EC_HD double ratio_of(const AdjArgs& A, int fi, int fj, int axis)
 {
   const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
   return pressure_ratio(face_state_at(A, fi - di, fj - dj, axis, 0.5).p,
                         face_state_at(A, fi, fj, axis, -0.5).p);
 }
 
-__global__ void adj_faces_kernel(AdjArgs A)
+EC_HD void adj_face_item(const AdjArgs& A, long long t)
 {
   const GridParams& g = A.g;
-  const long long nxf = (long long)(g.nx + 1) * g.ny, nyf = (long long)g.nx * (g.ny + 1);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nxf + nyf;
-       t += (long long)gridDim.x * blockDim.x) {
+  const long long nxf = (long long)(g.nx + 1) * g.ny;
+  {
     int axis, fi, fj;
     if (t < nxf) {
       axis = 0;
@@ -253,13 +253,23 @@ __global__ void adj_faces_kernel(AdjArgs A)
   }
 }
 
+__global__ void adj_faces_kernel(AdjArgs A)
+{
+  const long long nf = (long long)(A.g.nx + 1) * A.g.ny + (long long)A.g.nx * (A.g.ny + 1);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nf;
+       t += (long long)gridDim.x * blockDim.x) {
+    adj_face_item(A, t);
+  }
+}
+
 // HLLC-P: gather each face ratio's bar and pass it to its face pressures.
-__global__ void adj_ratios_kernel(AdjArgs A)
+EC_HD void adj_ratio_item(const AdjArgs& A, long long t)
 {
   const GridParams& g = A.g;
   const long long nxf = (long long)(g.nx + 1) * g.ny;
   const long long nxr = (long long)(g.nx + 1) * (g.ny + 2), nyr = (long long)(g.nx + 2) * (g.ny + 1);
   auto xf = [&](int fi, int j, int want) -> double {  // x flux face (fi, j)
+    (void)nxr;
     if (fi < 0 || fi > g.nx || j < 0 || j >= g.ny) return 0.0;
     const long long t = (long long)j * (g.nx + 1) + fi;
     return A.sel[t] == want ? A.mbar[t] : 0.0;
@@ -270,8 +280,8 @@ __global__ void adj_ratios_kernel(AdjArgs A)
     return A.sel[t] == want ? A.mbar[t] : 0.0;
   };
   const long long G = (long long)(g.nx + 4) * (g.ny + 4);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nxr + nyr;
-       t += (long long)gridDim.x * blockDim.x) {
+  (void)nyr;
+  {
     int axis, fi, fj;
     double bar;
     if (t < nxr) {
@@ -288,7 +298,7 @@ __global__ void adj_ratios_kernel(AdjArgs A)
       bar = yf(fi, fj, 0) + xf(fi + 1, fj, 1) + xf(fi + 1, fj - 1, 2) + xf(fi, fj, 3) +
             xf(fi, fj - 1, 4);
     }
-    if (bar == 0.0) continue;
+    if (bar == 0.0) return;
     const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
     const double pl = face_state_at(A, fi - di, fj - dj, axis, 0.5).p;
     const double pr = face_state_at(A, fi, fj, axis, -0.5).p;
@@ -299,7 +309,16 @@ __global__ void adj_ratios_kernel(AdjArgs A)
   }
 }
 
-__device__ __forceinline__ Prim qb_load(const AdjArgs& A, int side, int i, int j)
+__global__ void adj_ratios_kernel(AdjArgs A)
+{
+  const long long nr = (long long)(A.g.nx + 1) * (A.g.ny + 2) + (long long)(A.g.nx + 2) * (A.g.ny + 1);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nr;
+       t += (long long)gridDim.x * blockDim.x) {
+    adj_ratio_item(A, t);
+  }
+}
+
+EC_HD Prim qb_load(const AdjArgs& A, int side, int i, int j)
 {
   if (i < -2 || j < -2 || i > A.g.nx + 1 || j > A.g.ny + 1) return Prim{0.0, 0.0, 0.0, 0.0};
   const long long G = (long long)(A.g.nx + 4) * (A.g.ny + 4);
@@ -308,19 +327,18 @@ __device__ __forceinline__ Prim qb_load(const AdjArgs& A, int side, int i, int j
 }
 
 // Bar of the slope of cell (i,j) along an axis: 0.5 (bar_plus_state - bar_minus_state).
-__device__ __forceinline__ Prim slope_bar(const AdjArgs& A, int i, int j, int axis)
+EC_HD Prim slope_bar(const AdjArgs& A, int i, int j, int axis)
 {
   const Prim a = qb_load(A, axis == 0 ? 0 : 2, i, j), b = qb_load(A, axis == 0 ? 1 : 3, i, j);
   return Prim{0.5 * (a.r - b.r), 0.5 * (a.u - b.u), 0.5 * (a.v - b.v), 0.5 * (a.p - b.p)};
 }
 
-__global__ void adj_cells_kernel(AdjArgs A)
+EC_HD void adj_cell_item(const AdjArgs& A, long long t)
 {
   const GridParams& g = A.g;
   const int GX = g.nx + 4;
   const long long G = (long long)GX * (g.ny + 4);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < G;
-       t += (long long)gridDim.x * blockDim.x) {
+  {
     const int i = (int)(t % GX) - 2, j = (int)(t / GX) - 2;
     Prim qbar{0.0, 0.0, 0.0, 0.0};
     for (int side = 0; side < 4; ++side) add(qbar, qb_load(A, side, i, j));
@@ -367,9 +385,18 @@ __global__ void adj_cells_kernel(AdjArgs A)
   }
 }
 
+__global__ void adj_cells_kernel(AdjArgs A)
+{
+  const long long G = (long long)(A.g.nx + 4) * (A.g.ny + 4);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < G;
+       t += (long long)gridDim.x * blockDim.x) {
+    adj_cell_item(A, t);
+  }
+}
+
 // Where the explicit ghost rule reads grown cell (i,j) from: valid index and
 // momentum flips, or false for a fixed state.
-__device__ __forceinline__ bool ghost_source(const GridParams& g, int i, int j, int& ii, int& jj,
+EC_HD bool ghost_source(const GridParams& g, int i, int j, int& ii, int& jj,
                                              bool& rx, bool& ry)
 {
   if (i < 0 && g.bc[0] == BC_FIXED) return false;
@@ -381,14 +408,12 @@ __device__ __forceinline__ bool ghost_source(const GridParams& g, int i, int j, 
   return true;
 }
 
-__global__ void adj_fold_kernel(AdjArgs A)
+EC_HD void adj_fold_item(const AdjArgs& A, long long t)
 {
   const GridParams& g = A.g;
   const int GX = g.nx + 4;
   const long long G = (long long)GX * (g.ny + 4);
-  const long long n = (long long)g.nx * g.ny;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
+  {
     const int i = (int)(t % g.nx), j = (int)(t / g.nx);
     const int cx[5] = {i, -2, -1, g.nx, g.nx + 1};
     const int cy[5] = {j, -2, -1, g.ny, g.ny + 1};
@@ -409,6 +434,15 @@ __global__ void adj_fold_kernel(AdjArgs A)
     }
     const long long k = (long long)j * A.rs + i;
     for (int c = 0; c < 4; ++c) A.lam0[c * A.cs + k] = A.lam1[c * A.cs + k] + acc[c];
+  }
+}
+
+__global__ void adj_fold_kernel(AdjArgs A)
+{
+  const long long n = (long long)A.g.nx * A.g.ny;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    adj_fold_item(A, t);
   }
 }
 
@@ -444,6 +478,28 @@ ec_status launch_adjoint_step(ec_ctx* ctx, const double* U, const double* lam1, 
 }
 
 }  // namespace ec
+
+extern "C" ec_status ec_adjoint_step_host(const ec_grid_desc* desc, const double* U_n,
+                                          const double* lambda_next, double* lambda_n, double dt)
+{
+  if (!desc || !U_n || !lambda_next || !lambda_n) return EC_ERR_INVALID_ARGUMENT;
+  const GridParams g = make_grid_params(*desc);
+  const long long G = (long long)(g.nx + 4) * (g.ny + 4);
+  const long long nf = (long long)(g.nx + 1) * g.ny + (long long)g.nx * (g.ny + 1);
+  const long long nr = (long long)(g.nx + 1) * (g.ny + 2) + (long long)(g.nx + 2) * (g.ny + 1);
+  std::vector<double> qb(16 * G, 0.0), ub(4 * G, 0.0), mbar(nf, 0.0);
+  std::vector<int> sel(nf, -1);
+  const long long n = (long long)g.nx * g.ny;
+  AdjArgs A{U_n, lambda_next, lambda_n, g.nx, n, g, dt, qb.data(), ub.data(),
+            g.riemann == 3 ? sel.data() : nullptr, mbar.data()};
+  for (long long t = 0; t < nf; ++t) adj_face_item(A, t);
+  if (g.riemann == 3) {
+    for (long long t = 0; t < nr; ++t) adj_ratio_item(A, t);
+  }
+  for (long long t = 0; t < G; ++t) adj_cell_item(A, t);
+  for (long long t = 0; t < n; ++t) adj_fold_item(A, t);
+  return EC_OK;
+}
 
 extern "C" ec_status ec_adjoint_step(ec_ctx* ctx, const double* U_n, const double* lambda_next,
                                      double* lambda_n, const ec_layout* lay, double dt)

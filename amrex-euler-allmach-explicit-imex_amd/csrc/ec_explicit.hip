@@ -121,34 +121,44 @@ __device__ __forceinline__ Red block_reduce(Red r, Red* scratch)
   return r;
 }
 
-// Publishes this workgroup's partial; the last workgroup to arrive reduces
-// all partials and updates the run control.  mode: 0 host-dt step, 1 device
+// Run control of a device-loop step, as every workgroup of the step kernel
+// and its finalize kernel derive it from the control block.
+__device__ __forceinline__ void loop_control(const DevCtrl* c, double& dt, bool& active, bool& dt_bad)
+{
+  active = c->status == 0 && c->step < c->max_step && c->time < c->stop_time - 1.0e-14;
+  dt_bad = false;
+  dt = 0.0;
+  if (active) {
+    dt = euler_dt(c->max_rate, c->cfl, c->time, c->stop_time);
+    if (!isfinite(dt) || dt <= 0.0) {
+      active = false;
+      dt_bad = true;
+    }
+  }
+}
+
+// Reduces the per-workgroup partials of a step (or of a state reduction) and
+// updates the run control.  One workgroup; launched after the producer, so
+// the kernel boundary orders the partial stores before these loads and no
+// device-scope fence is needed anywhere.  mode: 0 host-dt step, 1 device
 // loop step, 2 state diagnostics only, 3 initial diagnostics of a run.
 template <int NT>
-__device__ void finish_reduction(const Red& mine, Partial* partials, int slot, int nparts,
-                                 DevCtrl* ctrl, int mode, bool active, bool dt_bad, double dt,
-                                 Red* scratch, double* dt_log = nullptr)
+__global__ void __launch_bounds__(NT) finalize_kernel(const Partial* partials, int nparts, DevCtrl* ctrl,
+                                                      int mode, double* dt_log)
 {
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    partials[slot] = Partial{mine.rate, mine.rmin, mine.pmin, mine.bad, mine.deg};
-    __threadfence();
-    const unsigned prev = atomicAdd(&ctrl->counter, 1u);
-    s_last = prev == (unsigned)(nparts - 1);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
+  __shared__ Red scratch[NT / 64];
   Red r = red_identity();
   for (int p = threadIdx.x; p < nparts; p += NT) {
     const Partial q = partials[p];
     red_combine(r, Red{q.rate, q.rho_min, q.p_min, q.bad, q.degenerate});
   }
-  __syncthreads();
   r = block_reduce<NT>(r, scratch);
   if (threadIdx.x != 0) return;
   DevCtrl& c = *ctrl;
   if (mode == 1) {
+    double dt;
+    bool active, dt_bad;
+    loop_control(&c, dt, active, dt_bad);
     if (active) {
       if (dt_log) dt_log[c.step] = dt;
       c.time = c.time + dt;
@@ -177,7 +187,11 @@ __device__ void finish_reduction(const Red& mine, Partial* partials, int slot, i
     c.nonfinite = r.bad;
     if (mode == 3 && (r.bad > 0 || r.rmin <= 0.0 || r.pmin <= 0.0)) c.status = 1;
   }
-  c.counter = 0u;
+}
+
+__device__ __forceinline__ void store_partial(Partial* partials, int slot, const Red& r)
+{
+  if (threadIdx.x == 0) partials[slot] = Partial{r.rate, r.rmin, r.pmin, r.bad, r.deg};
 }
 
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
@@ -218,17 +232,8 @@ __global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
   // Run control: every workgroup derives the same dt and activity.
   double dt = A.dt;
   bool active = true, dt_bad = false;
-  if (A.device_dt) {
-    const DevCtrl* c = A.ctrl;
-    active = c->status == 0 && c->step < c->max_step && c->time < c->stop_time - 1.0e-14;
-    if (active) {
-      dt = euler_dt(c->max_rate, c->cfl, c->time, c->stop_time);
-      if (!isfinite(dt) || dt <= 0.0) {
-        active = false;
-        dt_bad = true;
-      }
-    }
-  }
+  if (A.device_dt) loop_control(A.ctrl, dt, active, dt_bad);
+  (void)dt_bad;
 
   Red mine = red_identity();
   if (!active) {
@@ -242,9 +247,7 @@ __global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
         for (int c = 0; c < 4; ++c) A.uout[c * A.cs + k] = A.uin[c * A.cs + k];
       }
     }
-    mine = block_reduce<NT>(mine, red_scratch);
-    finish_reduction<NT>(mine, A.partials, tile, A.nblocks, A.ctrl, A.device_dt ? 1 : 0, false,
-                         dt_bad, dt, red_scratch);
+    store_partial(A.partials, tile, mine);
     return;
   }
 
@@ -404,8 +407,7 @@ __global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
     }
   }
   mine = block_reduce<NT>(mine, red_scratch);
-  finish_reduction<NT>(mine, A.partials, tile, A.nblocks, A.ctrl, A.device_dt ? 1 : 0, true, false,
-                       dt, red_scratch, A.dt_log);
+  store_partial(A.partials, tile, mine);
 }
 
 // Standalone max-rate + diagnostics reduction over a state (part (b) of the
@@ -413,7 +415,7 @@ __global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
 template <int NT>
 __global__ void __launch_bounds__(NT)
     state_reduce_kernel(const double* __restrict__ u, long long rs, long long cs, GridParams g,
-                        DevCtrl* ctrl, Partial* partials, int mode)
+                        DevCtrl* /*ctrl*/, Partial* partials, int /*mode*/)
 {
   __shared__ Red red_scratch[NT / 64];
   Red mine = red_identity();
@@ -424,8 +426,7 @@ __global__ void __launch_bounds__(NT)
     red_cell(mine, Cons{u[k], u[cs + k], u[2 * cs + k], u[3 * cs + k]}, g);
   }
   mine = block_reduce<NT>(mine, red_scratch);
-  finish_reduction<NT>(mine, partials, blockIdx.x, gridDim.x, ctrl, mode, true, false, 0.0,
-                       red_scratch);
+  store_partial(partials, blockIdx.x, mine);
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -500,6 +501,8 @@ ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, lon
   }
   step_fn(ctx->tile_id, ctx->grid.riemann, ctx->grid.order)(a, dim3(a.nblocks), dim3(s.nt),
                                                              ctx->stream);
+  hipLaunchKernelGGL((finalize_kernel<256>), dim3(1), dim3(256), 0, ctx->stream, ctx->partials,
+                     a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
   EC_HIP_TRY(ctx, hipGetLastError());
   return EC_OK;
 }
@@ -518,6 +521,8 @@ ec_status launch_state_reduce(ec_ctx* ctx, const double* u, long long rs, long l
   const int nb = reduce_blocks(ctx);
   hipLaunchKernelGGL((state_reduce_kernel<256>), dim3(nb), dim3(256), 0, ctx->stream, u, rs, cs,
                      ctx->grid, ctx->ctrl, ctx->partials, mode);
+  hipLaunchKernelGGL((finalize_kernel<256>), dim3(1), dim3(256), 0, ctx->stream, ctx->partials, nb,
+                     ctx->ctrl, mode, nullptr);
   EC_HIP_TRY(ctx, hipGetLastError());
   return EC_OK;
 }

@@ -10,7 +10,7 @@ namespace ec {
 
 // Ghost rule of one axis; returns the valid index to read and whether the
 // normal momentum flips (boundary_conditions.hpp:2-25, :27-73).
-__device__ __forceinline__ int map_axis(int i, int n, int lo, int hi, bool& refl)
+EC_HD int map_axis(int i, int n, int lo, int hi, bool& refl)
 {
   refl = false;
   if (i < 0) {
@@ -34,12 +34,12 @@ __device__ __forceinline__ int map_axis(int i, int n, int lo, int hi, bool& refl
   return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
 }
 
-__device__ __forceinline__ Cons fixed_state(const GridParams& g, int side)
+EC_HD Cons fixed_state(const GridParams& g, int side)
 {
   return Cons{g.fixed[side][0], g.fixed[side][1], g.fixed[side][2], g.fixed[side][3]};
 }
 
-__device__ __forceinline__ Cons load_cell(const double* __restrict__ u, long long rs, long long cs,
+EC_HD Cons load_cell(const double* __restrict__ u, long long rs, long long cs,
                                           const GridParams& g, int i, int j)
 {
   if (i < 0 && g.bc[0] == BC_FIXED) return fixed_state(g, 0);

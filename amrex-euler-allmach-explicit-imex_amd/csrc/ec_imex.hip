@@ -210,14 +210,13 @@ struct DSum {
   double v[K];
 };
 
-// Sum-reduce K doubles per workgroup into partials[block*K + k]; the last
-// workgroup to arrive sums the partials (fixed order) into out[k].
+// Sum-reduce K doubles per workgroup into partials[block*K + k] (plain
+// stores; a finalize kernel launched afterwards sums them, so no
+// device-scope fence is ever needed).
 template <int K>
-__device__ void reduce_sum_last(DSum<K> s, double* partials, unsigned int* counter, double* out,
-                                int kactive)
+__device__ void reduce_sum_store(DSum<K> s, double* partials, int kactive)
 {
   __shared__ double sh[RB / 64][K];
-  __shared__ int s_last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -234,29 +233,24 @@ __device__ void reduce_sum_last(DSum<K> s, double* partials, unsigned int* count
     for (int w = 0; w < RB / 64; ++w) v += sh[w][threadIdx.x];
     partials[(long long)blockIdx.x * K + threadIdx.x] = v;
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned prev = atomicAdd(counter, 1u);
-    s_last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  if (threadIdx.x < kactive) {
-    double v = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) v += partials[(long long)b * K + threadIdx.x];
-    out[threadIdx.x] = v;
-  }
-  if (threadIdx.x == 0) *counter = 0u;
 }
 
-// Max-reduce one double (NaN dropped) and sum one double.
-__device__ void reduce_maxsum_last(double mx, double sm, double* partials, unsigned int* counter,
-                                   double* out_max, double* out_sum)
+// out[q] = sum over blocks of partials[b*K + q], q < k (fixed order).
+__global__ void finalize_sum_kernel(const double* partials, int nparts, int K, int k, double* out,
+                                    const GmCtl* ctl)
+{
+  if (ctl && ctl->done) return;
+  for (int q = threadIdx.x; q < k; q += blockDim.x) {
+    double v = 0.0;
+    for (int b = 0; b < nparts; ++b) v += partials[(long long)b * K + q];
+    out[q] = v;
+  }
+}
+
+// Per-workgroup (max, sum) of one value each, stored as partials[2b], [2b+1].
+__device__ void reduce_maxsum_store(double mx, double sm, double* partials)
 {
   __shared__ double shm[RB / 64], shs[RB / 64];
-  __shared__ int s_last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -276,21 +270,21 @@ __device__ void reduce_maxsum_last(double mx, double sm, double* partials, unsig
     }
     partials[2 * blockIdx.x] = a;
     partials[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    s_last = prev == gridDim.x - 1;
   }
-  __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
-  __threadfence();
-  double a = -INFINITY, b = 0.0;
-  for (unsigned q = 0; q < gridDim.x; ++q) {
+}
+
+// combine = 0: (max, sum); 1: (max, max).
+__global__ void finalize_pair_kernel(const double* partials, int nparts, int combine, double* out_a,
+                                     double* out_b)
+{
+  if (threadIdx.x != 0) return;
+  double a = -INFINITY, b = combine ? -INFINITY : 0.0;
+  for (int q = 0; q < nparts; ++q) {
     a = smax(a, partials[2 * q]);
-    b += partials[2 * q + 1];
+    b = combine ? smax(b, partials[2 * q + 1]) : b + partials[2 * q + 1];
   }
-  *out_max = a;
-  *out_sum = b;
-  *counter = 0u;
+  *out_a = a;
+  *out_b = b;
 }
 
 // ------------------------------------------------------------ star update
@@ -535,7 +529,8 @@ __global__ void residual_kernel(OpArgs<double> A, const double* __restrict__ x,
     mx = (mx < a || isnan(a)) ? a : mx;
     s2 += v * v;
   }
-  reduce_maxsum_last(mx, s2, partials, &ctl->counter, &ctl->rinf, &ctl->r2);
+  (void)ctl;
+  reduce_maxsum_store(mx, s2, partials);
 }
 
 // ------------------------------------------------------------ GMRES pieces
@@ -556,7 +551,7 @@ __global__ void __launch_bounds__(RB) multidot_kernel(const T* __restrict__ V, l
       if (q < k) s.v[q] += (double)V[q * n + t] * wt;
     }
   }
-  reduce_sum_last<MAXK>(s, partials, &ctl->counter, ctl->dots1, k);
+  reduce_sum_store<MAXK>(s, partials, k);
 }
 
 // w -= V h1 (h1 = dots1), then dots2[i] = V_i . w (CGS pass 2, fused).
@@ -581,7 +576,7 @@ __global__ void __launch_bounds__(RB) update_dot_kernel(const T* __restrict__ V,
       if (q < k) s.v[q] += (double)V[q * n + t] * (double)wn;
     }
   }
-  reduce_sum_last<MAXK>(s, partials, &ctl->counter, ctl->dots2, k);
+  reduce_sum_store<MAXK>(s, partials, k);
 }
 
 // w -= V h2 (h2 = dots2) and |w|^2.
@@ -602,7 +597,7 @@ __global__ void __launch_bounds__(RB) update_norm_kernel(const T* __restrict__ V
     w[t] = wn;
     s.v[0] += (double)wn * (double)wn;
   }
-  reduce_sum_last<1>(s, partials, &ctl->counter, &ctl->nrm2, 1);
+  reduce_sum_store<1>(s, partials, 1);
 }
 
 // Column j of the Hessenberg matrix, Givens rotations, residual estimate and
@@ -754,7 +749,7 @@ __global__ void closeout_kernel(ImexGrid G, const double* __restrict__ S, const 
     U[3 * cs + k] = E;
   }
   if (bad) atomicAdd(&ctl->nonfinite, bad);
-  reduce_maxsum_last(rmax, 0.0, partials, &ctl->counter, &ctl->coupled_res, &ctl->r2);
+  reduce_maxsum_store(rmax, 0.0, partials);
 }
 
 // compute_imex_dt reductions (diagnostic_metrics_timestep.hpp:262-326): the
@@ -776,9 +771,7 @@ __global__ void imex_rates_kernel(const double* __restrict__ U, long long rs, lo
     if (!isnan(m)) mat = smax(mat, m);
     if (!isnan(a)) aco = smax(aco, a);
   }
-  // max of both through the (max, sum) reducer: encode aco in a second pass
   __shared__ double sh[2][RB / 64];
-  __shared__ int s_last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -798,20 +791,8 @@ __global__ void imex_rates_kernel(const double* __restrict__ U, long long rs, lo
     }
     partials[2 * blockIdx.x] = a;
     partials[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    s_last = atomicAdd(&ctl->counter, 1u) == gridDim.x - 1;
   }
-  __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
-  __threadfence();
-  double a = -INFINITY, b = -INFINITY;
-  for (unsigned q = 0; q < gridDim.x; ++q) {
-    a = smax(a, partials[2 * q]);
-    b = smax(b, partials[2 * q + 1]);
-  }
-  ctl->mat_max = a;
-  ctl->aco_max = b;
-  ctl->counter = 0u;
+  (void)ctl;
 }
 
 // ----------------------------------------------------------------- host
@@ -921,10 +902,16 @@ static ec_status gmres_cycle(ec_ctx* ctx, ImexWork* w, const OpArgs<T>& A, T* V,
     hipLaunchKernelGGL((matvec_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, A, vj, wv, w->ctl);
     hipLaunchKernelGGL((multidot_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, V, n, j + 1, wv,
                        w->partials, w->ctl);
+    hipLaunchKernelGGL(finalize_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials, nb, MAXK,
+                       j + 1, w->ctl->dots1, w->ctl);
     hipLaunchKernelGGL((update_dot_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, V, n, j + 1, wv,
                        w->partials, w->ctl);
+    hipLaunchKernelGGL(finalize_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials, nb, MAXK,
+                       j + 1, w->ctl->dots2, w->ctl);
     hipLaunchKernelGGL((update_norm_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, V, n, j + 1, wv,
                        w->partials, w->ctl);
+    hipLaunchKernelGGL(finalize_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials, nb, 1, 1,
+                       &w->ctl->nrm2, w->ctl);
     hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(64), 0, ctx->stream, w->R, w->g, w->cs, w->sn, j,
                        w->m, w->ctl, breakdown_tol);
     hipLaunchKernelGGL((normalize_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, wv,
@@ -957,6 +944,8 @@ static ec_status true_residual(ec_ctx* ctx, ImexWork* w, const OpArgs<double>& A
   const long long n = (long long)w->nx * w->ny;
   hipLaunchKernelGGL(residual_kernel, dim3(grid_blocks(n)), dim3(RB), 0, ctx->stream, A64, w->x,
                      w->b, w->r, w->partials, w->ctl);
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials,
+                     grid_blocks(n), 0, &w->ctl->rinf, &w->ctl->r2);
   EC_HIP_TRY(ctx, hipGetLastError());
   HostCtl hc{ctx, w};
   const ec_status st = hc.pull();
@@ -1181,6 +1170,8 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
                      w->hx, w->hy, nullptr, nullptr, w->fx, w->fy, w->ctl);
   hipLaunchKernelGGL(closeout_kernel, dim3(nb), dim3(RB), 0, ctx->stream, G, w->star, w->mom, w->p,
                      w->fx, w->fy, U, lay->row_stride, lay->comp_stride, w->partials, w->ctl);
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials, nb, 0,
+                     &w->ctl->coupled_res, &w->ctl->r2);
   if (p_out) {
     EC_HIP_TRY(ctx, hipMemcpyAsync(p_out, w->p, n * sizeof(double), hipMemcpyDeviceToDevice,
                                    ctx->stream));
@@ -1206,6 +1197,8 @@ extern "C" ec_status ec_imex_dt(ec_ctx* ctx, const double* U, const ec_layout* l
   const long long n = (long long)w->nx * w->ny;
   hipLaunchKernelGGL(imex_rates_kernel, dim3(grid_blocks(n)), dim3(RB), 0, ctx->stream, U,
                      lay->row_stride, lay->comp_stride, ctx->grid, w->partials, w->ctl);
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials,
+                     grid_blocks(n), 1, &w->ctl->mat_max, &w->ctl->aco_max);
   EC_HIP_TRY(ctx, hipGetLastError());
   HostCtl hc{ctx, w};
   if ((st = hc.pull()) != EC_OK) return st;

@@ -47,6 +47,9 @@ def declare(L) -> None:
     L.ec_init_state.restype = st
     L.ec_post_shock_state.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
     L.ec_post_shock_state.restype = st
+    from ._lib import GridDesc
+    L.ec_adjoint_step_host.argtypes = [C.POINTER(GridDesc), dp, dp, dp, C.c_double]
+    L.ec_adjoint_step_host.restype = st
     L.ec_flux_host.argtypes = [dp, dp, C.c_int, C.c_double, C.c_int, C.c_double, dp]
     L.ec_flux_host.restype = st
     L.ec_flux_vjp_host.argtypes = [dp, dp, C.c_int, C.c_double, C.c_int, C.c_double, dp, dp, dp, dp]

@@ -1,0 +1,623 @@
+// euler_compare_amd -- drop-in host driver for the MI355X update path.
+//
+// Reads the reference's inputs files and key=value overrides (ParmParse
+// syntax: `key = v1 v2 ...`, '#' comments, later definitions win, argv[1] is
+// the inputs file when it has no '='), applies the reference's defaults,
+// aliases and checks (core/run_config.hpp:2-55, core/config_reader.hpp:4-327),
+// sets up the test problem on the host, runs the driver loop of
+// app/driver_main.hpp:224-340 with the device entry points of ec_amd.h, and
+// prints the reference's key=value report and final CSVs
+// (app/driver_main.hpp:407-455, io/output_cases.hpp:16-107).
+// Citations are relative to amrex/apps/euler_compare/src/euler_compare/.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ec_amd.h"
+
+namespace {
+
+// ------------------------------------------------------------- ParmParse
+struct Params {
+  std::map<std::string, std::vector<std::string>> kv;
+
+  void add_line(const std::string& raw)
+  {
+    std::string line = raw.substr(0, raw.find('#'));
+    const size_t eq = line.find('=');
+    if (eq == std::string::npos) return;
+    std::string key = trim(line.substr(0, eq));
+    std::istringstream vals(line.substr(eq + 1));
+    std::vector<std::string> v;
+    std::string tok;
+    while (vals >> tok) {
+      if (tok.size() >= 2 && tok.front() == '"' && tok.back() == '"') tok = tok.substr(1, tok.size() - 2);
+      v.push_back(tok);
+    }
+    if (!key.empty()) kv[key] = v;
+  }
+  static std::string trim(const std::string& s)
+  {
+    const size_t a = s.find_first_not_of(" \t\r\n");
+    const size_t b = s.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? "" : s.substr(a, b - a + 1);
+  }
+  bool has(const std::string& k) const { return kv.count(k) && !kv.at(k).empty(); }
+  template <typename T>
+  bool query(const std::string& k, T& out, size_t idx = 0) const
+  {
+    auto it = kv.find(k);
+    if (it == kv.end() || it->second.size() <= idx) return false;
+    std::istringstream s(it->second[idx]);
+    s >> out;
+    return true;
+  }
+  bool query(const std::string& k, std::string& out) const
+  {
+    auto it = kv.find(k);
+    if (it == kv.end() || it->second.empty()) return false;
+    out = it->second[0];
+    for (size_t i = 1; i < it->second.size(); ++i) out += " " + it->second[i];
+    return true;
+  }
+};
+
+[[noreturn]] void abort_msg(const std::string& m)
+{
+  std::cerr << "amrex::Abort::0::" << m << " !!!\n";
+  std::exit(1);
+}
+
+std::string lower(std::string s)
+{
+  std::transform(s.begin(), s.end(), s.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+  return s;
+}
+
+// ---------------------------------------------------------------- config
+struct Cfg {
+  int max_step = 8, max_grid_size = 32, plot_int = 8, toro_test = 1, spatial_order = 1;
+  double stop_time = 0.05, gamma = 1.4, cfl = 0.4, imex_cfl = 0.8, imex_acoustic_cfl_cap = 0.0;
+  double imex_solver_tol = 1.0e-10, velocity[2] = {1.0, 0.5}, density_inner = 2.0,
+         density_outer = 1.0, pressure = 1.0, mach = 0.1, blob_center[2] = {0.35, 0.35},
+         blob_radius = 0.15, vortex_center[2] = {0.0, 0.0}, isentropic_vortex_strength = 5.0,
+         riemann_interface_x = 0.5, riemann_interface_y = 0.5;
+  int imex_solver_max_iter = 100, imex_acoustic_startup = 1, imex_picard_iterations = 2;
+  int n_cell[2] = {64, 64}, is_periodic[2] = {1, 1};
+  double prob_lo[2] = {0.0, 0.0}, prob_hi[2] = {1.0, 1.0};
+  std::string problem = "advection_blob", riemann = "hllc", method = "explicit",
+              imex_form = "bdltv20_t1_s2_source_map_picard", slope_limiter = "minmod",
+              imex_predictor_dissipation = "material", field_boundary = "outflow",
+              imex_pressure_stabilization = "off", bdltv20_paper_t1_s2 = "off", final_csv,
+              imex_gmres_precision = "fp64", snapshot_times = "0,0.06,0.12,0.18,0.24,0.30",
+              snapshot_dir;
+  int imex_gmres_restart = 30;
+  int problem_kind = EC_PROBLEM_ADVECTION_BLOB, riemann_kind = 1, limiter_kind = 0;
+};
+
+// core/config_parse_helpers.hpp:11-42
+int parse_problem(const std::string& v)
+{
+  const std::string l = lower(v);
+  if (l == "advection_blob" || l == "blob") return EC_PROBLEM_ADVECTION_BLOB;
+  if (l == "toro1" || l == "toro_test_1" || l == "sod" || l == "sod_x") return EC_PROBLEM_TORO;
+  if (l == "gresho" || l == "gresho_vortex") return EC_PROBLEM_GRESHO;
+  if (l == "riemann_quadrant" || l == "quadrant_riemann" || l == "liska_wendroff_quadrant") {
+    return EC_PROBLEM_RIEMANN_QUADRANT;
+  }
+  if (l == "isentropic_vortex" || l == "bdltv20_isentropic_vortex") return EC_PROBLEM_ISENTROPIC_VORTEX;
+  if (l == "shock_density_bubble_2d" || l == "same_gamma_shock_density_bubble" ||
+      l == "shock_density_bubble") {
+    return EC_PROBLEM_SHOCK_DENSITY_BUBBLE;
+  }
+  if (l == "shock_bubble_mach") return EC_PROBLEM_SHOCK_BUBBLE_MACH;
+  abort_msg("Unknown euler.problem=" + v);
+}
+
+const char* problem_name(int k)
+{
+  switch (k) {
+    case EC_PROBLEM_TORO: return "toro1";
+    case EC_PROBLEM_GRESHO: return "gresho_vortex";
+    case EC_PROBLEM_RIEMANN_QUADRANT: return "riemann_quadrant";
+    case EC_PROBLEM_ISENTROPIC_VORTEX: return "isentropic_vortex";
+    case EC_PROBLEM_SHOCK_DENSITY_BUBBLE: return "shock_density_bubble_2d";
+    case EC_PROBLEM_SHOCK_BUBBLE_MACH: return "shock_bubble_mach";
+    default: return "advection_blob";
+  }
+}
+
+// core/config_parse_helpers.hpp:44-61, 86-101
+int parse_riemann(const std::string& v)
+{
+  const std::string l = lower(v);
+  if (l == "rusanov" || l == "local_lax_friedrichs" || l == "llf") return EC_RIEMANN_RUSANOV;
+  if (l == "hllc") return EC_RIEMANN_HLLC;
+  if (l == "lowmach_hllc" || l == "lm_hllc") return EC_RIEMANN_LOW_MACH_HLLC;
+  if (l == "xie_am_hllc_p" || l == "am_hllc_p" || l == "xie_hllc_p") return EC_RIEMANN_XIE_AM_HLLC_P;
+  abort_msg("Unknown euler.riemann=" + v);
+}
+
+const char* riemann_name(int k)
+{
+  switch (k) {
+    case 0: return "rusanov";
+    case 2: return "lowmach_hllc";
+    case 3: return "xie_am_hllc_p";
+    default: return "hllc";
+  }
+}
+
+int parse_limiter(const std::string& v)
+{
+  const std::string l = lower(v);
+  if (l == "minmod") return EC_LIMITER_MINMOD;
+  if (l == "mc" || l == "monotonized_central" || l == "monotonized-central") return EC_LIMITER_MC;
+  if (l == "vanleer" || l == "van_leer" || l == "van-leer") return EC_LIMITER_VAN_LEER;
+  abort_msg("Unknown euler.slope_limiter=" + v);
+}
+
+const char* limiter_name(int k) { return k == 1 ? "mc" : (k == 2 ? "vanleer" : "minmod"); }
+
+Cfg read_config(const Params& pp)
+{
+  Cfg c;
+  pp.query("max_step", c.max_step);
+  pp.query("stop_time", c.stop_time);
+  pp.query("amr.n_cell", c.n_cell[0], 0);
+  pp.query("amr.n_cell", c.n_cell[1], 1);
+  pp.query("nx", c.n_cell[0]);
+  pp.query("ny", c.n_cell[1]);
+  pp.query("amr.max_grid_size", c.max_grid_size);
+  pp.query("amr.plot_int", c.plot_int);
+  for (int d = 0; d < 2; ++d) {
+    pp.query("geometry.prob_lo", c.prob_lo[d], d);
+    pp.query("geometry.prob_hi", c.prob_hi[d], d);
+    pp.query("geometry.is_periodic", c.is_periodic[d], d);
+  }
+  pp.query("x_lower", c.prob_lo[0]);
+  pp.query("x_upper", c.prob_hi[0]);
+  pp.query("y_lower", c.prob_lo[1]);
+  pp.query("y_upper", c.prob_hi[1]);
+  pp.query("euler.gamma", c.gamma);
+  pp.query("euler.cfl", c.cfl);
+  pp.query("euler.imex_cfl", c.imex_cfl);
+  pp.query("euler.imex_acoustic_cfl_cap", c.imex_acoustic_cfl_cap);
+  pp.query("euler.imex_solver_tol", c.imex_solver_tol);
+  pp.query("euler.imex_solver_max_iter", c.imex_solver_max_iter);
+  pp.query("euler.imex_acoustic_startup", c.imex_acoustic_startup);
+  pp.query("euler.imex_picard_iterations", c.imex_picard_iterations);
+  pp.query("euler.imex_gmres_precision", c.imex_gmres_precision);
+  pp.query("euler.imex_gmres_restart", c.imex_gmres_restart);
+  pp.query("euler.problem", c.problem);
+  pp.query("euler.riemann", c.riemann);
+  pp.query("euler.method", c.method);
+  pp.query("euler.imex_form", c.imex_form);
+  pp.query("euler.slope_limiter", c.slope_limiter);
+  pp.query("euler.imex_predictor_dissipation", c.imex_predictor_dissipation);
+  pp.query("euler.field_boundary", c.field_boundary);
+  pp.query("euler.imex_pressure_stabilization", c.imex_pressure_stabilization);
+  pp.query("euler.bdltv20_paper_t1_s2", c.bdltv20_paper_t1_s2);
+  pp.query("euler.shock_density_bubble_snapshot_times", c.snapshot_times);
+  pp.query("euler.shock_density_bubble_snapshot_dir", c.snapshot_dir);
+  pp.query("euler.spatial_order", c.spatial_order);
+  pp.query("euler.final_csv", c.final_csv);
+  // root-level aliases (core/config_reader.hpp:100-110)
+  pp.query("case", c.problem);
+  pp.query("problem", c.problem);
+  pp.query("method", c.method);
+  pp.query("riemann", c.riemann);
+  pp.query("order", c.spatial_order);
+  pp.query("spatial_order", c.spatial_order);
+  pp.query("gamma", c.gamma);
+  pp.query("cfl", c.cfl);
+  pp.query("final_time", c.stop_time);
+  pp.query("snapshot_times", c.snapshot_times);
+  for (int d = 0; d < 2; ++d) {
+    pp.query("euler.velocity", c.velocity[d], d);
+    pp.query("euler.blob_center", c.blob_center[d], d);
+    pp.query("euler.vortex_center", c.vortex_center[d], d);
+  }
+  pp.query("euler.density_inner", c.density_inner);
+  pp.query("euler.density_outer", c.density_outer);
+  pp.query("euler.pressure", c.pressure);
+  pp.query("euler.mach", c.mach);
+  pp.query("euler.blob_radius", c.blob_radius);
+  pp.query("euler.isentropic_vortex_strength", c.isentropic_vortex_strength);
+  pp.query("euler.riemann_interface_x", c.riemann_interface_x);
+  pp.query("euler.riemann_interface_y", c.riemann_interface_y);
+  pp.query("prob.test", c.toro_test);
+  pp.query("prob.x0", c.riemann_interface_x);
+  pp.query("prob.y0", c.riemann_interface_y);
+
+  c.problem_kind = parse_problem(c.problem);
+  c.riemann_kind = parse_riemann(c.riemann);
+  c.limiter_kind = parse_limiter(c.slope_limiter);
+  c.method = lower(c.method);
+  c.field_boundary = lower(c.field_boundary);
+  if (c.method != "explicit" && c.method != "imex") abort_msg("Unknown euler.method=" + c.method);
+  if (c.field_boundary != "outflow" && c.field_boundary != "exact_dirichlet") {
+    abort_msg("euler.field_boundary must be outflow or exact_dirichlet.");
+  }
+  if (c.spatial_order != 1 && c.spatial_order != 2) abort_msg("euler.spatial_order must be 1 or 2.");
+  if (c.imex_picard_iterations < 1) abort_msg("euler.imex_picard_iterations must be at least one.");
+  if (c.bdltv20_paper_t1_s2 != "off") {
+    abort_msg("euler.bdltv20_paper_t1_s2 (the direct BDLTV20 driver) is not part of this build.");
+  }
+  if (c.field_boundary == "exact_dirichlet" && (!c.is_periodic[0] || !c.is_periodic[1])) {
+    abort_msg("euler.field_boundary=exact_dirichlet is not part of this build.");
+  }
+  if (c.method == "imex") {
+    // core/config_reader.hpp:177-196
+    if (c.spatial_order != 2) abort_msg("euler.method=imex requires euler.spatial_order=2 in this code pack.");
+    if (c.imex_pressure_stabilization != "off") {
+      abort_msg("IMEX rows in this code pack use euler.imex_pressure_stabilization=off.");
+    }
+    if (lower(c.imex_predictor_dissipation) != "material") {
+      abort_msg("IMEX rows in this code pack use euler.imex_predictor_dissipation=material.");
+    }
+    if (c.limiter_kind != 0) abort_msg("IMEX rows in this code pack use euler.slope_limiter=minmod.");
+    if (c.imex_acoustic_startup != 0 || c.imex_acoustic_cfl_cap != 0.0) {
+      abort_msg("IMEX rows require material dissipation, acoustic startup off, acoustic cap zero, "
+                "and no pressure stabilization.");
+    }
+  }
+  if (c.problem_kind == EC_PROBLEM_SHOCK_DENSITY_BUBBLE) {
+    // the reference's fixed shock-bubble geometry (core/config_reader.hpp:288-295)
+    c.prob_lo[0] = 0.0; c.prob_lo[1] = 0.0; c.prob_hi[0] = 2.0; c.prob_hi[1] = 0.5;
+    c.is_periodic[0] = 0; c.is_periodic[1] = 0;
+  }
+  return c;
+}
+
+std::vector<double> parse_times(const std::string& s, double stop)
+{
+  std::vector<double> t;
+  std::string tok;
+  std::istringstream in(s);
+  while (std::getline(in, tok, ',')) {
+    tok = Params::trim(tok);
+    if (tok.empty()) continue;
+    const double v = std::atof(tok.c_str());
+    if (v >= 0.0 && v <= stop + 1.0e-13) t.push_back(v);
+  }
+  std::sort(t.begin(), t.end());
+  return t;
+}
+
+#define HIP_OK(x)                                                          \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess) abort_msg(std::string("HIP: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define EC_OK_OR_DIE(ctx, x)                                               \
+  do {                                                                     \
+    ec_status s_ = (x);                                                    \
+    if (s_ != EC_OK) abort_msg(std::string(#x) + ": " + ec_status_string(s_) + " " + ec_last_error(ctx)); \
+  } while (0)
+
+const char* imex_status_name(int s)
+{
+  static const char* n[] = {"converged_absolute", "converged_relative", "linear_solve_not_converged",
+                            "gmres_breakdown_not_converged", "density_denominator_floor_used",
+                            "source_map_pressure_row_nonfinite", "linear_residual_nonfinite",
+                            "source_map_lagged_momentum_nonfinite", "not_run"};
+  return (s >= 0 && s <= 8) ? n[s] : "unknown";
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+  Params pp;
+  int first = 1;
+  if (argc > 1 && std::string(argv[1]).find('=') == std::string::npos) {
+    std::ifstream in(argv[1]);
+    if (!in) abort_msg(std::string("cannot open inputs file ") + argv[1]);
+    std::string line;
+    while (std::getline(in, line)) pp.add_line(line);
+    first = 2;
+  }
+  for (int a = first; a < argc; ++a) pp.add_line(argv[a]);
+  const Cfg cfg = read_config(pp);
+  const int nx = cfg.n_cell[0], ny = cfg.n_cell[1];
+  const double dx = (cfg.prob_hi[0] - cfg.prob_lo[0]) / nx;
+  const double dy = (cfg.prob_hi[1] - cfg.prob_lo[1]) / ny;
+  const bool sb = cfg.problem_kind == EC_PROBLEM_SHOCK_DENSITY_BUBBLE;
+
+  // initial data on the host
+  ec_problem_desc pd;
+  ec_problem_defaults(cfg.problem_kind, &pd);
+  pd.nx = nx;
+  pd.ny = ny;
+  for (int d = 0; d < 2; ++d) {
+    pd.prob_lo[d] = cfg.prob_lo[d];
+    pd.prob_hi[d] = cfg.prob_hi[d];
+    pd.velocity[d] = cfg.velocity[d];
+    pd.blob_center[d] = cfg.blob_center[d];
+    pd.vortex_center[d] = cfg.vortex_center[d];
+  }
+  pd.gamma = cfg.gamma;
+  pd.toro_test = cfg.toro_test;
+  pd.riemann_interface_x = cfg.riemann_interface_x;
+  pd.riemann_interface_y = cfg.riemann_interface_y;
+  pd.density_inner = cfg.density_inner;
+  pd.density_outer = cfg.density_outer;
+  pd.pressure = cfg.pressure;
+  pd.mach = cfg.mach;
+  pd.blob_radius = cfg.blob_radius;
+  pd.isentropic_vortex_strength = cfg.isentropic_vortex_strength;
+  const size_t n = (size_t)nx * ny;
+  std::vector<double> host(4 * n);
+  ec_layout lay{nx, (long long)n};
+  if (ec_init_state(&pd, host.data(), &lay) != EC_OK) abort_msg("Unsupported Toro test id.");
+
+  ec_grid_desc gd{};
+  gd.nx = nx;
+  gd.ny = ny;
+  gd.dx = dx;
+  gd.dy = dy;
+  gd.gamma = cfg.gamma;
+  gd.riemann = cfg.riemann_kind;
+  gd.limiter = cfg.limiter_kind;
+  gd.spatial_order = cfg.spatial_order;
+  for (int d = 0; d < 2; ++d) {
+    gd.bc[2 * d] = gd.bc[2 * d + 1] = cfg.is_periodic[d] ? EC_BC_PERIODIC : EC_BC_OUTFLOW;
+  }
+  if (sb) {
+    // problems/boundary_conditions.hpp:27-73
+    const double q[4] = {2.8181818181818183, 1.6064386578049976, 0.0, 5.0};
+    gd.bc[0] = EC_BC_FIXED;
+    gd.bc[1] = EC_BC_OUTFLOW;
+    gd.bc[2] = EC_BC_REFLECT;
+    gd.bc[3] = EC_BC_OUTFLOW;
+    const double kin = 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2]);
+    gd.fixed_state[0][0] = q[0];
+    gd.fixed_state[0][1] = q[0] * q[1];
+    gd.fixed_state[0][2] = q[0] * q[2];
+    gd.fixed_state[0][3] = q[3] / (cfg.gamma - 1.0) + kin;
+  }
+
+  HIP_OK(hipSetDevice(0));
+  ec_ctx* ctx = nullptr;
+  {
+    const ec_status st = ec_create(&gd, nullptr, &ctx);
+    if (st != EC_OK) abort_msg(std::string("ec_create: ") + ec_status_string(st));
+  }
+  double *A = nullptr, *B = nullptr;
+  HIP_OK(hipMalloc(&A, 4 * n * sizeof(double)));
+  HIP_OK(hipMalloc(&B, 4 * n * sizeof(double)));
+  HIP_OK(hipMemcpy(A, host.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
+
+  const double volume = dx * dy;
+  auto sums = [&](const std::vector<double>& U, double& mass, double& energy) {
+    mass = 0.0;
+    energy = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+      mass += U[k];
+      energy += U[3 * n + k];
+    }
+    mass *= volume;
+    energy *= volume;
+  };
+  double m0, e0;
+  sums(host, m0, e0);
+
+  std::string status = "ok", failure = "none";
+  int step = 0, imex_steps = 0;
+  long long degenerate = 0;
+  double time = 0.0;
+  ec_step_diag diag{};
+  EC_OK_OR_DIE(ctx, ec_state_diagnostics(ctx, A, &lay, &diag));
+  const auto t_start = std::chrono::steady_clock::now();
+  if (diag.nonfinite_count > 0 || diag.rho_min <= 0.0 || diag.pressure_min <= 0.0) {
+    status = "failed";
+    failure = sb ? "stage_nonphysical" : "initial_nonphysical_state";
+  }
+  std::vector<double> snaps = sb ? parse_times(cfg.snapshot_times, cfg.stop_time) : std::vector<double>{};
+  size_t next_snap = 0;
+  while (next_snap < snaps.size() && snaps[next_snap] <= time + 1.0e-13) ++next_snap;
+  ec_imex_diag last_imex{};
+  int imex_iter_max = 0;
+  double imex_res_max = 0.0, imex_rel_max = 0.0, mat_max_run = 0.0, aco_max_run = 0.0;
+  ec_imex_opts io;
+  ec_imex_default_opts(&io);
+  io.picard_iterations = cfg.imex_picard_iterations;
+  io.tol = cfg.imex_solver_tol;
+  io.max_iter = cfg.imex_solver_max_iter;
+  io.restart = cfg.imex_gmres_restart;
+  io.precision = lower(cfg.imex_gmres_precision) == "mixed" ? EC_IMEX_PRECISION_MIXED : EC_IMEX_PRECISION_FP64;
+
+  const bool device_loop = cfg.method == "explicit" && snaps.empty();
+  if (device_loop && status == "ok") {
+    // the explicit branch entirely on the device, in chunks
+    ec_run_state rs{};
+    rs.stop_time = cfg.stop_time;
+    rs.cfl = cfg.cfl;
+    rs.max_step = cfg.max_step;
+    while (true) {
+      int in_b = 0;
+      const int before = rs.step;
+      EC_OK_OR_DIE(ctx, ec_explicit_advance(ctx, A, B, &lay, 512, &rs, &in_b));
+      if (in_b) std::swap(A, B);
+      if (rs.status != 0 || rs.step >= cfg.max_step || rs.time >= cfg.stop_time - 1.0e-14 ||
+          rs.step == before) {
+        break;
+      }
+    }
+    step = rs.step;
+    time = rs.time;
+    degenerate = rs.degenerate_total;
+    diag.rho_min = rs.rho_min;
+    diag.pressure_min = rs.pressure_min;
+    diag.nonfinite_count = rs.nonfinite_count;
+    static const char* fc[] = {"none", "initial_nonphysical_state", "invalid_time_step",
+                               "nonfinite_state", "nonpositive_density", "nonpositive_pressure"};
+    if (rs.status != 0) {
+      status = "failed";
+      failure = fc[rs.status];
+    }
+  } else {
+    // app/driver_main.hpp:224-333 with a host dt (snapshot clipping, IMEX)
+    while (status == "ok" && step < cfg.max_step && time < cfg.stop_time - 1.0e-14) {
+      double dt = cfg.stop_time - time;
+      if (cfg.method == "imex") {
+        double mm = 0.0, am = 0.0;
+        EC_OK_OR_DIE(ctx, ec_imex_dt(ctx, A, &lay, cfg.imex_cfl, time, cfg.stop_time,
+                                     cfg.imex_acoustic_startup, cfg.imex_acoustic_cfl_cap, &dt, &mm, &am));
+        mat_max_run = std::max(mat_max_run, mm);
+        aco_max_run = std::max(aco_max_run, am);
+      } else {
+        EC_OK_OR_DIE(ctx, ec_state_diagnostics(ctx, A, &lay, &diag));
+        dt = ec_euler_dt(diag.max_rate, cfg.cfl, time, cfg.stop_time);
+      }
+      if (!std::isfinite(dt) || dt <= 0.0) {
+        status = "failed";
+        failure = sb ? "invalid_cfl_dt" : "invalid_time_step";
+        break;
+      }
+      if (next_snap < snaps.size()) {
+        const double ts = snaps[next_snap];
+        if (ts > time + 1.0e-13 && time + dt > ts + 1.0e-13) dt = ts - time;
+      }
+      if (cfg.method == "imex") {
+        EC_OK_OR_DIE(ctx, ec_imex_step(ctx, A, &lay, dt, &io, &last_imex, nullptr));
+        ++imex_steps;
+        imex_iter_max = std::max(imex_iter_max, last_imex.iterations);
+        imex_res_max = std::max(imex_res_max, last_imex.residual_linf);
+        imex_rel_max = std::max(imex_rel_max, last_imex.relative_residual_linf);
+        if (last_imex.status != EC_IMEX_CONVERGED_ABSOLUTE && last_imex.status != EC_IMEX_CONVERGED_RELATIVE) {
+          status = "failed";
+          failure = sb ? "imex_pressure_solve_nonconverged"
+                       : std::string("imex_linear_solver_") + imex_status_name(last_imex.status);
+          break;
+        }
+      } else {
+        ec_step_diag sd{};
+        EC_OK_OR_DIE(ctx, ec_explicit_step(ctx, A, B, &lay, dt, &sd));
+        std::swap(A, B);
+        degenerate += sd.degenerate_count;
+      }
+      time += dt;
+      ++step;
+      EC_OK_OR_DIE(ctx, ec_state_diagnostics(ctx, A, &lay, &diag));
+      if (diag.nonfinite_count > 0) {
+        status = "failed";
+        failure = "nonfinite_state";
+      } else if (diag.rho_min <= 0.0) {
+        status = "failed";
+        failure = "nonpositive_density";
+      } else if (diag.pressure_min <= 0.0) {
+        status = "failed";
+        failure = "nonpositive_pressure";
+      }
+      while (next_snap < snaps.size() && snaps[next_snap] <= time + 1.0e-13) ++next_snap;
+    }
+  }
+  if (status == "ok" && step >= cfg.max_step && time < cfg.stop_time - 1.0e-14) {
+    status = sb ? "failed" : "capped_max_step";
+    failure = sb ? "max_steps_exceeded" : "max_step_before_stop_time";
+  }
+  if (sb && status == "ok") failure = "ok";
+  HIP_OK(hipMemcpy(host.data(), A, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+  const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  double m1, e1;
+  sums(host, m1, e1);
+
+  // final CSVs (io/output_cases.hpp:16-107)
+  if (!cfg.final_csv.empty() && (cfg.problem_kind == EC_PROBLEM_TORO || cfg.problem_kind == EC_PROBLEM_GRESHO)) {
+    std::ofstream out(cfg.final_csv);
+    out << std::setprecision(17);
+    const double g = cfg.gamma;
+    auto prim = [&](size_t k, double* q) {
+      const double inv = 1.0 / host[k];
+      q[0] = host[k];
+      q[1] = host[n + k] * inv;
+      q[2] = host[2 * n + k] * inv;
+      const double kin = 0.5 * host[k] * (q[1] * q[1] + q[2] * q[2]);
+      q[3] = (g - 1.0) * (host[3 * n + k] - kin);
+    };
+    if (cfg.problem_kind == EC_PROBLEM_TORO) {
+      out << "i,j,x,y,rho,u,v,p,exact_rho,exact_u,exact_v,exact_p,rho_error,u_error,v_error,p_error\n";
+      for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+          double q[4];
+          prim((size_t)j * nx + i, q);
+          out << i << ',' << j << ',' << cfg.prob_lo[0] + (i + 0.5) * dx << ',' << cfg.prob_lo[1] + (j + 0.5) * dy
+              << ',' << q[0] << ',' << q[1] << ',' << q[2] << ',' << q[3]
+              << ",nan,nan,nan,nan,nan,nan,nan,nan\n";
+        }
+      }
+    } else {
+      out << "x,y,rho,u,v,pressure\n";
+      for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+          double q[4];
+          prim((size_t)j * nx + i, q);
+          out << cfg.prob_lo[0] + (i + 0.5) * dx << ',' << cfg.prob_lo[1] + (j + 0.5) * dy << ',' << q[0]
+              << ',' << q[1] << ',' << q[2] << ',' << q[3] << '\n';
+        }
+      }
+    }
+  }
+
+  const bool imex = cfg.method == "imex";
+  std::cout << "project_amrex_euler_compare_status=" << status << "\n";
+  std::cout << "failure_category=" << failure << "\n";
+  std::cout << "problem=" << problem_name(cfg.problem_kind) << "\n";
+  std::cout << "method=" << cfg.method << "\n";
+  std::cout << "riemann=" << (imex ? "not_applicable" : riemann_name(cfg.riemann_kind)) << "\n";
+  std::cout << "imex_form=" << (imex ? "bdltv20_t1_s2_source_map_picard" : "not_applicable") << "\n";
+  std::cout << "slope_limiter=" << limiter_name(cfg.limiter_kind) << "\n";
+  std::cout << "spatial_order=" << cfg.spatial_order << "\n";
+  std::cout << "toro_test=" << cfg.toro_test << "\n";
+  std::cout << "field_boundary=" << cfg.field_boundary << "\n";
+  std::cout << "geometry_is_periodic=" << cfg.is_periodic[0] << " " << cfg.is_periodic[1] << "\n";
+  std::cout << "completed_steps=" << step << "\n";
+  std::cout << "completed_time=" << time << "\n";
+  std::cout << "target_time=" << cfg.stop_time << "\n";
+  std::cout << "max_step=" << cfg.max_step << "\n";
+  std::cout << "wall_time_sec=" << wall << "\n";
+  std::cout << "initial_mass=" << m0 << "\n";
+  std::cout << "final_mass=" << m1 << "\n";
+  std::cout << "mass_drift=" << m1 - m0 << "\n";
+  std::cout << "initial_energy=" << e0 << "\n";
+  std::cout << "final_energy=" << e1 << "\n";
+  std::cout << "energy_drift=" << e1 - e0 << "\n";
+  std::cout << "rho_min=" << diag.rho_min << "\n";
+  std::cout << "pressure_min=" << diag.pressure_min << "\n";
+  std::cout << "nonfinite_count=" << diag.nonfinite_count << "\n";
+  std::cout << "hllc_degenerate_star_count=" << degenerate << "\n";
+  std::cout << "imex_step_count=" << imex_steps << "\n";
+  if (imex) {
+    std::cout << "imex_cfl=" << cfg.imex_cfl << "\n";
+    std::cout << "imex_timestep_material_rate_max=" << mat_max_run << "\n";
+    std::cout << "imex_timestep_acoustic_rate_max=" << aco_max_run << "\n";
+    std::cout << "imex_solver=device_gmres_matrix_free_source_map_pressure_row\n";
+    std::cout << "imex_solver_status=" << imex_status_name(last_imex.status) << "\n";
+    std::cout << "imex_solver_iterations_last=" << last_imex.iterations << "\n";
+    std::cout << "imex_solver_iterations_max=" << imex_iter_max << "\n";
+    std::cout << "imex_solver_residual_last=" << last_imex.residual_linf << "\n";
+    std::cout << "imex_solver_residual_max=" << imex_res_max << "\n";
+    std::cout << "imex_solver_relative_residual_last=" << last_imex.relative_residual_linf << "\n";
+    std::cout << "imex_solver_relative_residual_max=" << imex_rel_max << "\n";
+    std::cout << "imex_gmres_precision=" << (io.precision ? "mixed" : "fp64") << "\n";
+    std::cout << "imex_picard_pressure_solves_used=" << last_imex.picard_solves << "\n";
+  }
+  HIP_OK(hipFree(A));
+  HIP_OK(hipFree(B));
+  ec_destroy(ctx);
+  return status == "failed" ? 2 : 0;
+}

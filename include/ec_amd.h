@@ -253,6 +253,10 @@ ec_status ec_adjoint_checkpointed(ec_ctx* ctx, const double* checkpoints, const 
  * finite differences without a GPU. */
 ec_status ec_flux_host(const double* qL, const double* qR, int axis, double gamma, int riemann,
                        double sensor_f, double* F);
+/* The four adjoint passes of ec_adjoint_step run on the host, on host
+ * memory (compact layout), with the device code's arithmetic. */
+ec_status ec_adjoint_step_host(const ec_grid_desc* desc, const double* U_n,
+                               const double* lambda_next, double* lambda_n, double dt);
 ec_status ec_flux_vjp_host(const double* qL, const double* qR, int axis, double gamma,
                            int riemann, double sensor_f, const double* Fbar, double* qLbar,
                            double* qRbar, double* sensor_bar);

@@ -1,0 +1,81 @@
+"""The drop-in driver executable (euler_compare_amd) behind the reference's
+inputs syntax: it must reproduce the reference's Sod row (347 steps) and
+write the final CSV of the oracle's state."""
+import csv
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+EXE = ROOT / "amrex-euler-allmach-explicit-imex_amd" / "euler_compare_amd"
+
+# The reference's smoke inputs layout (key = value lines, '#' comments).
+INPUTS = """# smoke inputs
+max_step = 8
+stop_time = 0.05
+amr.n_cell = 64 64
+amr.max_grid_size = 32
+amr.plot_int = 8
+geometry.prob_lo = 0.0 0.0
+geometry.prob_hi = 1.0 1.0
+geometry.is_periodic = 1 1
+euler.gamma = 1.4
+euler.cfl = 0.4
+"""
+
+
+def run(tmp_path, *overrides):
+    inp = tmp_path / "inputs"
+    inp.write_text(INPUTS)
+    r = subprocess.run([str(EXE), str(inp), *overrides], capture_output=True, text=True, timeout=120)
+    kv = dict(line.split("=", 1) for line in r.stdout.splitlines() if "=" in line)
+    return r.returncode, kv, r.stderr
+
+
+def test_sod_row(tmp_path, ecor):
+    out_csv = tmp_path / "sod.csv"
+    rc, kv, err = run(tmp_path, "amr.n_cell=400 4", "geometry.prob_hi=1 0.01",
+                      "geometry.is_periodic=0 1", "euler.problem=toro1", "prob.test=1",
+                      "euler.cfl=0.8", "stop_time=0.2", "max_step=100000", "euler.method=explicit",
+                      "euler.spatial_order=2", "euler.slope_limiter=minmod", "euler.riemann=hllc",
+                      f"euler.final_csv={out_csv}")
+    assert rc == 0, err
+    assert kv["project_amrex_euler_compare_status"] == "ok"
+    assert kv["completed_steps"] == "347"
+    assert abs(float(kv["mass_drift"])) < 1e-15
+    g = ecor.make_grid(400, 4, 1 / 400, 0.01 / 4, riemann=ecor.HLLC, bc=(1, 1, 0, 0))
+    ref, res, _ = ecor.run_explicit(g, ecor.init_toro(400, 4, [1, 0, 0, 1], [0.125, 0, 0, 0.1]),
+                                    0.8, 0.2, 100000)
+    rows = list(csv.DictReader(open(out_csv)))
+    assert len(rows) == 1600
+    rho = np.array([float(r["rho"]) for r in rows]).reshape(4, 400)
+    assert np.array_equal(rho, ref[0])
+
+
+def test_capped_and_failure_rows(tmp_path):
+    rc, kv, _ = run(tmp_path, "euler.problem=gresho_vortex", "euler.vortex_center=0.5 0.5",
+                    "euler.mach=0.01", "euler.pressure=7142.857142857143", "euler.riemann=xie_am_hllc_p",
+                    "euler.spatial_order=2", "max_step=5", "stop_time=1.0")
+    assert rc == 0 and kv["project_amrex_euler_compare_status"] == "capped_max_step"
+    assert kv["completed_steps"] == "5" and kv["riemann"] == "xie_am_hllc_p"
+    rc, kv, _ = run(tmp_path, "euler.problem=gresho_vortex", "euler.pressure=-1.0", "max_step=5")
+    assert rc == 2 and kv["failure_category"] == "initial_nonphysical_state"
+    r = subprocess.run([str(EXE), "euler.riemann=bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Unknown euler.riemann=bogus" in r.stderr
+
+
+def test_imex_row(tmp_path):
+    rc, kv, err = run(tmp_path, "amr.n_cell=64 64", "euler.problem=gresho_vortex",
+                      "euler.vortex_center=0.5 0.5", "euler.mach=0.001",
+                      "euler.pressure=714285.7142857143", "euler.method=imex", "euler.spatial_order=2",
+                      "euler.imex_acoustic_startup=0", "euler.imex_solver_tol=1e-10",
+                      "euler.imex_solver_max_iter=1000", "max_step=3", "stop_time=1.0",
+                      "euler.imex_gmres_precision=mixed")
+    assert rc == 0, err
+    assert kv["imex_step_count"] == "3"
+    assert kv["imex_solver_status"] in ("converged_absolute", "converged_relative")
+    assert float(kv["imex_solver_relative_residual_max"]) <= 1e-10
