@@ -20,6 +20,8 @@
 // Workgroups are numbered so that the 8 XCDs each receive a contiguous band
 // of tile rows: neighbouring tiles then share halo rows through one L2.
 
+#include <type_traits>
+
 #include "ec_context.hpp"
 #include "ec_ghost.hpp"
 #include "ec_state.hpp"
@@ -147,10 +149,20 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const Partial* partials, i
                                                       int mode, double* dt_log)
 {
   __shared__ Red scratch[NT / 64];
+  // Every operand of the combine is order independent (max, min, integer
+  // sums), so the loads are batched: kLoads partials in flight per thread.
+  constexpr int kLoads = 8;
   Red r = red_identity();
-  for (int p = threadIdx.x; p < nparts; p += NT) {
-    const Partial q = partials[p];
-    red_combine(r, Red{q.rate, q.rho_min, q.p_min, q.bad, q.degenerate});
+  for (int base = 0; base < nparts; base += NT * kLoads) {
+    Partial q[kLoads];
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+      const int p = base + u * NT + (int)threadIdx.x;
+      q[u] = p < nparts ? partials[p] : Partial{-INFINITY, INFINITY, INFINITY, 0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u)
+      red_combine(r, Red{q[u].rate, q[u].rho_min, q[u].p_min, q[u].bad, q[u].degenerate});
   }
   r = block_reduce<NT>(r, scratch);
   if (threadIdx.x != 0) return;
@@ -203,8 +215,6 @@ __global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
   constexpr int NYF = TX * (TY + 1);                           // flux y-faces [0,TX-1]x[0,TY]
   constexpr int XRX = TX + 1, NXR = XRX * (TY + 2);            // x ratios [0,TX]x[-1,TY]
   constexpr int YRX = TX + 2, NYR = YRX * (TY + 1);            // y ratios [-1,TX]x[0,TY]
-  constexpr int NF = NXF + NYF;
-  constexpr int NP = (NF + NT - 1) / NT;
   constexpr bool HP = RIEMANN == 3;
   constexpr bool O2 = ORDER == 2;
   constexpr int OFF_Q = 0;
@@ -319,69 +329,87 @@ __global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
   auto rx = [&](int fi, int j) { return lds[OFF_RX + (j + 1) * XRX + fi]; };
   auto ry = [&](int i, int fj) { return lds[OFF_RY + fj * YRX + (i + 1)]; };
 
-  // D. every face flux of the tile once.
+  // D. every face flux of the tile once: all x-faces, then all y-faces, so
+  // that the axis is a compile-time constant of each pass and the selects on
+  // it fold away.
   const int lim_x = g.nx - i0, lim_y = g.ny - j0;  // cells of this tile inside the domain
-  Cons flux[NP];
-  unsigned char dflag[NP];
-#pragma unroll
-  for (int p = 0; p < NP; ++p) {
-    const int f = tid + p * NT;
-    dflag[p] = 0;
-    if (f < NF) {
-      int axis, li, lj, ri, rj;
-      bool needed;
-      double sensor = 1.0;
-      if (f < NXF) {
-        const int fi = f % XFX, j = f / XFX;
-        axis = 0; li = fi - 1; lj = j; ri = fi; rj = j;
-        needed = fi <= lim_x && j < lim_y;
-        if constexpr (HP) {
-          double m = rx(fi, j);
-          m = smin(m, ry(fi - 1, j));
-          m = smin(m, ry(fi - 1, j + 1));
-          m = smin(m, ry(fi, j));
-          m = smin(m, ry(fi, j + 1));
-          const double c = clamp01(m);
-          sensor = c * c * c;
-        }
-      } else {
-        const int s = f - NXF;
-        const int i = s % TX, fj = s / TX;
-        axis = 1; li = i; lj = fj - 1; ri = i; rj = fj;
-        needed = i < lim_x && fj <= lim_y;
-        if constexpr (HP) {
-          double m = ry(i, fj);
-          m = smin(m, rx(i, fj - 1));
-          m = smin(m, rx(i + 1, fj - 1));
-          m = smin(m, rx(i, fj));
-          m = smin(m, rx(i + 1, fj));
-          const double c = clamp01(m);
-          sensor = c * c * c;
-        }
+  constexpr int NPX = (NXF + NT - 1) / NT, NPY = (NYF + NT - 1) / NT;
+  auto face = [&](auto axis_c, int f, Cons& out, unsigned char& dflag) {
+    constexpr int AX = decltype(axis_c)::value;
+    int li, lj, ri, rj;
+    bool needed;
+    double sensor = 1.0;
+    if constexpr (AX == 0) {
+      const int fi = f % XFX, j = f / XFX;
+      li = fi - 1; lj = j; ri = fi; rj = j;
+      needed = fi <= lim_x && j < lim_y;
+      if constexpr (HP) {
+        double m = rx(fi, j);
+        m = smin(m, ry(fi - 1, j));
+        m = smin(m, ry(fi - 1, j + 1));
+        m = smin(m, ry(fi, j));
+        m = smin(m, ry(fi, j + 1));
+        const double c = clamp01(m);
+        sensor = c * c * c;
       }
-      if (needed) {
-        const Prim L = face_state(li, lj, axis, 0.5);
-        const Prim R = face_state(ri, rj, axis, -0.5);
-        flux[p] = interface_flux(L, R, axis, gamma, RIEMANN, sensor);
-        if constexpr (RIEMANN != 0) dflag[p] = face_degenerate(L, R, axis, gamma, RIEMANN) ? 1 : 0;
-      } else {
-        flux[p] = Cons{0.0, 0.0, 0.0, 0.0};
+    } else {
+      const int i = f % TX, fj = f / TX;
+      li = i; lj = fj - 1; ri = i; rj = fj;
+      needed = i < lim_x && fj <= lim_y;
+      if constexpr (HP) {
+        double m = ry(i, fj);
+        m = smin(m, rx(i, fj - 1));
+        m = smin(m, rx(i + 1, fj - 1));
+        m = smin(m, rx(i, fj));
+        m = smin(m, rx(i + 1, fj));
+        const double c = clamp01(m);
+        sensor = c * c * c;
       }
     }
+    dflag = 0;
+    out = Cons{0.0, 0.0, 0.0, 0.0};
+    if (needed) {
+      const Prim L = face_state(li, lj, AX, 0.5);
+      const Prim R = face_state(ri, rj, AX, -0.5);
+      out = interface_flux(L, R, AX, gamma, RIEMANN, sensor);
+      if constexpr (RIEMANN != 0) dflag = face_degenerate(L, R, AX, gamma, RIEMANN) ? 1 : 0;
+    }
+  };
+  Cons fxv[NPX], fyv[NPY];
+  unsigned char dfx[NPX], dfy[NPY];
+#pragma unroll
+  for (int p = 0; p < NPX; ++p) {
+    const int f = tid + p * NT;
+    if (f < NXF) face(std::integral_constant<int, 0>{}, f, fxv[p], dfx[p]);
+  }
+#pragma unroll
+  for (int p = 0; p < NPY; ++p) {
+    const int f = tid + p * NT;
+    if (f < NYF) face(std::integral_constant<int, 1>{}, f, fyv[p], dfy[p]);
   }
   __syncthreads();  // stage A-C storage is dead from here on
 #pragma unroll
-  for (int p = 0; p < NP; ++p) {
+  for (int p = 0; p < NPX; ++p) {
     const int f = tid + p * NT;
-    if (f < NF) {
-      double* base = f < NXF ? lds + OFF_FX + f : lds + OFF_FY + (f - NXF);
-      const int stride = f < NXF ? NXF : NYF;
-      base[0] = flux[p].r;
-      base[stride] = flux[p].mx;
-      base[2 * stride] = flux[p].my;
-      base[3 * stride] = flux[p].E;
-      if (f < NXF) degx[f] = dflag[p];
-      else degy[f - NXF] = dflag[p];
+    if (f < NXF) {
+      double* base = lds + OFF_FX + f;
+      base[0] = fxv[p].r;
+      base[NXF] = fxv[p].mx;
+      base[2 * NXF] = fxv[p].my;
+      base[3 * NXF] = fxv[p].E;
+      degx[f] = dfx[p];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NPY; ++p) {
+    const int f = tid + p * NT;
+    if (f < NYF) {
+      double* base = lds + OFF_FY + f;
+      base[0] = fyv[p].r;
+      base[NYF] = fyv[p].mx;
+      base[2 * NYF] = fyv[p].my;
+      base[3 * NYF] = fyv[p].E;
+      degy[f] = dfy[p];
     }
   }
   __syncthreads();
@@ -430,6 +458,8 @@ __global__ void __launch_bounds__(NT)
 }
 
 // ---------------------------------------------------------------- dispatch
+constexpr int kFinalizeThreads = 1024;
+
 struct TileShape {
   int tx, ty, nt;
 };
@@ -501,8 +531,8 @@ ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, lon
   }
   step_fn(ctx->tile_id, ctx->grid.riemann, ctx->grid.order)(a, dim3(a.nblocks), dim3(s.nt),
                                                              ctx->stream);
-  hipLaunchKernelGGL((finalize_kernel<256>), dim3(1), dim3(256), 0, ctx->stream, ctx->partials,
-                     a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
+  hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0,
+                     ctx->stream, ctx->partials, a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
   EC_HIP_TRY(ctx, hipGetLastError());
   return EC_OK;
 }
@@ -521,8 +551,8 @@ ec_status launch_state_reduce(ec_ctx* ctx, const double* u, long long rs, long l
   const int nb = reduce_blocks(ctx);
   hipLaunchKernelGGL((state_reduce_kernel<256>), dim3(nb), dim3(256), 0, ctx->stream, u, rs, cs,
                      ctx->grid, ctx->ctrl, ctx->partials, mode);
-  hipLaunchKernelGGL((finalize_kernel<256>), dim3(1), dim3(256), 0, ctx->stream, ctx->partials, nb,
-                     ctx->ctrl, mode, nullptr);
+  hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0,
+                     ctx->stream, ctx->partials, nb, ctx->ctrl, mode, nullptr);
   EC_HIP_TRY(ctx, hipGetLastError());
   return EC_OK;
 }
