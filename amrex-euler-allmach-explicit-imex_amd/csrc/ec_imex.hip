@@ -40,6 +40,7 @@ constexpr int MAXK = 64;       // largest GMRES restart length
 // ------------------------------------------------------------ device control
 struct GmCtl {
   int done, kk, breakdown, bad;
+  int mlim, pad0;              // iterations allowed in this cycle
   double thr, est, beta, inv_norm;
   double rinf, r2;             // true residual norms
   double dots1[MAXK], dots2[MAXK], nrm2;
@@ -47,6 +48,9 @@ struct GmCtl {
   int floor_count, nonfinite, guard_count, mom_bad;
   double coupled_res, mat_max, aco_max;
 };
+
+template <typename T>
+struct OpArgs;
 
 struct ImexWork {
   int nx = 0, ny = 0, m = 0;
@@ -66,8 +70,6 @@ struct ImexWork {
   float* V32 = nullptr;     // (m+1) n
   double* w = nullptr;
   float* w32 = nullptr;
-  float* b32 = nullptr;
-  float* x32 = nullptr;
   double* R = nullptr;      // (m+1) x m Hessenberg / triangular factor
   double* g = nullptr;      // m+1
   double* cs = nullptr;
@@ -77,6 +79,16 @@ struct ImexWork {
   int nparts = 0;
   GmCtl* ctl = nullptr;
   GmCtl* ctl_host = nullptr;
+  // Restart cycles run as hipGraphs (one per precision) on a private stream;
+  // the operator arguments they read live in device memory.
+  OpArgs<double>* dA64 = nullptr;
+  OpArgs<float>* dA32 = nullptr;
+  OpArgs<double>* hA64 = nullptr;   // pinned staging of the two blocks
+  OpArgs<float>* hA32 = nullptr;
+  hipStream_t gs = nullptr;
+  hipEvent_t ev = nullptr;
+  hipGraphExec_t exec64 = nullptr;
+  hipGraphExec_t exec32 = nullptr;
 };
 
 void imex_release(ec_ctx* ctx)
@@ -85,12 +97,18 @@ void imex_release(ec_ctx* ctx)
   if (!w) return;
   void* ptrs[] = {w->star, w->p,   w->x,    w->mom, w->hx,       w->hy,      w->fx, w->fy,
                   w->hx32, w->hy32, w->b,   w->r,   w->V,        w->V32,     w->w,  w->w32,
-                  w->b32,  w->x32, w->R,    w->g,   w->cs,       w->sn,      w->y,  w->partials,
-                  w->ctl};
+                  w->R,    w->g,    w->cs,   w->sn,  w->y,        w->partials, w->ctl,
+                  w->dA64, w->dA32};
   for (void* p : ptrs) {
     if (p) (void)hipFree(p);
   }
   if (w->ctl_host) (void)hipHostFree(w->ctl_host);
+  if (w->hA64) (void)hipHostFree(w->hA64);
+  if (w->hA32) (void)hipHostFree(w->hA32);
+  if (w->exec64) (void)hipGraphExecDestroy(w->exec64);
+  if (w->exec32) (void)hipGraphExecDestroy(w->exec32);
+  if (w->ev) (void)hipEventDestroy(w->ev);
+  if (w->gs) (void)hipStreamDestroy(w->gs);
   delete w;
   ctx->imex = nullptr;
 }
@@ -235,15 +253,35 @@ __device__ void reduce_sum_store(DSum<K> s, double* partials, int kactive)
   }
 }
 
-// out[q] = sum over blocks of partials[b*K + q], q < k (fixed order).
-__global__ void finalize_sum_kernel(const double* partials, int nparts, int K, int k, double* out,
-                                    const GmCtl* ctl)
+// Workgroup sums of the k products prod(q) (q < k) of this thread's row,
+// stored as partials[block*MAXK + q].  One row per thread, so nothing is
+// carried across rows and the accumulators do not occupy registers.
+template <typename F>
+__device__ __forceinline__ void rowdots_store(const F& prod, int k, double* partials)
 {
-  if (ctl && ctl->done) return;
-  for (int q = threadIdx.x; q < k; q += blockDim.x) {
-    double v = 0.0;
-    for (int b = 0; b < nparts; ++b) v += partials[(long long)b * K + q];
-    out[q] = v;
+  __shared__ double sh[RB / 64][MAXK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // Eight products (and their basis loads) in flight at a time.
+  for (int q0 = 0; q0 < k; q0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = q0 + u < k ? prod(q0 + u) : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] += __shfl_xor(v[u], off);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < k) sh[wave][q0 + u] = v[u];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double v = sh[0][threadIdx.x];
+    for (int w = 1; w < RB / 64; ++w) v += sh[w][threadIdx.x];
+    partials[(long long)blockIdx.x * MAXK + threadIdx.x] = v;
   }
 }
 
@@ -273,18 +311,37 @@ __device__ void reduce_maxsum_store(double mx, double sm, double* partials)
   }
 }
 
-// combine = 0: (max, sum); 1: (max, max).
-__global__ void finalize_pair_kernel(const double* partials, int nparts, int combine, double* out_a,
-                                     double* out_b)
+// combine = 0: (max, sum); 1: (max, max).  One workgroup; the sum runs in a
+// fixed order (run-to-run reproducible).
+__global__ void __launch_bounds__(RB) finalize_pair_kernel(const double* partials, int nparts,
+                                                           int combine, double* out_a, double* out_b)
 {
-  if (threadIdx.x != 0) return;
+  __shared__ double sha[RB / 64], shb[RB / 64];
   double a = -INFINITY, b = combine ? -INFINITY : 0.0;
-  for (int q = 0; q < nparts; ++q) {
+  for (int q = threadIdx.x; q < nparts; q += RB) {
     a = smax(a, partials[2 * q]);
     b = combine ? smax(b, partials[2 * q + 1]) : b + partials[2 * q + 1];
   }
-  *out_a = a;
-  *out_b = b;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = smax(a, __shfl_xor(a, off));
+    const double ob = __shfl_xor(b, off);
+    b = combine ? smax(b, ob) : b + ob;
+  }
+  if (lane == 0) {
+    sha[wave] = a;
+    shb[wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < RB / 64; ++w) {
+      a = smax(a, sha[w]);
+      b = combine ? smax(b, shb[w]) : b + shb[w];
+    }
+    *out_a = a;
+    *out_b = b;
+  }
 }
 
 // ------------------------------------------------------------ star update
@@ -485,22 +542,31 @@ struct OpArgs {
   T cx, cy, diag;
 };
 
-template <typename T>
-__device__ __forceinline__ T apply_row(const OpArgs<T>& A, const T* __restrict__ x, int i, int j)
+// Row (i,j) of the operator applied to the vector whose entries get(k)
+// returns (k a column index).
+template <typename T, typename Get>
+__device__ __forceinline__ T apply_row_g(const OpArgs<T>& A, const Get& get, int i, int j)
 {
   const ImexGrid& G = A.G;
+  auto at = [&](int ii, int jj) -> T { return fixed_side(G, ii, jj) >= 0 ? T(0) : get(pcol(G, ii, jj)); };
   const long long k = (long long)j * G.nx + i;
-  const T pc = x[k];
+  const T pc = get(k);
   const T he = A.hx[(long long)j * (G.nx + 1) + i + 1], hw = A.hx[(long long)j * (G.nx + 1) + i];
   const T hn = A.hy[(long long)(j + 1) * G.nx + i], hs = A.hy[(long long)j * G.nx + i];
-  const T xe = x_at(G, x, i + 1, j), xw = x_at(G, x, i - 1, j);
-  const T xee = x_at(G, x, i + 2, j), xww = x_at(G, x, i - 2, j);
-  const T yn = x_at(G, x, i, j + 1), ys = x_at(G, x, i, j - 1);
-  const T ynn = x_at(G, x, i, j + 2), yss = x_at(G, x, i, j - 2);
+  const T xe = at(i + 1, j), xw = at(i - 1, j);
+  const T xee = at(i + 2, j), xww = at(i - 2, j);
+  const T yn = at(i, j + 1), ys = at(i, j - 1);
+  const T ynn = at(i, j + 2), yss = at(i, j - 2);
   const T cxd = xe - xw, cyd = yn - ys;
   const T xpart = he * ((xee - pc) + cxd) - hw * ((pc - xww) + cxd);
   const T ypart = hn * ((ynn - pc) + cyd) - hs * ((pc - yss) + cyd);
   return A.diag * pc - A.cx * xpart - A.cy * ypart;
+}
+
+template <typename T>
+__device__ __forceinline__ T apply_row(const OpArgs<T>& A, const T* __restrict__ x, int i, int j)
+{
+  return apply_row_g(A, [x](long long k) -> T { return x[k]; }, i, j);
 }
 
 template <typename T>
@@ -515,10 +581,11 @@ __global__ void matvec_kernel(OpArgs<T> A, const T* __restrict__ x, T* __restric
 }
 
 // r = b - A x (FP64), with |r|_inf and |r|_2^2 reduced into ctl.
-__global__ void residual_kernel(OpArgs<double> A, const double* __restrict__ x,
+__global__ void residual_kernel(const OpArgs<double>* __restrict__ Ap, const double* __restrict__ x,
                                 const double* __restrict__ b, double* __restrict__ r,
                                 double* partials, GmCtl* ctl)
 {
+  const OpArgs<double> A = *Ap;
   const long long n = (long long)A.G.nx * A.G.ny;
   double mx = 0.0, s2 = 0.0;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -534,27 +601,8 @@ __global__ void residual_kernel(OpArgs<double> A, const double* __restrict__ x,
 }
 
 // ------------------------------------------------------------ GMRES pieces
-// dots[i] = V_i . w for i < k (CGS pass 1).
-template <typename T>
-__global__ void __launch_bounds__(RB) multidot_kernel(const T* __restrict__ V, long long n, int k,
-                                                      const T* __restrict__ w, double* partials,
-                                                      GmCtl* ctl)
-{
-  if (ctl->done) return;
-  DSum<MAXK> s;
-#pragma unroll
-  for (int q = 0; q < MAXK; ++q) s.v[q] = 0.0;
-  for (long long t = (long long)blockIdx.x * RB + threadIdx.x; t < n; t += (long long)gridDim.x * RB) {
-    const double wt = (double)w[t];
-#pragma unroll
-    for (int q = 0; q < MAXK; ++q) {
-      if (q < k) s.v[q] += (double)V[q * n + t] * wt;
-    }
-  }
-  reduce_sum_store<MAXK>(s, partials, k);
-}
-
 // w -= V h1 (h1 = dots1), then dots2[i] = V_i . w (CGS pass 2, fused).
+// One row per thread (grid = ceil(n / RB)).
 template <typename T>
 __global__ void __launch_bounds__(RB) update_dot_kernel(const T* __restrict__ V, long long n, int k,
                                                         T* __restrict__ w, double* partials, GmCtl* ctl)
@@ -563,30 +611,28 @@ __global__ void __launch_bounds__(RB) update_dot_kernel(const T* __restrict__ V,
   __shared__ double h[MAXK];
   if (threadIdx.x < k) h[threadIdx.x] = ctl->dots1[threadIdx.x];
   __syncthreads();
-  DSum<MAXK> s;
-#pragma unroll
-  for (int q = 0; q < MAXK; ++q) s.v[q] = 0.0;
-  for (long long t = (long long)blockIdx.x * RB + threadIdx.x; t < n; t += (long long)gridDim.x * RB) {
+  const long long t = (long long)blockIdx.x * RB + threadIdx.x;
+  double wd = 0.0;
+  if (t < n) {
     double wt = (double)w[t];
     for (int q = 0; q < k; ++q) wt -= h[q] * (double)V[q * n + t];
     const T wn = (T)wt;
     w[t] = wn;
-#pragma unroll
-    for (int q = 0; q < MAXK; ++q) {
-      if (q < k) s.v[q] += (double)V[q * n + t] * (double)wn;
-    }
+    wd = (double)wn;
   }
-  reduce_sum_store<MAXK>(s, partials, k);
+  rowdots_store([&](int q) { return t < n ? (double)V[q * n + t] * wd : 0.0; }, k, partials);
 }
 
-// w -= V h2 (h2 = dots2) and |w|^2.
+// w -= V h (h = dots2 after a second Gram-Schmidt pass, dots1 with a single
+// pass) and |w|^2.
 template <typename T>
 __global__ void __launch_bounds__(RB) update_norm_kernel(const T* __restrict__ V, long long n, int k,
-                                                         T* __restrict__ w, double* partials, GmCtl* ctl)
+                                                         T* __restrict__ w, double* partials, GmCtl* ctl,
+                                                         int pass)
 {
   if (ctl->done) return;
   __shared__ double h[MAXK];
-  if (threadIdx.x < k) h[threadIdx.x] = ctl->dots2[threadIdx.x];
+  if (threadIdx.x < k) h[threadIdx.x] = pass == 1 ? ctl->dots1[threadIdx.x] : ctl->dots2[threadIdx.x];
   __syncthreads();
   DSum<1> s;
   s.v[0] = 0.0;
@@ -600,53 +646,131 @@ __global__ void __launch_bounds__(RB) update_norm_kernel(const T* __restrict__ V
   reduce_sum_store<1>(s, partials, 1);
 }
 
-// Column j of the Hessenberg matrix, Givens rotations, residual estimate and
-// the early-exit decision (one wavefront; the work is O(j)).
-__global__ void givens_kernel(double* R, double* g, double* cs, double* sn, int j, int m, GmCtl* ctl,
-                              double breakdown_tol)
-{
-  if (ctl->done || threadIdx.x != 0) return;
-  double* col = R + (long long)j * (m + 1);  // column-major (m+1) x m
-  for (int i = 0; i <= j; ++i) col[i] = ctl->dots1[i] + ctl->dots2[i];
-  const double hn = sqrt(ctl->nrm2);
-  col[j + 1] = hn;
-  for (int i = 0; i < j; ++i) {
-    const double a = col[i], b = col[i + 1];
-    col[i] = cs[i] * a + sn[i] * b;
-    col[i + 1] = -sn[i] * a + cs[i] * b;
-  }
-  const double a = col[j], b = col[j + 1];
-  const double rr = hypot(a, b);
-  cs[j] = rr == 0.0 ? 1.0 : a / rr;
-  sn[j] = rr == 0.0 ? 0.0 : b / rr;
-  col[j] = rr;
-  col[j + 1] = 0.0;
-  g[j + 1] = -sn[j] * g[j];
-  g[j] = cs[j] * g[j];
-  ctl->est = fabs(g[j + 1]);
-  ctl->kk = j + 1;
-  ctl->inv_norm = 1.0 / hn;
-  if (!isfinite(hn) || !isfinite(ctl->est) || !isfinite(rr)) ctl->bad = 1;
-  if (hn <= breakdown_tol) ctl->breakdown = 1;
-  if (ctl->est <= ctl->thr || j + 1 >= m || ctl->bad || ctl->breakdown) ctl->done = 1;
-}
-
+// Arnoldi iteration j, first half: v_j = w_prev / |w_prev| (for j > 0; v_0
+// is already in V) is formed on the fly and stored, y = A v_j, and the first
+// Gram-Schmidt pass's dots V_q . y (q <= j) are accumulated in the same sweep,
+// so the new column and the basis are each read once.
 template <typename T>
-__global__ void normalize_kernel(const T* __restrict__ w, T* __restrict__ v, long long n, GmCtl* ctl)
+__global__ void __launch_bounds__(RB) arnoldi_kernel(const OpArgs<T>* __restrict__ Ap, T* __restrict__ V,
+                                                     int j, const T* __restrict__ wprev,
+                                                     T* __restrict__ y, double* partials, GmCtl* ctl)
 {
   if (ctl->done) return;
+  const OpArgs<T> A = *Ap;
+  const long long n = (long long)A.G.nx * A.G.ny;
   const double s = ctl->inv_norm;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    v[t] = (T)((double)w[t] * s);
+  T* vj = V + (long long)j * n;
+  const long long t = (long long)blockIdx.x * RB + threadIdx.x;
+  double yd = 0.0, vd = 0.0;
+  if (t < n) {
+    const int i = (int)(t % A.G.nx), jj = (int)(t / A.G.nx);
+    T yt, vt;
+    if (j == 0) {
+      yt = apply_row(A, vj, i, jj);
+      vt = vj[t];
+    } else {
+      const auto get = [wprev, s](long long k) -> T { return (T)((double)wprev[k] * s); };
+      yt = apply_row_g(A, get, i, jj);
+      vt = get(t);
+      vj[t] = vt;
+    }
+    y[t] = yt;
+    yd = (double)yt;
+    vd = (double)vt;
   }
+  rowdots_store([&](int q) { return t >= n ? 0.0 : (q < j ? (double)V[q * n + t] : vd) * yd; }, j + 1,
+                partials);
+}
+
+// Fixed-order workgroup sum (valid in thread 0).
+__device__ __forceinline__ double block_sum_fixed(double v, double* sh)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = sh[0];
+    for (int w = 1; w < RB / 64; ++w) v += sh[w];
+  }
+  return v;
+}
+
+// out[q] = sum over blocks of partials[b*K + q]; one workgroup per q, each
+// summing in a fixed order (run-to-run reproducible).
+__global__ void __launch_bounds__(RB) finalize_dots_kernel(const double* partials, int nparts, int K,
+                                                           double* out, const GmCtl* ctl)
+{
+  __shared__ double sh[RB / 64];
+  if (ctl && ctl->done) return;
+  const int q = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += RB) v += partials[(long long)b * K + q];
+  v = block_sum_fixed(v, sh);
+  if (threadIdx.x == 0) out[q] = v;
+}
+
+// Arnoldi iteration j, closing half: |w|^2 from the partials, then column j
+// of the Hessenberg matrix, its Givens rotations, the residual estimate and
+// the early-exit decision (the column and rotations staged in LDS).
+__global__ void __launch_bounds__(RB) norm_givens_kernel(const double* partials, int nparts, double* R,
+                                                         double* g, double* cs, double* sn, int j,
+                                                         int m, GmCtl* ctl, int two_pass)
+{
+  __shared__ double sh[RB / 64];
+  __shared__ double col[MAXK + 1], c[MAXK], sv[MAXK];
+  if (ctl->done) return;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += RB) v += partials[b];
+  v = block_sum_fixed(v, sh);
+  for (int i = threadIdx.x; i <= j; i += RB) {
+    col[i] = two_pass ? ctl->dots1[i] + ctl->dots2[i] : ctl->dots1[i];
+    if (i < j) {
+      c[i] = cs[i];
+      sv[i] = sn[i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // breakdown tolerance 100 eps max(beta, 1) (imex_pressure_helpers.hpp:167-169)
+    const double breakdown_tol = 2.220446049250313e-16 * 100.0 * smax(ctl->beta, 1.0);
+    ctl->nrm2 = v;
+    const double hn = sqrt(v);
+    col[j + 1] = hn;
+    for (int i = 0; i < j; ++i) {
+      const double a = col[i], b = col[i + 1];
+      col[i] = c[i] * a + sv[i] * b;
+      col[i + 1] = -sv[i] * a + c[i] * b;
+    }
+    const double a = col[j], b = col[j + 1];
+    const double rr = hypot(a, b);
+    const double cj = rr == 0.0 ? 1.0 : a / rr;
+    const double sj = rr == 0.0 ? 0.0 : b / rr;
+    cs[j] = cj;
+    sn[j] = sj;
+    col[j] = rr;
+    col[j + 1] = 0.0;
+    g[j + 1] = -sj * g[j];
+    g[j] = cj * g[j];
+    ctl->est = fabs(g[j + 1]);
+    ctl->kk = j + 1;
+    ctl->inv_norm = 1.0 / hn;
+    if (!isfinite(hn) || !isfinite(ctl->est) || !isfinite(rr)) ctl->bad = 1;
+    if (hn <= breakdown_tol) ctl->breakdown = 1;
+    if (ctl->est <= ctl->thr || j + 1 >= ctl->mlim || ctl->bad || ctl->breakdown) ctl->done = 1;
+  }
+  __syncthreads();
+  double* rc = R + (long long)j * (m + 1);  // column-major (m+1) x m
+  for (int i = threadIdx.x; i <= j + 1; i += RB) rc[i] = col[i];
 }
 
 // Start a cycle: v0 = r / beta, g = beta e1.
 template <typename T>
 __global__ void cycle_start_kernel(const double* __restrict__ r, T* __restrict__ v0, long long n,
-                                   double beta, double* g)
+                                   const GmCtl* ctl, double* g)
 {
+  const double beta = ctl->beta;
   const double s = 1.0 / beta;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
@@ -678,23 +802,6 @@ __global__ void xupdate_kernel(const T* __restrict__ V, long long n, const doubl
     double s = 0.0;
     for (int q = 0; q < kk; ++q) s += y[q] * (double)V[q * n + t];
     x[t] = (X)((double)x[t] + scale * s);
-  }
-}
-
-__global__ void to_float_kernel(const double* __restrict__ a, float* __restrict__ b, long long n,
-                                double scale)
-{
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    b[t] = (float)(a[t] * scale);
-  }
-}
-
-__global__ void fill_kernel(float* __restrict__ a, long long n, float v)
-{
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    a[t] = v;
   }
 }
 
@@ -798,8 +905,7 @@ __global__ void imex_rates_kernel(const double* __restrict__ U, long long rs, lo
 // ----------------------------------------------------------------- host
 static int grid_blocks(long long n)
 {
-  long long b = (n + RB - 1) / RB;
-  if (b > 1024) b = 1024;
+  const long long b = (n + RB - 1) / RB;  // one row per thread
   return b < 1 ? 1 : (int)b;
 }
 
@@ -815,7 +921,7 @@ static ec_status imex_workspace(ec_ctx* ctx, int m)
   w->m = m;
   const size_t n = (size_t)w->nx * w->ny;
   const size_t nxf = (size_t)(w->nx + 1) * w->ny, nyf = (size_t)w->nx * (w->ny + 1);
-  w->nparts = 1024;
+  w->nparts = grid_blocks(n);
   bool ok = true;
   auto A = [&](void** p, size_t bytes) { ok = ok && hipMalloc(p, bytes) == hipSuccess; };
   A((void**)&w->star, 4 * n * sizeof(double));
@@ -832,10 +938,10 @@ static ec_status imex_workspace(ec_ctx* ctx, int m)
   A((void**)&w->r, n * sizeof(double));
   A((void**)&w->V, (size_t)(m + 1) * n * sizeof(double));
   A((void**)&w->V32, (size_t)(m + 1) * n * sizeof(float));
-  A((void**)&w->w, n * sizeof(double));
-  A((void**)&w->w32, n * sizeof(float));
-  A((void**)&w->b32, n * sizeof(float));
-  A((void**)&w->x32, n * sizeof(float));
+  A((void**)&w->w, 2 * n * sizeof(double));   // two halves: Arnoldi ping-pong
+  A((void**)&w->w32, 2 * n * sizeof(float));
+  A((void**)&w->dA64, sizeof(OpArgs<double>));
+  A((void**)&w->dA32, sizeof(OpArgs<float>));
   A((void**)&w->R, (size_t)(m + 1) * m * sizeof(double));
   A((void**)&w->g, (size_t)(m + 1) * sizeof(double));
   A((void**)&w->cs, (size_t)m * sizeof(double));
@@ -844,6 +950,10 @@ static ec_status imex_workspace(ec_ctx* ctx, int m)
   A((void**)&w->partials, (size_t)w->nparts * MAXK * sizeof(double));
   A((void**)&w->ctl, sizeof(GmCtl));
   ok = ok && hipHostMalloc((void**)&w->ctl_host, sizeof(GmCtl)) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&w->hA64, sizeof(OpArgs<double>)) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&w->hA32, sizeof(OpArgs<float>)) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&w->gs, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&w->ev, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     imex_release(ctx);
     return EC_ERR_OUT_OF_MEMORY;
@@ -874,59 +984,70 @@ struct HostCtl {
   }
 };
 
-// One GMRES(m) cycle on A z = rhs for precision T: v0 = r0/beta must already
-// be in V[0]; runs until the estimate meets thr or m columns, then updates
-// the iterate x (FP64 for the FP64 solver, FP32 for the inner solver).
-template <typename T, typename X>
-static ec_status gmres_cycle(ec_ctx* ctx, ImexWork* w, const OpArgs<T>& A, T* V, T* wv, X* x,
-                             double beta, double thr, double xscale, int m, int* kk_out,
-                             bool* bad_out, bool* breakdown_out)
+// Enqueues one restart cycle of GMRES(m) for precision T on stream st:
+// v0 = r / beta, up to ctl->mlim Arnoldi iterations (early exit through
+// ctl->done), the back-solve, x += V y (x in FP64), then the true residual
+// r = b - A x and its norms, and finally the control block back to the host.
+// Every value that changes between cycles or solves is read from device
+// memory (ctl, the operator block), so the sequence is captured once into a
+// hipGraph and replayed.  two_pass: classical Gram-Schmidt with
+// re-orthogonalisation (CGS2); otherwise one pass (CGS1, the FP32 inner
+// solver, whose loss of orthogonality the FP64 refinement absorbs).
+template <typename T>
+static void enqueue_cycle(ImexWork* w, hipStream_t st, const OpArgs<T>* dA, T* V, T* wv, int two_pass)
 {
-  HostCtl hc{ctx, w};
   const long long n = (long long)w->nx * w->ny;
   const int nb = grid_blocks(n);
-  GmCtl& h = *w->ctl_host;
-  h.done = 0;
-  h.kk = 0;
-  h.bad = 0;
-  h.breakdown = 0;
-  h.thr = thr;
-  h.beta = beta;
-  h.counter = 0;
-  // push the scalar fields only (the rest of the block is device state)
-  EC_HIP_TRY(ctx, hipMemcpyAsync(w->ctl, w->ctl_host, offsetof(GmCtl, dots1), hipMemcpyHostToDevice,
-                                 ctx->stream));
-  const double breakdown_tol = 2.220446049250313e-16 * 100.0 * smax(beta, 1.0);
+  const int m = w->m;
+  (void)hipMemcpyAsync(w->ctl, w->ctl_host, offsetof(GmCtl, dots1), hipMemcpyHostToDevice, st);
+  hipLaunchKernelGGL((cycle_start_kernel<T>), dim3(nb), dim3(RB), 0, st, w->r, V, n, w->ctl, w->g);
   for (int j = 0; j < m; ++j) {
-    T* vj = V + (long long)j * n;
-    hipLaunchKernelGGL((matvec_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, A, vj, wv, w->ctl);
-    hipLaunchKernelGGL((multidot_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, V, n, j + 1, wv,
-                       w->partials, w->ctl);
-    hipLaunchKernelGGL(finalize_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials, nb, MAXK,
-                       j + 1, w->ctl->dots1, w->ctl);
-    hipLaunchKernelGGL((update_dot_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, V, n, j + 1, wv,
-                       w->partials, w->ctl);
-    hipLaunchKernelGGL(finalize_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials, nb, MAXK,
-                       j + 1, w->ctl->dots2, w->ctl);
-    hipLaunchKernelGGL((update_norm_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, V, n, j + 1, wv,
-                       w->partials, w->ctl);
-    hipLaunchKernelGGL(finalize_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials, nb, 1, 1,
-                       &w->ctl->nrm2, w->ctl);
-    hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(64), 0, ctx->stream, w->R, w->g, w->cs, w->sn, j,
-                       w->m, w->ctl, breakdown_tol);
-    hipLaunchKernelGGL((normalize_kernel<T>), dim3(nb), dim3(RB), 0, ctx->stream, wv,
-                       V + (long long)(j + 1) * n, n, w->ctl);
+    T* wj = (j & 1) ? wv + n : wv;
+    const T* wp = (j & 1) ? wv : wv + n;
+    hipLaunchKernelGGL((arnoldi_kernel<T>), dim3(nb), dim3(RB), 0, st, dA, V, j, wp, wj, w->partials,
+                       w->ctl);
+    hipLaunchKernelGGL(finalize_dots_kernel, dim3(j + 1), dim3(RB), 0, st, w->partials, nb, MAXK,
+                       w->ctl->dots1, w->ctl);
+    if (two_pass) {
+      hipLaunchKernelGGL((update_dot_kernel<T>), dim3(nb), dim3(RB), 0, st, V, n, j + 1, wj,
+                         w->partials, w->ctl);
+      hipLaunchKernelGGL(finalize_dots_kernel, dim3(j + 1), dim3(RB), 0, st, w->partials, nb, MAXK,
+                         w->ctl->dots2, w->ctl);
+    }
+    hipLaunchKernelGGL((update_norm_kernel<T>), dim3(nb), dim3(RB), 0, st, V, n, j + 1, wj,
+                       w->partials, w->ctl, two_pass ? 2 : 1);
+    hipLaunchKernelGGL(norm_givens_kernel, dim3(1), dim3(RB), 0, st, w->partials, nb, w->R, w->g,
+                       w->cs, w->sn, j, m, w->ctl, two_pass);
   }
-  hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, ctx->stream, w->R, w->g, w->y, w->m,
+  hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, st, w->R, w->g, w->y, m, w->ctl);
+  hipLaunchKernelGGL((xupdate_kernel<T, double>), dim3(nb), dim3(RB), 0, st, V, n, w->y, w->x, 1.0,
                      w->ctl);
-  hipLaunchKernelGGL((xupdate_kernel<T, X>), dim3(nb), dim3(RB), 0, ctx->stream, V, n, w->y, x,
-                     xscale, w->ctl);
-  EC_HIP_TRY(ctx, hipGetLastError());
-  const ec_status st = hc.pull();
-  if (st != EC_OK) return st;
-  *kk_out = h.kk;
-  *bad_out = h.bad != 0;
-  *breakdown_out = h.breakdown != 0;
+  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(RB), 0, st, w->dA64, w->x, w->b, w->r,
+                     w->partials, w->ctl);
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, st, w->partials, nb, 0,
+                     &w->ctl->rinf, &w->ctl->r2);
+  (void)hipMemcpyAsync(w->ctl_host, w->ctl, sizeof(GmCtl), hipMemcpyDeviceToHost, st);
+}
+
+template <typename T>
+static ec_status cycle_graph(ec_ctx* ctx, ImexWork* w, const OpArgs<T>* dA, T* V, T* wv, int two_pass,
+                             hipGraphExec_t* exec)
+{
+  if (*exec) return EC_OK;
+  hipGraph_t g = nullptr;
+  EC_HIP_TRY(ctx, hipStreamBeginCapture(w->gs, hipStreamCaptureModeThreadLocal));
+  enqueue_cycle<T>(w, w->gs, dA, V, wv, two_pass);
+  const hipError_t e = hipStreamEndCapture(w->gs, &g);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(ctx, e, "hipStreamEndCapture (GMRES cycle)");
+  }
+  const hipError_t e2 = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    *exec = nullptr;
+    return hip_fail(ctx, e2, "hipGraphInstantiate (GMRES cycle)");
+  }
   return EC_OK;
 }
 
@@ -938,18 +1059,17 @@ struct SolveResult {
   double rinf = INFINITY, rel = INFINITY;
 };
 
-static ec_status true_residual(ec_ctx* ctx, ImexWork* w, const OpArgs<double>& A64, double* rinf,
-                               double* r2)
+// r = b - A x with |r|_inf and |r|_2^2, outside any cycle (the initial guess).
+static ec_status true_residual(ec_ctx* ctx, ImexWork* w, hipStream_t st, double* rinf, double* r2)
 {
   const long long n = (long long)w->nx * w->ny;
-  hipLaunchKernelGGL(residual_kernel, dim3(grid_blocks(n)), dim3(RB), 0, ctx->stream, A64, w->x,
-                     w->b, w->r, w->partials, w->ctl);
-  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials,
-                     grid_blocks(n), 0, &w->ctl->rinf, &w->ctl->r2);
+  hipLaunchKernelGGL(residual_kernel, dim3(grid_blocks(n)), dim3(RB), 0, st, w->dA64, w->x, w->b,
+                     w->r, w->partials, w->ctl);
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, st, w->partials, grid_blocks(n), 0,
+                     &w->ctl->rinf, &w->ctl->r2);
   EC_HIP_TRY(ctx, hipGetLastError());
-  HostCtl hc{ctx, w};
-  const ec_status st = hc.pull();
-  if (st != EC_OK) return st;
+  EC_HIP_TRY(ctx, hipMemcpyAsync(w->ctl_host, w->ctl, sizeof(GmCtl), hipMemcpyDeviceToHost, st));
+  EC_HIP_TRY(ctx, hipStreamSynchronize(st));
   *rinf = w->ctl_host->rinf;
   *r2 = w->ctl_host->r2;
   return EC_OK;
@@ -962,14 +1082,18 @@ static ec_status solve_pressure(ec_ctx* ctx, ImexWork* w, const OpArgs<double>& 
                                 const OpArgs<float>& A32, double rhs_norm, const ec_imex_opts& o,
                                 SolveResult* out)
 {
-  const long long n = (long long)w->nx * w->ny;
-  const int nb = grid_blocks(n);
   const int m = o.restart;
   const double target = o.tol * smax(1.0, rhs_norm);  // abs_ok or rel_ok
   const int max_it = o.max_iter;
   int total = 0;
+  (void)A64;
+  (void)A32;
+  // The private stream starts after the work already queued on the context's
+  // stream (the operator blocks are already in device memory).
+  EC_HIP_TRY(ctx, hipEventRecord(w->ev, ctx->stream));
+  EC_HIP_TRY(ctx, hipStreamWaitEvent(w->gs, w->ev, 0));
   double rinf, r2;
-  ec_status st = true_residual(ctx, w, A64, &rinf, &r2);
+  ec_status st = true_residual(ctx, w, w->gs, &rinf, &r2);
   if (st != EC_OK) return st;
   auto record = [&]() {
     out->rinf = rinf;
@@ -980,38 +1104,41 @@ static ec_status solve_pressure(ec_ctx* ctx, ImexWork* w, const OpArgs<double>& 
     out->bad = true;
     return EC_OK;
   }
+  const bool fp64 = o.precision == EC_IMEX_PRECISION_FP64;
+  hipGraphExec_t* exec = fp64 ? &w->exec64 : &w->exec32;
+  st = fp64 ? cycle_graph<double>(ctx, w, w->dA64, w->V, w->w, 1, exec)
+            : cycle_graph<float>(ctx, w, w->dA32, w->V32, w->w32, 0, exec);
+  if (st != EC_OK) return st;
+  GmCtl& h = *w->ctl_host;
   while (rinf > target && total < max_it) {
     const double beta = std::sqrt(r2);
     if (!(beta > 0.0)) break;
-    const int mm = std::min(m, max_it - total);
-    int kk = 0;
-    bool bad = false, brk = false;
-    if (o.precision == EC_IMEX_PRECISION_FP64) {
+    h.done = 0;
+    h.kk = 0;
+    h.bad = 0;
+    h.breakdown = 0;
+    h.mlim = std::min(m, max_it - total);
+    h.beta = beta;
+    h.counter = 0;
+    if (fp64) {
       // 2-norm estimate threshold scaled by the |r|_2 / |r|_inf ratio of the
       // cycle's start residual; the true residual decides convergence.
-      const double thr = target * (beta / rinf);
-      hipLaunchKernelGGL((cycle_start_kernel<double>), dim3(nb), dim3(RB), 0, ctx->stream, w->r, w->V,
-                         n, beta, w->g);
-      st = gmres_cycle<double, double>(ctx, w, A64, w->V, w->w, w->x, beta, thr, 1.0, mm, &kk, &bad,
-                                       &brk);
+      h.thr = target * (beta / rinf);
     } else {
-      // Iterative refinement: FP32 GMRES on A d = r/beta from d = 0, then
-      // x += beta d in FP64.
-      hipLaunchKernelGGL((cycle_start_kernel<float>), dim3(nb), dim3(RB), 0, ctx->stream, w->r,
-                         w->V32, n, beta, w->g);
-      hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(RB), 0, ctx->stream, w->x32, n, 0.0f);
-      // inner reduction of the 2-norm by o.inner_reduction, floored at what
-      // FP32 can represent, and never below the FP64 goal
-      const double want = smax(o.inner_reduction * beta, target * (beta / rinf));
-      st = gmres_cycle<float, double>(ctx, w, A32, w->V32, w->w32, w->x, beta, want, 1.0, mm, &kk,
-                                      &bad, &brk);
+      // Iterative refinement: FP32 GMRES on A d = r/beta from d = 0, x += d
+      // in FP64; the inner solve reduces the 2-norm by o.inner_reduction,
+      // never below the FP64 goal.
+      h.thr = smax(o.inner_reduction * beta, target * (beta / rinf));
       ++out->refinements;
     }
-    if (st != EC_OK) return st;
+    EC_HIP_TRY(ctx, hipGraphLaunch(*exec, w->gs));
+    EC_HIP_TRY(ctx, hipStreamSynchronize(w->gs));
+    const int kk = h.kk;
+    const bool bad = h.bad != 0, brk = h.breakdown != 0;
     total += kk;
     out->breakdown = brk;
-    st = true_residual(ctx, w, A64, &rinf, &r2);
-    if (st != EC_OK) return st;
+    rinf = h.rinf;
+    r2 = h.r2;
     record();
     if (bad || !std::isfinite(rinf)) {
       out->bad = true;
@@ -1110,6 +1237,10 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
   OpArgs<double> A64{G, w->hx, w->hy, 0.25 * dt * dt * G.inv_dx * G.inv_dx,
                      0.25 * dt * dt * G.inv_dy * G.inv_dy, 1.0 / (G.gamma - 1.0)};
   OpArgs<float> A32{G, w->hx32, w->hy32, (float)A64.cx, (float)A64.cy, (float)A64.diag};
+  *w->hA64 = A64;
+  *w->hA32 = A32;
+  EC_HIP_TRY(ctx, hipMemcpyAsync(w->dA64, w->hA64, sizeof(A64), hipMemcpyHostToDevice, ctx->stream));
+  EC_HIP_TRY(ctx, hipMemcpyAsync(w->dA32, w->hA32, sizeof(A32), hipMemcpyHostToDevice, ctx->stream));
   const bool mixed = opts->precision == EC_IMEX_PRECISION_MIXED;
   for (int it = 0; it < opts->picard_iterations; ++it) {
     hipLaunchKernelGGL(imex_faces_kernel, dim3(nbf), dim3(RB), 0, ctx->stream, G, w->star, w->mom,
@@ -1121,7 +1252,7 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
     // buffer; reuse r = b - A*0 = b.
     EC_HIP_TRY(ctx, hipMemsetAsync(w->x, 0, n * sizeof(double), ctx->stream));
     double binf, b2;
-    if ((st = true_residual(ctx, w, A64, &binf, &b2)) != EC_OK) return st;
+    if ((st = true_residual(ctx, w, ctx->stream, &binf, &b2)) != EC_OK) return st;
     if (h.nonfinite > 0 || !std::isfinite(binf)) {
       diag->status = EC_IMEX_ROW_NONFINITE;
       diag->nonfinite_count = h.nonfinite;
@@ -1170,7 +1301,7 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
                      w->hx, w->hy, nullptr, nullptr, w->fx, w->fy, w->ctl);
   hipLaunchKernelGGL(closeout_kernel, dim3(nb), dim3(RB), 0, ctx->stream, G, w->star, w->mom, w->p,
                      w->fx, w->fy, U, lay->row_stride, lay->comp_stride, w->partials, w->ctl);
-  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials, nb, 0,
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, ctx->stream, w->partials, nb, 0,
                      &w->ctl->coupled_res, &w->ctl->r2);
   if (p_out) {
     EC_HIP_TRY(ctx, hipMemcpyAsync(p_out, w->p, n * sizeof(double), hipMemcpyDeviceToDevice,
@@ -1197,7 +1328,7 @@ extern "C" ec_status ec_imex_dt(ec_ctx* ctx, const double* U, const ec_layout* l
   const long long n = (long long)w->nx * w->ny;
   hipLaunchKernelGGL(imex_rates_kernel, dim3(grid_blocks(n)), dim3(RB), 0, ctx->stream, U,
                      lay->row_stride, lay->comp_stride, ctx->grid, w->partials, w->ctl);
-  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(64), 0, ctx->stream, w->partials,
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, ctx->stream, w->partials,
                      grid_blocks(n), 1, &w->ctl->mat_max, &w->ctl->aco_max);
   EC_HIP_TRY(ctx, hipGetLastError());
   HostCtl hc{ctx, w};

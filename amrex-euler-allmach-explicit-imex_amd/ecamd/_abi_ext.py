@@ -31,6 +31,16 @@ class ImexDiag(C.Structure):
                 ("coupled_residual_linf", C.c_double)]
 
 
+class MemberDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("bc", C.c_int * 4), ("fixed_state", (C.c_double * 4) * 4), ("cfl", C.c_double),
+                ("stop_time", C.c_double), ("max_step", C.c_int)]
+
+
+class BatchTiming(C.Structure):
+    _fields_ = [("mode", C.c_int), ("tile_id", C.c_int), ("ms", C.c_double)]
+
+
 IMEX_STATUS = {0: "converged_absolute", 1: "converged_relative", 2: "linear_solve_not_converged",
                3: "gmres_breakdown_not_converged", 4: "density_denominator_floor_used",
                5: "source_map_pressure_row_nonfinite", 6: "linear_residual_nonfinite",
@@ -69,3 +79,26 @@ def declare(L) -> None:
     L.ec_imex_dt.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.c_double, C.c_double, C.c_int,
                              C.c_double, dp, dp, dp]
     L.ec_imex_dt.restype = st
+    from ._lib import RunState
+    ip = C.POINTER(C.c_int)
+    L.ec_batch_create.argtypes = [C.POINTER(MemberDesc), C.c_int, C.c_double, C.c_int, C.c_int,
+                                  C.c_int, vp, C.POINTER(vp)]
+    L.ec_batch_create.restype = st
+    L.ec_batch_destroy.argtypes = [vp]
+    L.ec_batch_destroy.restype = st
+    L.ec_batch_last_error.argtypes = [vp]
+    L.ec_batch_last_error.restype = C.c_char_p
+    L.ec_batch_set_state.argtypes = [vp, C.c_int, vp, C.POINTER(Layout)]
+    L.ec_batch_set_state.restype = st
+    L.ec_batch_get_state.argtypes = [vp, C.c_int, vp, C.POINTER(Layout)]
+    L.ec_batch_get_state.restype = st
+    L.ec_batch_tile_count.argtypes = []
+    L.ec_batch_tile_count.restype = C.c_int
+    L.ec_batch_tile_shape.argtypes = [C.c_int, ip, ip, ip]
+    L.ec_batch_tile_shape.restype = st
+    L.ec_batch_configure.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.ec_batch_configure.restype = st
+    L.ec_batch_autotune.argtypes = [vp, C.POINTER(BatchTiming), C.c_int, ip]
+    L.ec_batch_autotune.restype = st
+    L.ec_batch_run.argtypes = [vp, C.POINTER(RunState), C.POINTER(BatchTiming)]
+    L.ec_batch_run.restype = st

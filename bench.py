@@ -78,6 +78,195 @@ def copy_calibration(dev, n: int, steps: int) -> int:
     return 0
 
 
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def bench_imex(dev, n: int = 512, mach: float = 0.001, reps: int = 3) -> dict:
+    """IMEX T1/S2 BDLTV20 source-map Picard step, Gresho n x n periodic at
+    Mach `mach`: device GMRES(30) to 1e-10 relative residual in FP64 and in
+    mixed form (FP32 inner Krylov, FP64 residual refinement), same dt, same
+    initial state; time per step and the difference of the two results."""
+    import torch
+    import ecamd
+    from ecamd import problems
+    grid = ecamd.Grid(n, n, 1.0 / n, 1.0 / n, 1.4, "hllc", "minmod", 2)
+    s = ecamd.ExplicitSolver(grid, dev)
+    U0 = torch.from_numpy(problems.gresho(n, mach)).to(dev)
+    dt, _, _ = s.imex_dt(U0, 0.8, 0.0, 1.0)
+    out = {"workload": f"gresho_vortex_{n}x{n}_periodic_M{mach:g}_imex_t1s2_bdltv20_picard2",
+           "dt": dt, "tol": 1e-10, "restart": 30}
+    results = {}
+    for prec in ("fp64", "mixed"):
+        opts = s.imex_opts(precision=prec, tol=1e-10, max_iter=2000, restart=30)
+        U = U0.clone()
+        s.imex_step(U, dt, opts)            # warm-up
+        times = []
+        for _ in range(reps):
+            U = U0.clone()
+            torch.cuda.synchronize()
+            a, b = _events()
+            a.record()
+            d = s.imex_step(U, dt, opts)
+            b.record()
+            torch.cuda.synchronize()
+            times.append(a.elapsed_time(b))
+        results[prec] = U
+        out[prec] = {"ms_per_step": sorted(times)[len(times) // 2], "status": ecamd.IMEX_STATUS[d.status],
+                     "gmres_iterations": d.iterations, "refinements": d.refinements,
+                     "relative_residual_linf": d.relative_residual_linf}
+    diff = (results["mixed"] - results["fp64"]).abs().reshape(4, -1).sum(1)
+    den = results["fp64"].abs().reshape(4, -1).sum(1)
+    out["mixed_vs_fp64_rel_l1"] = [float(x) for x in (diff / den).cpu()]
+    out["speedup_mixed_over_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    s.close()
+    return out
+
+
+def bench_adjoint(dev, nx: int = 640, ny: int = 160, nsteps: int = 768, K: int = 32,
+                  ndirs: int = 16) -> dict:
+    """Checkpointed adjoint of the explicit O2 HLLC run: single-gamma shock
+    (Mach 1.22) into a light bubble on [0,4]x[0,1]; gradient of the bubble
+    kinetic energy after nsteps w.r.t. the initial density; cost ratio of the
+    adjoint sweep (with its recompute) to the forward run, and central
+    differences on ndirs seeded smooth directions."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    import ecamd
+    from ecamd import problems
+    from ecamd._lib import lib
+    from ecamd.adjoint import ExplicitRun, conserved_from_density, kinetic_energy_in_region, run_with_dts
+    lx, ly = 4.0, 1.0
+    U0 = problems.initial_state("shock_bubble_mach", nx, ny, prob_hi=(lx, ly), gamma=1.4,
+                                shock_mach=1.22, shock_x=0.5, density_outer=1.0, pressure=1.0,
+                                density_inner=0.138, blob_center=(1.0, 0.5), blob_radius=0.25)
+    post = problems.post_shock_state(1.4, 1.22)
+    inflow = (post[0], post[0] * post[1], 0.0, post[3] / 0.4 + 0.5 * post[0] * post[1] ** 2)
+    grid = ecamd.Grid(nx, ny, lx / nx, ly / ny, 1.4, "hllc", "minmod", 2,
+                      ("fixed", "outflow", "reflect", "outflow"), (inflow, None, None, None))
+    s = ecamd.Solver(grid, dev)
+    rho = torch.from_numpy(U0[0]).to(dev)
+    u = torch.from_numpy(U0[1] / U0[0]).to(dev)
+    v = torch.from_numpy(U0[2] / U0[0]).to(dev)
+    p = torch.from_numpy(0.4 * (U0[3] - 0.5 * (U0[1] ** 2 + U0[2] ** 2) / U0[0])).to(dev)
+    x = (torch.arange(nx, device=dev, dtype=torch.float64) + 0.5) * lx / nx
+    y = (torch.arange(ny, device=dev, dtype=torch.float64) + 0.5) * ly / ny
+    X, Y = torch.meshgrid(x, y, indexing="xy")
+    bubble = (((X - 1.0) ** 2 + (Y - 0.5) ** 2) < 0.25 ** 2).to(torch.float64)
+    region = (((X - 1.0) ** 2 + (Y - 0.5) ** 2) < 0.4 ** 2).to(torch.float64)
+    rho = rho * (1.0 + 0.05 * torch.sin(3 * X) * torch.cos(2 * Y) * bubble)
+    obj = lambda U: kinetic_energy_in_region(U, region, lx / nx * ly / ny)
+    cfl = 0.4
+    Ub = conserved_from_density(rho, u, v, p, 1.4)
+    # plain forward: the device driver loop, no checkpoints
+    A, B = Ub.clone(), torch.empty_like(Ub)
+    run = s.new_run(cfl, 1.0e30, nsteps)
+    s.advance(A, B, nsteps, run)
+    torch.cuda.synchronize()
+    A = Ub.clone()
+    run = s.new_run(cfl, 1.0e30, nsteps)
+    a, b = _events()
+    a.record()
+    s.advance(A, B, nsteps, run)
+    b.record()
+    torch.cuda.synchronize()
+    fwd_ms = a.elapsed_time(b)
+    # checkpointed forward and adjoint sweep
+    n = nx * ny
+    chk = torch.empty(((nsteps + K - 1) // K) * 4 * n, dtype=torch.float64, device=dev)
+    UN = torch.empty_like(Ub)
+    dts = np.zeros(nsteps)
+    done = C.c_int()
+    def fwd_chk():
+        lib().ec_forward_checkpointed(s.ctx, C.c_void_p(Ub.data_ptr()), C.c_void_p(UN.data_ptr()),
+                                      C.byref(s.layout), nsteps, K, cfl, 1.0e30,
+                                      C.c_void_p(chk.data_ptr()),
+                                      dts.ctypes.data_as(C.POINTER(C.c_double)), C.byref(done))
+    fwd_chk()
+    torch.cuda.synchronize()
+    a, b = _events()
+    a.record()
+    fwd_chk()
+    b.record()
+    torch.cuda.synchronize()
+    fwd_chk_ms = a.elapsed_time(b)
+    UNg = UN.detach().clone().requires_grad_(True)
+    (lamN,) = torch.autograd.grad(obj(UNg), UNg)
+    lamN = lamN.contiguous()
+    lam0 = torch.empty_like(lamN)
+    def adj():
+        lib().ec_adjoint_checkpointed(s.ctx, C.c_void_p(chk.data_ptr()), C.byref(s.layout), nsteps, K,
+                                      dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                      C.c_void_p(lamN.data_ptr()), C.c_void_p(lam0.data_ptr()))
+    adj()
+    torch.cuda.synchronize()
+    a, b = _events()
+    a.record()
+    adj()
+    b.record()
+    torch.cuda.synchronize()
+    adj_ms = a.elapsed_time(b)
+    # gradient through the autograd wrapper, checked by central differences
+    rho_req = rho.clone().requires_grad_(True)
+    J = ExplicitRun.apply(rho_req, u, v, p, s, nsteps, K, cfl, obj)
+    (grad,) = torch.autograd.grad(J, rho_req)
+    rng = np.random.default_rng(20261203)
+    errs = []
+    eps = 1e-6
+    for _ in range(ndirs):
+        ph = rng.uniform(0, 2 * np.pi, 2)
+        kx, ky = rng.integers(2, 7, 2)
+        d = torch.sin(float(kx) * X + ph[0]) * torch.cos(float(ky) * Y + ph[1]) * bubble
+        Jp = float(obj(run_with_dts(s, conserved_from_density(rho + eps * d, u, v, p, 1.4), dts)))
+        Jm = float(obj(run_with_dts(s, conserved_from_density(rho - eps * d, u, v, p, 1.4), dts)))
+        fd = (Jp - Jm) / (2 * eps)
+        ad = float((grad * d).sum())
+        errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
+    s.close()
+    return {"workload": f"shock_bubble_{nx}x{ny}_M1.22_hllc_o2_cfl{cfl:g}_{nsteps}steps_K{K}",
+            "forward_ms": fwd_ms, "forward_checkpointed_ms": fwd_chk_ms, "adjoint_ms": adj_ms,
+            "adjoint_over_forward": adj_ms / fwd_ms, "steps": nsteps, "checkpoint_every": K,
+            "fd_directions": ndirs, "fd_eps": eps, "fd_rel_err_max": max(errs),
+            "fd_rel_err_median": sorted(errs)[len(errs) // 2]}
+
+
+def bench_ensemble(dev, riemann: str, stop_time: float = 0.1, cfl: float = 0.8) -> dict:
+    """The 64 seeded Riemann strips (tests/golden/ensemble_64.json), each run
+    to stop_time: autotuned (mode, tile) table and the best run's throughput."""
+    import numpy as np
+    from ecamd.batch import Ensemble, riemann_strip_members, tile_shapes
+    table = json.loads((ROOT / "tests" / "golden" / "ensemble_64.json").read_text())
+    members = table["members"]
+    ens = Ensemble(riemann_strip_members(members, cfl, stop_time, 100000), gamma=1.4,
+                   riemann=riemann, device=dev)
+    g = 1.4
+    for k, m in enumerate(members):
+        nx = m["nx"]
+        xc = (np.arange(nx) + 0.5) / nx
+        U = np.empty((4, 4, nx))
+        for side, sel in (("left", xc < 0.5), ("right", xc >= 0.5)):
+            rho, uu, vv, pp = m[side]
+            cons = (rho, rho * uu, rho * vv, pp / (g - 1.0) + 0.5 * rho * (uu * uu + vv * vv))
+            for c in range(4):
+                U[c][:, sel] = cons[c]
+        ens.set_state(k, U)
+    tuned = ens.autotune()
+    res, used = ens.run()
+    cell_updates = sum(r.step * m["nx"] * 4 for r, m in zip(res, members))
+    shapes = tile_shapes()
+    out = {"workload": f"riemann_strips_64_nx100-1600x4_o2_minmod_{riemann}_cfl{cfl:g}_t{stop_time:g}",
+           "members": len(members), "cell_updates": cell_updates,
+           "max_member_steps": max(r.step for r in res), "best": {"mode": used.mode,
+           "tile": shapes[used.tile_id], "ms": used.ms},
+           "cell_updates_per_s": cell_updates / (used.ms / 1e3),
+           "autotune": [{"mode": t.mode, "tile": shapes[t.tile_id], "ms": t.ms} for t in tuned],
+           "statuses_ok": all(r.status == 0 for r in res)}
+    ens.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +282,10 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=-1, help="explicit tile shape id (-1: default)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the IMEX, adjoint and ensemble measurements")
+    ap.add_argument("--only", choices=("imex", "adjoint", "ensemble"),
+                    help="print only that measurement")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -108,6 +301,11 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     n = args.n
+    if args.only:
+        fn = {"imex": lambda: bench_imex(dev), "adjoint": lambda: bench_adjoint(dev),
+              "ensemble": lambda: {r: bench_ensemble(dev, r) for r in ("xie_am_hllc_p", "hllc")}}
+        print(json.dumps({args.only: fn[args.only]()}))
+        return 0
     if args.mode == "copy":
         return copy_calibration(dev, n, args.steps)
     grid = ecamd.Grid(n, n, 1.0 / n, 1.0 / n, 1.4, args.riemann, "minmod", args.order)
@@ -173,6 +371,10 @@ def main() -> int:
         "wall_s": wall,
         "sim_time_advanced": run.time,
     }
+    if not args.no_extras:
+        line["imex"] = bench_imex(dev)
+        line["adjoint"] = bench_adjoint(dev)
+        line["ensemble"] = {r: bench_ensemble(dev, r) for r in ("xie_am_hllc_p", "hllc")}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(n, args.mach, args.cfl, args.cpu_steps)
     print(json.dumps(line))

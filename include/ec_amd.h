@@ -261,6 +261,65 @@ ec_status ec_flux_vjp_host(const double* qL, const double* qR, int axis, double 
                            int riemann, double sensor_f, const double* Fbar, double* qLbar,
                            double* qRbar, double* sensor_bar);
 
+/* -------------------------------------------------------------- ensembles */
+/* A batch of independent explicit runs of uneven resolution (Riemann strips,
+ * Gresho boxes, ...) stepped together on one GPU.  No reference counterpart:
+ * the reference runs one problem per process (app/driver_main.hpp:17-333).
+ * Every member follows that driver loop exactly as ec_explicit_advance does
+ * on the member alone (same dt sequence, statuses and bit-identical states);
+ * the members share gamma, riemann, limiter and spatial order. */
+typedef struct {
+  int nx, ny;               /* valid cells */
+  double dx, dy;
+  int bc[4];                /* ec_bc for x-low, x-high, y-low, y-high */
+  double fixed_state[4][4]; /* conserved state of an EC_BC_FIXED side */
+  double cfl;               /* euler.cfl */
+  double stop_time;         /* the run starts at time 0 */
+  int max_step;
+} ec_member_desc;
+
+typedef struct ec_batch ec_batch;
+
+/* How rounds (one step of every running member) are separated:
+ * GRAPH: one kernel launch per round, rounds captured in a hipGraph and
+ * replayed; PERSISTENT: one cooperative launch that loops over rounds with
+ * bounded per-member waits; AUTO: the autotuner's choice. */
+typedef enum { EC_BATCH_AUTO = 0, EC_BATCH_GRAPH = 1, EC_BATCH_PERSISTENT = 2 } ec_batch_mode;
+
+typedef struct {
+  int mode;     /* ec_batch_mode */
+  int tile_id;  /* ec_batch_tile_shape index */
+  double ms;    /* wall time of one complete run, -1 if the pair cannot run */
+} ec_batch_timing;
+
+/* Creates the batch on the current HIP device; member states start at zero.
+ * stream: the caller's hipStream_t (or NULL); the batch orders its own
+ * stream after it on every call. */
+ec_status ec_batch_create(const ec_member_desc* members, int n_members, double gamma,
+                          int riemann, int limiter, int spatial_order, void* stream,
+                          ec_batch** out);
+ec_status ec_batch_destroy(ec_batch* b);
+const char* ec_batch_last_error(const ec_batch* b);
+/* Copies a member's state in or out; the pointer may be host or device
+ * memory with the given layout.  Synchronous. */
+ec_status ec_batch_set_state(ec_batch* b, int member, const double* src, const ec_layout* layout);
+ec_status ec_batch_get_state(ec_batch* b, int member, double* dst, const ec_layout* layout);
+/* Compiled tile shapes of the ensemble kernels. */
+int ec_batch_tile_count(void);
+ec_status ec_batch_tile_shape(int tile_id, int* tx, int* ty, int* threads);
+/* mode: ec_batch_mode; tile_id -1 lets the autotuner choose; rounds_per_graph
+ * (even, 0 keeps the current value) is the number of rounds per hipGraph. */
+ec_status ec_batch_configure(ec_batch* b, int mode, int tile_id, int rounds_per_graph);
+/* Times one complete run of every (mode, tile shape) pair on a copy of the
+ * current states (which are restored) and keeps the fastest.  table may be
+ * NULL; *n_entries receives the number of pairs timed. */
+ec_status ec_batch_autotune(ec_batch* b, ec_batch_timing* table, int table_cap, int* n_entries);
+/* Runs every member from its current state at time 0 until stop_time,
+ * max_step or a failure status; the final states replace the current ones.
+ * results (n_members entries, may be NULL) receive the members' run states;
+ * used (may be NULL) the mode, tile shape and wall time of the run. */
+ec_status ec_batch_run(ec_batch* b, ec_run_state* results, ec_batch_timing* used);
+
 /* ---------------------------------------------------- host initial data */
 /* Test problems of problems/problem_initialization.hpp:420-435 (selector
  * euler.problem, core/config_parse_helpers.hpp:11-42), plus
