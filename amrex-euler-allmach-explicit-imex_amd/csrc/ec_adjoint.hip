@@ -2,6 +2,7 @@
 // pointwise flux / VJP entry points used to verify the hand-written reverse
 // mode against finite differences.
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "ec_adjoint.hpp"
@@ -74,6 +75,7 @@ struct AdjWork {
   double* ub = nullptr;    // 4 comps x GX*GY conserved bars of grown cells
   int* sel = nullptr;      // HLLC-P: sensor argmin per flux face
   double* mbar = nullptr;  // HLLC-P: bar of the selected ratio per flux face
+  double* q = nullptr;     // 12 x GX*GY: primitives, x-slopes, y-slopes of grown cells
 };
 
 void adjoint_release(ec_ctx* ctx)
@@ -84,6 +86,7 @@ void adjoint_release(ec_ctx* ctx)
   if (w->ub) (void)hipFree(w->ub);
   if (w->sel) (void)hipFree(w->sel);
   if (w->mbar) (void)hipFree(w->mbar);
+  if (w->q) (void)hipFree(w->q);
   if (w->states) (void)hipFree(w->states);
   if (w->lam) (void)hipFree(w->lam);
   if (w->dtlog) (void)hipFree(w->dtlog);
@@ -104,7 +107,8 @@ static ec_status adjoint_workspace(ec_ctx* ctx, AdjWork** out)
     bool ok = hipMalloc((void**)&w->qb, 16 * G * sizeof(double)) == hipSuccess &&
               hipMalloc((void**)&w->ub, 4 * G * sizeof(double)) == hipSuccess &&
               hipMalloc((void**)&w->sel, nf * sizeof(int)) == hipSuccess &&
-              hipMalloc((void**)&w->mbar, nf * sizeof(double)) == hipSuccess;
+              hipMalloc((void**)&w->mbar, nf * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&w->q, 12 * G * sizeof(double)) == hipSuccess;
     if (!ok) {
       adjoint_release(ctx);
       return EC_ERR_OUT_OF_MEMORY;
@@ -125,6 +129,7 @@ struct AdjArgs {
   double* ub;
   int* sel;
   double* mbar;
+  double* q;  // 12 x GX*GY: primitives, x-slopes, y-slopes (written by the prep pass)
 };
 
 // Face-state bar storage: side 0 E (x,+), 1 W (x,-), 2 N (y,+), 3 S (y,-).
@@ -139,25 +144,58 @@ EC_HD double* qb_at(const AdjArgs& A, int side, int i, int j)
   return A.qb + (long long)side * 4 * G + gidx(A.g, i, j);
 }
 
-EC_HD Prim prim_at(const AdjArgs& A, int i, int j)
+// Primitive state of grown cell (i,j) through the explicit ghost rule.
+EC_HD Prim prim_eval(const AdjArgs& A, int i, int j)
 {
   return to_prim(load_cell(A.U, A.rs, A.cs, A.g, i, j), A.g.gamma);
 }
 
-EC_HD Prim slope_of(const AdjArgs& A, int i, int j, int axis)
+EC_HD Prim slope_eval(const AdjArgs& A, int i, int j, int axis, const Prim& c)
 {
   const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
-  const Prim m = prim_at(A, i - di, j - dj), c = prim_at(A, i, j), p = prim_at(A, i + di, j + dj);
+  const Prim m = prim_eval(A, i - di, j - dj), p = prim_eval(A, i + di, j + dj);
   const int lim = A.g.limiter;
   return Prim{limited_slope(c.r - m.r, p.r - c.r, lim), limited_slope(c.u - m.u, p.u - c.u, lim),
               limited_slope(c.v - m.v, p.v - c.v, lim), limited_slope(c.p - m.p, p.p - c.p, lim)};
 }
 
+EC_HD long long grown_count(const GridParams& g) { return (long long)(g.nx + 4) * (g.ny + 4); }
+
+EC_HD Prim qslot(const AdjArgs& A, int block, int i, int j)
+{
+  const long long G = grown_count(A.g);
+  const double* b = A.q + (long long)block * 4 * G + gidx(A.g, i, j);
+  return Prim{b[0], b[G], b[2 * G], b[3 * G]};
+}
+
+// Prep pass: the primitives of every grown cell and, for O2, the limited
+// slopes of the cells that own face states ([-1,nx] x [-1,ny]), computed once
+// per step (the forward step's stages A and B) for the passes below.
+EC_HD void adj_prep_item(const AdjArgs& A, long long t)
+{
+  const GridParams& g = A.g;
+  const int GX = g.nx + 4;
+  const long long G = grown_count(g);
+  const int i = (int)(t % GX) - 2, j = (int)(t / GX) - 2;
+  const Prim c = prim_eval(A, i, j);
+  double* q = A.q + t;
+  q[0] = c.r; q[G] = c.u; q[2 * G] = c.v; q[3 * G] = c.p;
+  if (g.order == 2 && i >= -1 && j >= -1 && i <= g.nx && j <= g.ny) {
+    for (int axis = 0; axis < 2; ++axis) {
+      const Prim s = slope_eval(A, i, j, axis, c);
+      double* d = A.q + (long long)(4 + 4 * axis) * G + t;
+      d[0] = s.r; d[G] = s.u; d[2 * G] = s.v; d[3 * G] = s.p;
+    }
+  }
+}
+
+EC_HD Prim prim_at(const AdjArgs& A, int i, int j) { return qslot(A, 0, i, j); }
+
 EC_HD Prim face_state_at(const AdjArgs& A, int i, int j, int axis, double hs)
 {
   Prim q = prim_at(A, i, j);
   if (A.g.order == 2) {
-    const Prim s = slope_of(A, i, j, axis);
+    const Prim s = qslot(A, axis == 0 ? 1 : 2, i, j);
     q.r = q.r + hs * s.r;
     q.u = q.u + hs * s.u;
     q.v = q.v + hs * s.v;
@@ -180,23 +218,16 @@ EC_HD double ratio_of(const AdjArgs& A, int fi, int fj, int axis)
                         face_state_at(A, fi, fj, axis, -0.5).p);
 }
 
-EC_HD void adj_face_item(const AdjArgs& A, long long t)
+// One flux face's VJP; the axis and the Riemann solver are compile-time
+// constants (RIEM < 0: read from the grid at run time, the host path).
+template <int axis, int RIEM>
+EC_HD void adj_face_body(const AdjArgs& A, long long t, int fi, int fj)
 {
   const GridParams& g = A.g;
-  const long long nxf = (long long)(g.nx + 1) * g.ny;
   {
-    int axis, fi, fj;
-    if (t < nxf) {
-      axis = 0;
-      fi = (int)(t % (g.nx + 1));
-      fj = (int)(t / (g.nx + 1));
-    } else {
-      axis = 1;
-      fi = (int)((t - nxf) % g.nx);
-      fj = (int)((t - nxf) / g.nx);
-    }
     const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
     const int li = fi - di, lj = fj - dj;
+    const int riemann = RIEM >= 0 ? RIEM : g.riemann;
     const double s = (axis == 0 ? g.inv_dx : g.inv_dy) * A.dt;
     const Cons lr = lam_at(A, fi, fj), ll = lam_at(A, li, lj);
     const Cons Fb{s * (lr.r - ll.r), s * (lr.mx - ll.mx), s * (lr.my - ll.my), s * (lr.E - ll.E)};
@@ -207,7 +238,7 @@ EC_HD void adj_face_item(const AdjArgs& A, long long t)
       const Prim L = face_state_at(A, li, lj, axis, 0.5);
       const Prim R = face_state_at(A, fi, fj, axis, -0.5);
       double sensor = 1.0, m = 1.0;
-      if (g.riemann == 3) {
+      if (riemann == 3) {
         // reconstruction_fluxes.hpp:421-440, the first minimal ratio wins
         double r[5];
         if (axis == 0) {
@@ -235,8 +266,8 @@ EC_HD void adj_face_item(const AdjArgs& A, long long t)
         sensor = c * c * c;
       }
       double sb = 0.0;
-      interface_flux_vjp(L, R, axis, g.gamma, g.riemann, sensor, Fb, Lb, Rb, sb);
-      if (g.riemann == 3) {
+      interface_flux_vjp(L, R, axis, g.gamma, riemann, sensor, Fb, Lb, Rb, sb);
+      if (riemann == 3) {
         const double c = clamp01(m);
         mb = clamp01_vjp(m, 3.0 * c * c * sb);
       }
@@ -253,12 +284,68 @@ EC_HD void adj_face_item(const AdjArgs& A, long long t)
   }
 }
 
-__global__ void adj_faces_kernel(AdjArgs A)
+// Number of items of the face pass: every flux face, plus, for HLLC-P, the
+// faces outside the flux set whose pressure ratios the 5-face sensor reads
+// (x-faces of rows -1 and ny, y-faces of columns -1 and nx).
+EC_HD long long face_items(const GridParams& g)
 {
-  const long long nf = (long long)(A.g.nx + 1) * A.g.ny + (long long)A.g.nx * (A.g.ny + 1);
+  const long long nf = (long long)(g.nx + 1) * g.ny + (long long)g.nx * (g.ny + 1);
+  return g.riemann == 3 ? nf + 2LL * (g.nx + 1) + 2LL * (g.ny + 1) : nf;
+}
+
+EC_HD void zero_slot(const AdjArgs& A, int side, int i, int j)
+{
+  const long long G = grown_count(A.g);
+  double* p = qb_at(A, side, i, j);
+  p[0] = 0.0; p[G] = 0.0; p[2 * G] = 0.0; p[3 * G] = 0.0;
+}
+
+template <int RIEM = -1>
+EC_HD void adj_face_item(const AdjArgs& A, long long t)
+{
+  const GridParams& g = A.g;
+  const long long nxf = (long long)(g.nx + 1) * g.ny;
+  const long long nf = nxf + (long long)g.nx * (g.ny + 1);
+  if (t >= nf) {
+    // a ratio-only face: its face-state bars start at zero (the ratio pass
+    // adds the pressure part)
+    const long long e = t - nf;
+    if (e < 2LL * (g.nx + 1)) {
+      const int fi = (int)(e % (g.nx + 1)), fj = e / (g.nx + 1) == 0 ? -1 : g.ny;
+      zero_slot(A, 0, fi - 1, fj);
+      zero_slot(A, 1, fi, fj);
+    } else {
+      const long long e2 = e - 2LL * (g.nx + 1);
+      const int fj = (int)(e2 % (g.ny + 1)), fi = e2 / (g.ny + 1) == 0 ? -1 : g.nx;
+      zero_slot(A, 2, fi, fj - 1);
+      zero_slot(A, 3, fi, fj);
+    }
+    return;
+  }
+  // the axis is a compile-time constant of each body (its selects fold away)
+  if (t < nxf) {
+    adj_face_body<0, RIEM>(A, t, (int)(t % (g.nx + 1)), (int)(t / (g.nx + 1)));
+  } else {
+    adj_face_body<1, RIEM>(A, t, (int)((t - nxf) % g.nx), (int)((t - nxf) / g.nx));
+  }
+}
+
+__global__ void __launch_bounds__(256) adj_prep_kernel(AdjArgs A)
+{
+  const long long G = grown_count(A.g);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < G;
+       t += (long long)gridDim.x * blockDim.x) {
+    adj_prep_item(A, t);
+  }
+}
+
+template <int RIEM>
+__global__ void __launch_bounds__(256) adj_faces_kernel(AdjArgs A)
+{
+  const long long nf = face_items(A.g);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nf;
        t += (long long)gridDim.x * blockDim.x) {
-    adj_face_item(A, t);
+    adj_face_item<RIEM>(A, t);
   }
 }
 
@@ -309,7 +396,7 @@ EC_HD void adj_ratio_item(const AdjArgs& A, long long t)
   }
 }
 
-__global__ void adj_ratios_kernel(AdjArgs A)
+__global__ void __launch_bounds__(256) adj_ratios_kernel(AdjArgs A)
 {
   const long long nr = (long long)(A.g.nx + 1) * (A.g.ny + 2) + (long long)(A.g.nx + 2) * (A.g.ny + 1);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nr;
@@ -318,9 +405,22 @@ __global__ void adj_ratios_kernel(AdjArgs A)
   }
 }
 
+// Whether the face pass wrote slot (side, i, j) this step; every other slot
+// reads as zero (no face state of that side exists there).
+EC_HD bool slot_written(const GridParams& g, int side, int i, int j)
+{
+  const bool hp = g.riemann == 3;
+  if (side < 2) {
+    const bool row = (j >= 0 && j < g.ny) || (hp && (j == -1 || j == g.ny));
+    return row && (side == 0 ? (i >= -1 && i < g.nx) : (i >= 0 && i <= g.nx));
+  }
+  const bool col = (i >= 0 && i < g.nx) || (hp && (i == -1 || i == g.nx));
+  return col && (side == 2 ? (j >= -1 && j < g.ny) : (j >= 0 && j <= g.ny));
+}
+
 EC_HD Prim qb_load(const AdjArgs& A, int side, int i, int j)
 {
-  if (i < -2 || j < -2 || i > A.g.nx + 1 || j > A.g.ny + 1) return Prim{0.0, 0.0, 0.0, 0.0};
+  if (!slot_written(A.g, side, i, j)) return Prim{0.0, 0.0, 0.0, 0.0};
   const long long G = (long long)(A.g.nx + 4) * (A.g.ny + 4);
   const double* p = qb_at(A, side, i, j);
   return Prim{p[0], p[G], p[2 * G], p[3 * G]};
@@ -331,6 +431,13 @@ EC_HD Prim slope_bar(const AdjArgs& A, int i, int j, int axis)
 {
   const Prim a = qb_load(A, axis == 0 ? 0 : 2, i, j), b = qb_load(A, axis == 0 ? 1 : 3, i, j);
   return Prim{0.5 * (a.r - b.r), 0.5 * (a.u - b.u), 0.5 * (a.v - b.v), 0.5 * (a.p - b.p)};
+}
+
+// Valid cells at least 2 cells from every side: no ghost cell is read from
+// them, so their adjoint is complete after the cell pass.
+EC_HD bool fold_interior(const GridParams& g, int i, int j)
+{
+  return i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3;
 }
 
 EC_HD void adj_cell_item(const AdjArgs& A, long long t)
@@ -378,6 +485,13 @@ EC_HD void adj_cell_item(const AdjArgs& A, long long t)
     if (qbar.r != 0.0 || qbar.u != 0.0 || qbar.v != 0.0 || qbar.p != 0.0) {
       to_prim_vjp(load_cell(A.U, A.rs, A.cs, g, i, j), g.gamma, qbar, ubar);
     }
+    if (fold_interior(g, i, j)) {
+      // no ghost cell reads (i,j): lambda_n = lambda_next + its own bar
+      const long long k = (long long)j * A.rs + i;
+      const double acc[4] = {0.0 + ubar.r, 0.0 + ubar.mx, 0.0 + ubar.my, 0.0 + ubar.E};
+      for (int c = 0; c < 4; ++c) A.lam0[c * A.cs + k] = A.lam1[c * A.cs + k] + acc[c];
+      return;
+    }
     A.ub[t] = ubar.r;
     A.ub[G + t] = ubar.mx;
     A.ub[2 * G + t] = ubar.my;
@@ -385,13 +499,98 @@ EC_HD void adj_cell_item(const AdjArgs& A, long long t)
   }
 }
 
-__global__ void adj_cells_kernel(AdjArgs A)
+// The cell pass as LDS tiles of CTX x CTY grown cells (one per thread): the
+// primitives of the tile plus a 2-cell ring and the slope bars of the tile
+// plus a 1-cell ring are staged once, so each limiter gradient and slot is
+// read from LDS rather than recomputed per neighbour.  Same arithmetic, in
+// the same order, as adj_cell_item.
+constexpr int CTX = 32, CTY = 8, CNT = CTX * CTY;
+__global__ void __launch_bounds__(CNT) adj_cells_tile_kernel(AdjArgs A)
 {
-  const long long G = (long long)(A.g.nx + 4) * (A.g.ny + 4);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < G;
-       t += (long long)gridDim.x * blockDim.x) {
-    adj_cell_item(A, t);
+  constexpr int QX = CTX + 4, QY = CTY + 4, NQ = QX * QY;  // cells [-2, CT+1]
+  constexpr int BX = CTX + 2, BY = CTY + 2, NB = BX * BY;  // cells [-1, CT]
+  __shared__ double qs[4][NQ];
+  __shared__ double sbs[8][NB];
+  const GridParams& g = A.g;
+  const int GX = g.nx + 4, GY = g.ny + 4;
+  const int tiles_x = (GX + CTX - 1) / CTX;
+  const int gi0 = (int)(blockIdx.x % tiles_x) * CTX - 2, gj0 = (int)(blockIdx.x / tiles_x) * CTY - 2;
+  const long long G = grown_count(g);
+  const bool o2 = g.order == 2;
+  if (o2) {
+    for (int t = threadIdx.x; t < NQ; t += CNT) {
+      const int i = gi0 - 2 + t % QX, j = gj0 - 2 + t / QX;
+      if (i >= -2 && j >= -2 && i <= g.nx + 1 && j <= g.ny + 1) {
+        const long long k = gidx(g, i, j);
+        qs[0][t] = A.q[k];
+        qs[1][t] = A.q[G + k];
+        qs[2][t] = A.q[2 * G + k];
+        qs[3][t] = A.q[3 * G + k];
+      }
+    }
+    for (int t = threadIdx.x; t < NB; t += CNT) {
+      const int i = gi0 - 1 + t % BX, j = gj0 - 1 + t / BX;
+      const Prim sx = slope_bar(A, i, j, 0), sy = slope_bar(A, i, j, 1);
+      sbs[0][t] = sx.r; sbs[1][t] = sx.u; sbs[2][t] = sx.v; sbs[3][t] = sx.p;
+      sbs[4][t] = sy.r; sbs[5][t] = sy.u; sbs[6][t] = sy.v; sbs[7][t] = sy.p;
+    }
+    __syncthreads();
   }
+  const int li = threadIdx.x % CTX, lj = threadIdx.x / CTX;
+  const int i = gi0 + li, j = gj0 + lj;
+  if (i > g.nx + 1 || j > g.ny + 1) return;
+  auto qat = [&](int ci, int cj) {
+    const int t = (cj - gj0 + 2) * QX + (ci - gi0 + 2);
+    return Prim{qs[0][t], qs[1][t], qs[2][t], qs[3][t]};
+  };
+  Prim qbar{0.0, 0.0, 0.0, 0.0};
+  for (int side = 0; side < 4; ++side) add(qbar, qb_load(A, side, i, j));
+  if (o2) {
+    for (int axis = 0; axis < 2; ++axis) {
+      const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
+      for (int o = -1; o <= 1; ++o) {
+        const int ci = i + o * di, cj = j + o * dj;
+        if (ci < -1 || cj < -1 || ci > g.nx || cj > g.ny) continue;  // no face states there
+        const int bt = (cj - gj0 + 1) * BX + (ci - gi0 + 1);
+        const Prim sb{sbs[4 * axis][bt], sbs[4 * axis + 1][bt], sbs[4 * axis + 2][bt],
+                      sbs[4 * axis + 3][bt]};
+        if (sb.r == 0.0 && sb.u == 0.0 && sb.v == 0.0 && sb.p == 0.0) continue;
+        const Prim qm = qat(ci - di, cj - dj), qc = qat(ci, cj), qp = qat(ci + di, cj + dj);
+        const double bk[4] = {qc.r - qm.r, qc.u - qm.u, qc.v - qm.v, qc.p - qm.p};
+        const double fw[4] = {qp.r - qc.r, qp.u - qc.u, qp.v - qc.v, qp.p - qc.p};
+        const double sbv[4] = {sb.r, sb.u, sb.v, sb.p};
+        double contrib[4];
+        for (int c = 0; c < 4; ++c) {
+          double db, df;
+          limited_slope_grad(bk[c], fw[c], g.limiter, db, df);
+          double d = 0.0;
+          if (o == 0) d = db - df;
+          else if (o == 1) d = -db;
+          else d = df;
+          contrib[c] = sbv[c] * d;
+        }
+        qbar.r += contrib[0];
+        qbar.u += contrib[1];
+        qbar.v += contrib[2];
+        qbar.p += contrib[3];
+      }
+    }
+  }
+  Cons ubar{0.0, 0.0, 0.0, 0.0};
+  if (qbar.r != 0.0 || qbar.u != 0.0 || qbar.v != 0.0 || qbar.p != 0.0) {
+    to_prim_vjp(load_cell(A.U, A.rs, A.cs, g, i, j), g.gamma, qbar, ubar);
+  }
+  if (fold_interior(g, i, j)) {
+    const long long k = (long long)j * A.rs + i;
+    const double acc[4] = {0.0 + ubar.r, 0.0 + ubar.mx, 0.0 + ubar.my, 0.0 + ubar.E};
+    for (int c = 0; c < 4; ++c) A.lam0[c * A.cs + k] = A.lam1[c * A.cs + k] + acc[c];
+    return;
+  }
+  const long long t = gidx(g, i, j);
+  A.ub[t] = ubar.r;
+  A.ub[G + t] = ubar.mx;
+  A.ub[2 * G + t] = ubar.my;
+  A.ub[3 * G + t] = ubar.E;
 }
 
 // Where the explicit ghost rule reads grown cell (i,j) from: valid index and
@@ -408,41 +607,87 @@ EC_HD bool ghost_source(const GridParams& g, int i, int j, int& ii, int& jj,
   return true;
 }
 
-EC_HD void adj_fold_item(const AdjArgs& A, long long t)
+// Fold of the ghost-cell bars onto valid cell (i,j) (the transpose of the
+// ghost rule): lambda_n = lambda_next + the bars of every grown cell the
+// explicit ghost rule reads from (i,j), with the reflected momentum negated.
+EC_HD void adj_fold_cell(const AdjArgs& A, int i, int j)
 {
   const GridParams& g = A.g;
-  const int GX = g.nx + 4;
-  const long long G = (long long)GX * (g.ny + 4);
-  {
-    const int i = (int)(t % g.nx), j = (int)(t / g.nx);
-    const int cx[5] = {i, -2, -1, g.nx, g.nx + 1};
-    const int cy[5] = {j, -2, -1, g.ny, g.ny + 1};
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = 0; b < 5; ++b) {
-      for (int a = 0; a < 5; ++a) {
-        const int gi = cx[a], gj = cy[b];
-        if ((a > 0 && gi == i) || (b > 0 && gj == j)) continue;  // duplicate candidate
-        int ii, jj;
-        bool rx, ry;
-        if (!ghost_source(g, gi, gj, ii, jj, rx, ry) || ii != i || jj != j) continue;
-        const long long k = gidx(g, gi, gj);
-        acc[0] += A.ub[k];
-        acc[1] += rx ? -A.ub[G + k] : A.ub[G + k];
-        acc[2] += ry ? -A.ub[2 * G + k] : A.ub[2 * G + k];
-        acc[3] += A.ub[3 * G + k];
-      }
-    }
-    const long long k = (long long)j * A.rs + i;
-    for (int c = 0; c < 4; ++c) A.lam0[c * A.cs + k] = A.lam1[c * A.cs + k] + acc[c];
+  const long long G = grown_count(g);
+  // grown columns / rows that can read (i,j): itself, then the ghosts in the
+  // order -2, -1, n, n+1 (a ghost equal to i or j is the same candidate)
+  const int ax[5] = {i, -2, -1, g.nx, g.nx + 1};
+  const int ay[5] = {j, -2, -1, g.ny, g.ny + 1};
+  int cx[5], cy[5], nxc = 0, nyc = 0;
+  for (int a = 0; a < 5; ++a) {
+    if (a > 0 && ax[a] == i) continue;
+    bool r;
+    if (map_axis(ax[a], g.nx, g.bc[0], g.bc[1], r) == i) cx[nxc++] = ax[a];
   }
+  for (int b = 0; b < 5; ++b) {
+    if (b > 0 && ay[b] == j) continue;
+    bool r;
+    if (map_axis(ay[b], g.ny, g.bc[2], g.bc[3], r) == j) cy[nyc++] = ay[b];
+  }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < nyc; ++b) {
+    for (int a = 0; a < nxc; ++a) {
+      int ii, jj;
+      bool rx, ry;
+      if (!ghost_source(g, cx[a], cy[b], ii, jj, rx, ry) || ii != i || jj != j) continue;
+      const long long k = gidx(g, cx[a], cy[b]);
+      acc[0] += A.ub[k];
+      acc[1] += rx ? -A.ub[G + k] : A.ub[G + k];
+      acc[2] += ry ? -A.ub[2 * G + k] : A.ub[2 * G + k];
+      acc[3] += A.ub[3 * G + k];
+    }
+  }
+  const long long k = (long long)j * A.rs + i;
+  for (int c = 0; c < 4; ++c) A.lam0[c * A.cs + k] = A.lam1[c * A.cs + k] + acc[c];
 }
 
-__global__ void adj_fold_kernel(AdjArgs A)
+EC_HD void adj_fold_item(const AdjArgs& A, long long t)
 {
-  const long long n = (long long)A.g.nx * A.g.ny;
+  const int i = (int)(t % A.g.nx), j = (int)(t / A.g.nx);
+  if (fold_interior(A.g, i, j)) return;  // written by the cell pass
+  adj_fold_cell(A, i, j);
+}
+
+// The valid cells that are not fold-interior: rows 0, 1, ny-2, ny-1 and
+// columns 0, 1, nx-2, nx-1 (all cells when the grid is narrower than 5).
+EC_HD long long ring_count(const GridParams& g)
+{
+  if (g.nx < 5 || g.ny < 5) return (long long)g.nx * g.ny;
+  return 4LL * g.nx + 4LL * (g.ny - 4);
+}
+
+EC_HD void ring_cell(const GridParams& g, long long t, int& i, int& j)
+{
+  if (g.nx < 5 || g.ny < 5) {
+    i = (int)(t % g.nx);
+    j = (int)(t / g.nx);
+    return;
+  }
+  if (t < 4LL * g.nx) {
+    const int r = (int)(t / g.nx);
+    i = (int)(t % g.nx);
+    j = r < 2 ? r : g.ny - 4 + r;
+    return;
+  }
+  t -= 4LL * g.nx;
+  const int c = (int)(t / (g.ny - 4));
+  j = 2 + (int)(t % (g.ny - 4));
+  i = c < 2 ? c : g.nx - 4 + c;
+}
+
+__global__ void __launch_bounds__(256) adj_fold_kernel(AdjArgs A)
+{
+  const long long n = ring_count(A.g);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
-    adj_fold_item(A, t);
+    int i, j;
+    ring_cell(A.g, t, i, j);
+    if (!fold_interior(A.g, i, j)) adj_fold_cell(A, i, j);
   }
 }
 
@@ -453,26 +698,73 @@ static int adj_blocks(long long n)
   return b < 1 ? 1 : (int)b;
 }
 
-ec_status launch_adjoint_step(ec_ctx* ctx, const double* U, const double* lam1, double* lam0,
-                              long long rs, long long cs, double dt)
+// The fold of one adjoint step and the prep pass of the next (earlier) one
+// in a single launch: they touch disjoint data (the fold reads the bars and
+// lambda of the later step, the prep writes the primitives of the earlier
+// state), so the fold's latency hides behind the prep's work.
+__global__ void __launch_bounds__(256) adj_fold_prep_kernel(AdjArgs F, AdjArgs P, int fold_blocks)
+{
+  if ((int)blockIdx.x < fold_blocks) {
+    const long long n = ring_count(F.g);
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)fold_blocks * blockDim.x) {
+      int i, j;
+      ring_cell(F.g, t, i, j);
+      if (!fold_interior(F.g, i, j)) adj_fold_cell(F, i, j);
+    }
+    return;
+  }
+  const long long G = grown_count(P.g);
+  const long long nb = (long long)gridDim.x - fold_blocks;
+  for (long long t = ((long long)blockIdx.x - fold_blocks) * blockDim.x + threadIdx.x; t < G;
+       t += nb * blockDim.x) {
+    adj_prep_item(P, t);
+  }
+}
+
+static ec_status adjoint_args(ec_ctx* ctx, const double* U, const double* lam1, double* lam0,
+                              long long rs, long long cs, double dt, AdjArgs* out)
 {
   AdjWork* w = nullptr;
   ec_status st = adjoint_workspace(ctx, &w);
   if (st != EC_OK) return st;
-  AdjArgs A{U, lam1, lam0, rs, cs, ctx->grid, dt, w->qb, w->ub,
-            ctx->grid.riemann == 3 ? w->sel : nullptr, w->mbar};
-  const GridParams& g = ctx->grid;
-  const long long G = (long long)(g.nx + 4) * (g.ny + 4);
-  EC_HIP_TRY(ctx, hipMemsetAsync(w->qb, 0, 16 * G * sizeof(double), ctx->stream));
-  const long long nf = (long long)(g.nx + 1) * g.ny + (long long)g.nx * (g.ny + 1);
-  hipLaunchKernelGGL(adj_faces_kernel, dim3(adj_blocks(nf)), dim3(256), 0, ctx->stream, A);
+  *out = AdjArgs{U, lam1, lam0, rs, cs, ctx->grid, dt, w->qb, w->ub,
+                 ctx->grid.riemann == 3 ? w->sel : nullptr, w->mbar, w->q};
+  return EC_OK;
+}
+
+// Face, ratio and cell passes of one step (the prep pass must have run).
+static ec_status adjoint_passes(ec_ctx* ctx, const AdjArgs& A)
+{
+  const GridParams& g = A.g;
+  const dim3 fgrid(adj_blocks(face_items(g)));
+  switch (g.riemann) {
+    case 0: hipLaunchKernelGGL(adj_faces_kernel<0>, fgrid, dim3(256), 0, ctx->stream, A); break;
+    case 1: hipLaunchKernelGGL(adj_faces_kernel<1>, fgrid, dim3(256), 0, ctx->stream, A); break;
+    case 2: hipLaunchKernelGGL(adj_faces_kernel<2>, fgrid, dim3(256), 0, ctx->stream, A); break;
+    default: hipLaunchKernelGGL(adj_faces_kernel<3>, fgrid, dim3(256), 0, ctx->stream, A); break;
+  }
   if (g.riemann == 3) {
     const long long nr = (long long)(g.nx + 1) * (g.ny + 2) + (long long)(g.nx + 2) * (g.ny + 1);
     hipLaunchKernelGGL(adj_ratios_kernel, dim3(adj_blocks(nr)), dim3(256), 0, ctx->stream, A);
   }
-  hipLaunchKernelGGL(adj_cells_kernel, dim3(adj_blocks(G)), dim3(256), 0, ctx->stream, A);
-  hipLaunchKernelGGL(adj_fold_kernel, dim3(adj_blocks((long long)g.nx * g.ny)), dim3(256), 0,
-                     ctx->stream, A);
+  {
+    const int tx = (g.nx + 4 + CTX - 1) / CTX, ty = (g.ny + 4 + CTY - 1) / CTY;
+    hipLaunchKernelGGL(adj_cells_tile_kernel, dim3(tx * ty), dim3(CNT), 0, ctx->stream, A);
+  }
+  EC_HIP_TRY(ctx, hipGetLastError());
+  return EC_OK;
+}
+
+ec_status launch_adjoint_step(ec_ctx* ctx, const double* U, const double* lam1, double* lam0,
+                              long long rs, long long cs, double dt)
+{
+  AdjArgs A;
+  ec_status st = adjoint_args(ctx, U, lam1, lam0, rs, cs, dt, &A);
+  if (st != EC_OK) return st;
+  hipLaunchKernelGGL(adj_prep_kernel, dim3(adj_blocks(grown_count(A.g))), dim3(256), 0, ctx->stream, A);
+  if ((st = adjoint_passes(ctx, A)) != EC_OK) return st;
+  hipLaunchKernelGGL(adj_fold_kernel, dim3(adj_blocks(ring_count(A.g))), dim3(256), 0, ctx->stream, A);
   EC_HIP_TRY(ctx, hipGetLastError());
   return EC_OK;
 }
@@ -487,12 +779,16 @@ extern "C" ec_status ec_adjoint_step_host(const ec_grid_desc* desc, const double
   const long long G = (long long)(g.nx + 4) * (g.ny + 4);
   const long long nf = (long long)(g.nx + 1) * g.ny + (long long)g.nx * (g.ny + 1);
   const long long nr = (long long)(g.nx + 1) * (g.ny + 2) + (long long)(g.nx + 2) * (g.ny + 1);
-  std::vector<double> qb(16 * G, 0.0), ub(4 * G, 0.0), mbar(nf, 0.0);
+  // NaN-filled scratch: a pass that read a slot no earlier pass wrote would
+  // show up in the result.
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  std::vector<double> qb(16 * G, nan), ub(4 * G, nan), mbar(nf, nan), q(12 * G, nan);
   std::vector<int> sel(nf, -1);
   const long long n = (long long)g.nx * g.ny;
   AdjArgs A{U_n, lambda_next, lambda_n, g.nx, n, g, dt, qb.data(), ub.data(),
-            g.riemann == 3 ? sel.data() : nullptr, mbar.data()};
-  for (long long t = 0; t < nf; ++t) adj_face_item(A, t);
+            g.riemann == 3 ? sel.data() : nullptr, mbar.data(), q.data()};
+  for (long long t = 0; t < G; ++t) adj_prep_item(A, t);
+  for (long long t = 0; t < face_items(g); ++t) adj_face_item(A, t);
   if (g.riemann == 3) {
     for (long long t = 0; t < nr; ++t) adj_ratio_item(A, t);
   }
@@ -640,6 +936,8 @@ extern "C" ec_status ec_adjoint_checkpointed(ec_ctx* ctx, const double* checkpoi
   double* lb = w->lam + 4 * n;
   if ((st = copy_state(ctx, la, nx, n, lambda_N, lay->row_stride, lay->comp_stride)) != EC_OK) return st;
   const int nseg = (nsteps + K - 1) / K;
+  AdjArgs prev{};
+  bool pending = false;  // a step whose fold has not been launched yet
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int start = seg * K;
     const int len = nsteps - start < K ? nsteps - start : K;
@@ -648,16 +946,36 @@ extern "C" ec_status ec_adjoint_checkpointed(ec_ctx* ctx, const double* checkpoi
                                    hipMemcpyDeviceToDevice, ctx->stream));
     for (int s = 1; s < len; ++s) {
       st = launch_explicit_step(ctx, w->states + (size_t)(s - 1) * 4 * n, w->states + (size_t)s * 4 * n,
-                                nx, n, dts[start + s - 1], 0);
+                                nx, n, dts[start + s - 1], 2);
       if (st != EC_OK) return st;
     }
     for (int s = len - 1; s >= 0; --s) {
-      st = launch_adjoint_step(ctx, w->states + (size_t)s * 4 * n, la, lb, nx, n, dts[start + s]);
-      if (st != EC_OK) return st;
+      AdjArgs A;
+      if ((st = adjoint_args(ctx, w->states + (size_t)s * 4 * n, la, lb, nx, n, dts[start + s], &A)) !=
+          EC_OK) {
+        return st;
+      }
+      const long long G = grown_count(A.g);
+      if (pending) {
+        // the previous step's fold shares a launch with this step's prep
+        const int fb = adj_blocks(ring_count(A.g));
+        hipLaunchKernelGGL(adj_fold_prep_kernel, dim3(fb + adj_blocks(G)), dim3(256), 0, ctx->stream,
+                           prev, A, fb);
+      } else {
+        hipLaunchKernelGGL(adj_prep_kernel, dim3(adj_blocks(G)), dim3(256), 0, ctx->stream, A);
+      }
+      if ((st = adjoint_passes(ctx, A)) != EC_OK) return st;
+      prev = A;
+      pending = true;
       double* t = la;
       la = lb;
       lb = t;
     }
+  }
+  if (pending) {
+    hipLaunchKernelGGL(adj_fold_kernel, dim3(adj_blocks(ring_count(prev.g))), dim3(256), 0, ctx->stream,
+                       prev);
+    EC_HIP_TRY(ctx, hipGetLastError());
   }
   if ((st = copy_state(ctx, lambda_0, lay->row_stride, lay->comp_stride, la, nx, n)) != EC_OK) return st;
   EC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

@@ -23,6 +23,7 @@ struct StepArgs {
   GridParams g;
   double dt;        // used when device_dt == 0
   int device_dt;    // 1: dt from ctrl (device-resident loop)
+  int diag;         // 0: state only (no epilogue reductions, no finalize)
   DevCtrl* ctrl;
   Partial* partials;
   double* dt_log;   // optional: dt of every device-loop step, indexed by step
@@ -79,8 +80,8 @@ __global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
     store_partial(A.partials, tile, red_identity());
     return;
   }
-  const Red r = T::run(sm, A.uin, A.uout, A.rs, A.cs, A.g, i0, j0, dt);
-  store_partial(A.partials, tile, r);
+  const Red r = T::run(sm, A.uin, A.uout, A.rs, A.cs, A.g, i0, j0, dt, A.diag != 0);
+  if (A.diag) store_partial(A.partials, tile, r);
 }
 
 // Standalone max-rate + diagnostics reduction over a state (part (b) of the
@@ -152,6 +153,8 @@ int step_partials_needed(const ec_ctx* ctx, int tile_id)
   return tx * ty;
 }
 
+// device_dt: 0 host dt, 1 device-loop dt, 2 host dt without the diagnostics
+// epilogue (state only; the run control is left untouched).
 ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, long long rs,
                                long long cs, double dt, int device_dt)
 {
@@ -163,7 +166,8 @@ ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, lon
   a.cs = cs;
   a.g = ctx->grid;
   a.dt = dt;
-  a.device_dt = device_dt;
+  a.device_dt = device_dt == 1 ? 1 : 0;
+  a.diag = device_dt == 2 ? 0 : 1;
   a.ctrl = ctx->ctrl;
   a.partials = ctx->partials;
   a.dt_log = device_dt ? ctx->dt_log : nullptr;
@@ -176,8 +180,10 @@ ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, lon
   }
   step_fn(ctx->tile_id, ctx->grid.riemann, ctx->grid.order)(a, dim3(a.nblocks), dim3(s.nt),
                                                              ctx->stream);
-  hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0,
-                     ctx->stream, ctx->partials, a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
+  if (a.diag) {
+    hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0,
+                       ctx->stream, ctx->partials, a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
+  }
   EC_HIP_TRY(ctx, hipGetLastError());
   return EC_OK;
 }

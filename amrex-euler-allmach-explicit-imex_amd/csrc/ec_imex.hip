@@ -381,7 +381,7 @@ __device__ __forceinline__ Cons lf_flux(const Prim& L, const Prim& R, int axis, 
               0.5 * (fl.my + fr.my) - hs * (ur.my - ul.my), 0.5 * (fl.E + fr.E) - hs * (ur.E - ul.E)};
 }
 
-__global__ void imex_star_kernel(const double* __restrict__ U, long long rs, long long cs, GridParams g,
+__global__ void __launch_bounds__(256) imex_star_kernel(const double* __restrict__ U, long long rs, long long cs, GridParams g,
                                  double dt, int acoustic, double* __restrict__ S)
 {
   const long long n = (long long)g.nx * g.ny;
@@ -412,7 +412,7 @@ __global__ void imex_star_kernel(const double* __restrict__ U, long long rs, lon
 
 // Initial pressure from the old state, lagged momentum from the star state,
 // density floor count (imex_source_map_step.hpp:418-451).
-__global__ void imex_init_kernel(const double* __restrict__ U, long long rs, long long cs, double gamma,
+__global__ void __launch_bounds__(256) imex_init_kernel(const double* __restrict__ U, long long rs, long long cs, double gamma,
                                  int nx, int ny, const double* __restrict__ S, double* __restrict__ P,
                                  double* __restrict__ M, GmCtl* ctl)
 {
@@ -453,7 +453,7 @@ __device__ __forceinline__ void face_h(const ImexGrid& G, const double* S, const
   flux = hh * (0.5 * (ql + qr));
 }
 
-__global__ void imex_faces_kernel(ImexGrid G, const double* __restrict__ S, const double* __restrict__ M,
+__global__ void __launch_bounds__(256) imex_faces_kernel(ImexGrid G, const double* __restrict__ S, const double* __restrict__ M,
                                   const double* __restrict__ P, double* __restrict__ hx,
                                   double* __restrict__ hy, float* __restrict__ hx32,
                                   float* __restrict__ hy32, double* __restrict__ fx,
@@ -490,7 +490,7 @@ __global__ void imex_faces_kernel(ImexGrid G, const double* __restrict__ S, cons
 // Right-hand side of each pressure row (imex_source_map_helpers.hpp:545-566),
 // with the Dirichlet parts of FIXED-side terms added first as the reference
 // does while it builds the row (:488-507).
-__global__ void imex_rhs_kernel(ImexGrid G, const double* __restrict__ S, const double* __restrict__ M,
+__global__ void __launch_bounds__(256) imex_rhs_kernel(ImexGrid G, const double* __restrict__ S, const double* __restrict__ M,
                                 const double* __restrict__ hx, const double* __restrict__ hy,
                                 double* __restrict__ b, GmCtl* ctl)
 {
@@ -581,7 +581,7 @@ __global__ void matvec_kernel(OpArgs<T> A, const T* __restrict__ x, T* __restric
 }
 
 // r = b - A x (FP64), with |r|_inf and |r|_2^2 reduced into ctl.
-__global__ void residual_kernel(const OpArgs<double>* __restrict__ Ap, const double* __restrict__ x,
+__global__ void __launch_bounds__(256) residual_kernel(const OpArgs<double>* __restrict__ Ap, const double* __restrict__ x,
                                 const double* __restrict__ b, double* __restrict__ r,
                                 double* partials, GmCtl* ctl)
 {
@@ -767,7 +767,7 @@ __global__ void __launch_bounds__(RB) norm_givens_kernel(const double* partials,
 
 // Start a cycle: v0 = r / beta, g = beta e1.
 template <typename T>
-__global__ void cycle_start_kernel(const double* __restrict__ r, T* __restrict__ v0, long long n,
+__global__ void __launch_bounds__(256) cycle_start_kernel(const double* __restrict__ r, T* __restrict__ v0, long long n,
                                    const GmCtl* ctl, double* g)
 {
   const double beta = ctl->beta;
@@ -793,7 +793,7 @@ __global__ void backsolve_kernel(const double* R, const double* g, double* y, in
 }
 
 template <typename T, typename X>
-__global__ void xupdate_kernel(const T* __restrict__ V, long long n, const double* __restrict__ y,
+__global__ void __launch_bounds__(256) xupdate_kernel(const T* __restrict__ V, long long n, const double* __restrict__ y,
                                X* __restrict__ x, double scale, const GmCtl* ctl)
 {
   const int kk = ctl->kk;
@@ -808,7 +808,7 @@ __global__ void xupdate_kernel(const T* __restrict__ V, long long n, const doubl
 // ------------------------------------------------------ refresh / closeout
 // m = m* - dt grad_c p (imex_source_map_helpers.hpp:249-268, with the
 // physical-boundary pressure rule).
-__global__ void momentum_refresh_kernel(ImexGrid G, const double* __restrict__ S,
+__global__ void __launch_bounds__(256) momentum_refresh_kernel(ImexGrid G, const double* __restrict__ S,
                                         const double* __restrict__ P, double* __restrict__ M, GmCtl* ctl)
 {
   const long long n = (long long)G.nx * G.ny;
@@ -828,7 +828,7 @@ __global__ void momentum_refresh_kernel(ImexGrid G, const double* __restrict__ S
 
 // E = E* - dt div(h m), the new state, and the coupled residual
 // p/(g-1) - E + K (imex_source_map_step.hpp:95-187).
-__global__ void closeout_kernel(ImexGrid G, const double* __restrict__ S, const double* __restrict__ M,
+__global__ void __launch_bounds__(256) closeout_kernel(ImexGrid G, const double* __restrict__ S, const double* __restrict__ M,
                                 const double* __restrict__ P, const double* __restrict__ fx,
                                 const double* __restrict__ fy, double* __restrict__ U, long long rs,
                                 long long cs, double* partials, GmCtl* ctl)
@@ -861,7 +861,7 @@ __global__ void closeout_kernel(ImexGrid G, const double* __restrict__ S, const 
 
 // compute_imex_dt reductions (diagnostic_metrics_timestep.hpp:262-326): the
 // eq79 material rate 2|u|/min_dx and the acoustic rate.
-__global__ void imex_rates_kernel(const double* __restrict__ U, long long rs, long long cs, GridParams g,
+__global__ void __launch_bounds__(256) imex_rates_kernel(const double* __restrict__ U, long long rs, long long cs, GridParams g,
                                   double* partials, GmCtl* ctl)
 {
   const long long n = (long long)g.nx * g.ny;

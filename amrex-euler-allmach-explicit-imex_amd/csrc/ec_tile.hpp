@@ -213,9 +213,11 @@ struct TileStep {
 
   // Advances the tile whose first cell is (i0, j0) by dt from uin to uout and
   // returns the tile's epilogue reduction (valid in thread 0).
+  // diag = false skips the epilogue (CFL rate, diagnostics, degenerate
+  // count): a replay with known dts needs only the state.
   __device__ static Red run(Smem& sm, const double* __restrict__ uin, double* __restrict__ uout,
                             long long rs, long long cs, const GridParams& g, int i0, int j0,
-                            double dt)
+                            double dt, bool diag = true)
   {
     double* lds = sm.lds;
     unsigned char* degx = sm.degx;
@@ -335,7 +337,9 @@ struct TileStep {
         const Prim L = face_state(li, lj, AX, 0.5);
         const Prim R = face_state(ri, rj, AX, -0.5);
         out = interface_flux(L, R, AX, gamma, RIEMANN, sensor);
-        if constexpr (RIEMANN != 0) dflag = face_degenerate(L, R, AX, gamma, RIEMANN) ? 1 : 0;
+        if constexpr (RIEMANN != 0) {
+          if (diag) dflag = face_degenerate(L, R, AX, gamma, RIEMANN) ? 1 : 0;
+        }
       }
     };
     Cons fxv[NPX], fyv[NPY];
@@ -393,10 +397,13 @@ struct TileStep {
           nu[c] = old - dt * ((fxr - fxl) * inv_dx + (fyt - fyb) * inv_dy);
           uout[c * cs + k] = nu[c];
         }
-        mine.deg += degx[xl] + degx[xl + 1] + degy[yb] + degy[yb + TX];
-        red_cell(mine, Cons{nu[0], nu[1], nu[2], nu[3]}, g);
+        if (diag) {
+          mine.deg += degx[xl] + degx[xl + 1] + degy[yb] + degy[yb + TX];
+          red_cell(mine, Cons{nu[0], nu[1], nu[2], nu[3]}, g);
+        }
       }
     }
+    if (!diag) return mine;
     return block_reduce<NT>(mine, sm.red_scratch);
   }
 };

@@ -117,3 +117,71 @@ def test_checkpointed_gradient_matches_fd(ecamd, gpu):
         fd = (Jp - Jm) / (2 * eps)
         ad = float((grad * d).sum())
         assert ad == pytest.approx(fd, rel=1e-6), (k, ad, fd)
+
+
+@pytest.mark.parametrize("riemann", ["hllc", "xie_am_hllc_p"])
+def test_checkpointed_sweep_equals_chained_steps(ecamd, gpu, riemann):
+    """The checkpointed sweep (recompute without diagnostics, each step's
+    fold sharing a launch with the next step's prep) gives bit for bit the
+    lambda_0 of chaining ec_adjoint_step over the recorded trajectory."""
+    import ctypes as C
+    from ecamd._lib import lib
+    from ecamd.adjoint import adjoint_step
+    nx, ny = 40, 24
+    U = torch.from_numpy(smooth_state(nx, ny, 3)).to(gpu)
+    grid = ecamd.Grid(nx, ny, 1.0 / nx, 1.0 / ny, 1.4, riemann, "minmod", 2,
+                      ("outflow", "outflow", "periodic", "periodic"))
+    s = ecamd.Solver(grid, gpu)
+    nsteps, K = 11, 4
+    n = nx * ny
+    chk = torch.empty(((nsteps + K - 1) // K) * 4 * n, dtype=torch.float64, device=gpu)
+    UN = torch.empty_like(U)
+    dts = np.zeros(nsteps)
+    done = C.c_int()
+    assert lib().ec_forward_checkpointed(s.ctx, C.c_void_p(U.data_ptr()), C.c_void_p(UN.data_ptr()),
+                                         C.byref(s.layout), nsteps, K, 0.4, 1e30,
+                                         C.c_void_p(chk.data_ptr()),
+                                         dts.ctypes.data_as(C.POINTER(C.c_double)), C.byref(done)) == 0
+    assert done.value == nsteps
+    lamN = torch.from_numpy(np.random.default_rng(4).standard_normal((4, ny, nx))).to(gpu)
+    lam0 = torch.empty_like(lamN)
+    assert lib().ec_adjoint_checkpointed(s.ctx, C.c_void_p(chk.data_ptr()), C.byref(s.layout), nsteps, K,
+                                         dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                         C.c_void_p(lamN.data_ptr()), C.c_void_p(lam0.data_ptr())) == 0
+    states = [U.clone()]
+    out = torch.empty_like(U)
+    for k in range(nsteps):
+        s.step(states[-1], out, float(dts[k]), diag=False)
+        states.append(out.clone())
+    assert torch.equal(states[-1], UN)
+    lam = lamN.clone()
+    for k in range(nsteps - 1, -1, -1):
+        lam = adjoint_step(s, states[k], lam, float(dts[k]))
+    assert torch.equal(lam, lam0)
+
+
+@pytest.mark.parametrize("riemann", ["rusanov", "hllc", "lowmach_hllc", "xie_am_hllc_p"])
+@pytest.mark.parametrize("bcs", [("periodic",) * 4, ("fixed", "outflow", "reflect", "outflow")])
+def test_device_adjoint_step_equals_host_passes(ecamd, gpu, riemann, bcs):
+    """The device passes (prep, faces, ratios, LDS-tiled cells, ring fold)
+    reproduce the host run of the same passes bit for bit; the host run is
+    pinned against oracle finite differences on the CPU."""
+    import ctypes as C
+    from ecamd._lib import lib
+    from ecamd.adjoint import adjoint_step
+    nx, ny = 70, 37   # several cell tiles, ragged edges
+    U = smooth_state(nx, ny, 8)
+    inflow = tuple(U[:, ny // 2, 0] * np.array([1.07, 0.9, 1.1, 1.05]))
+    fixed = (inflow, None, None, None) if bcs[0] == "fixed" else (None,) * 4
+    grid = ecamd.Grid(nx, ny, 1.0 / nx, 1.0 / ny, 1.4, riemann, "minmod", 2, bcs, fixed)
+    s = ecamd.Solver(grid, gpu)
+    Ud = torch.from_numpy(U).to(gpu)
+    dt = 0.3 / s.diagnostics(Ud).max_rate
+    w = np.random.default_rng(5).standard_normal(U.shape)
+    lam_dev = adjoint_step(s, Ud, torch.from_numpy(w).to(gpu), dt).cpu().numpy()
+    lam_host = np.zeros_like(U)
+    dp = C.POINTER(C.c_double)
+    Uc = np.ascontiguousarray(U)
+    assert lib().ec_adjoint_step_host(C.byref(grid.desc()), Uc.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                      lam_host.ctypes.data_as(dp), dt) == 0
+    assert np.array_equal(lam_dev, lam_host)
