@@ -115,6 +115,7 @@ extern "C" ec_status ec_destroy(ec_ctx* ctx)
   else (void)hipDeviceSynchronize();
   imex_release(ctx);
   adjoint_release(ctx);
+  for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
   if (ctx->ctrl) (void)hipFree(ctx->ctrl);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
   if (ctx->partials) (void)hipFree(ctx->partials);

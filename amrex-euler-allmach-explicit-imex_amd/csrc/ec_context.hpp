@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/ec_amd.h"
 
@@ -57,6 +58,11 @@ struct ec_ctx {
   // IMEX, adjoint and batch workspaces are attached lazily by their modules.
   void* imex = nullptr;
   void* adjoint = nullptr;
+  // Optional step-kernel timing: event pairs recorded around each explicit
+  // step kernel launch on the context's stream.
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  int tused = 0;
 };
 
 namespace ec {

@@ -178,8 +178,14 @@ ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, lon
     ctx->last_error = "partials buffer too small for the tile shape";
     return EC_ERR_INVALID_ARGUMENT;
   }
+  const bool timed = ctx->timing && ctx->tused + 2 <= (int)ctx->tev.size();
+  if (timed) EC_HIP_TRY(ctx, hipEventRecord(ctx->tev[ctx->tused], ctx->stream));
   step_fn(ctx->tile_id, ctx->grid.riemann, ctx->grid.order)(a, dim3(a.nblocks), dim3(s.nt),
                                                              ctx->stream);
+  if (timed) {
+    EC_HIP_TRY(ctx, hipEventRecord(ctx->tev[ctx->tused + 1], ctx->stream));
+    ctx->tused += 2;
+  }
   if (a.diag) {
     hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0,
                        ctx->stream, ctx->partials, a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
@@ -362,5 +368,37 @@ extern "C" ec_status ec_set_explicit_tile(ec_ctx* ctx, int tile_id)
   if (!ctx || tile_id < 0 || tile_id >= num_tile_shapes()) return EC_ERR_INVALID_ARGUMENT;
   if (step_partials_needed(ctx, tile_id) > ctx->max_partials) return EC_ERR_INVALID_ARGUMENT;
   ctx->tile_id = tile_id;
+  return EC_OK;
+}
+
+extern "C" ec_status ec_step_kernel_timing(ec_ctx* ctx, int enable, int max_launches)
+{
+  if (!ctx || max_launches < 0) return EC_ERR_INVALID_ARGUMENT;
+  EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  EC_HIP_TRY(ctx, ctx->stream ? hipStreamSynchronize(ctx->stream) : hipDeviceSynchronize());
+  for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
+  ctx->tev.clear();
+  ctx->tused = 0;
+  ctx->timing = enable != 0;
+  if (!ctx->timing) return EC_OK;
+  ctx->tev.resize(2 * (size_t)max_launches, nullptr);
+  for (hipEvent_t& e : ctx->tev) EC_HIP_TRY(ctx, hipEventCreate(&e));
+  return EC_OK;
+}
+
+extern "C" ec_status ec_step_kernel_time(ec_ctx* ctx, double* total_ms, int* launches)
+{
+  if (!ctx || !total_ms || !launches) return EC_ERR_INVALID_ARGUMENT;
+  EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  double sum = 0.0;
+  for (int k = 0; k + 1 < ctx->tused; k += 2) {
+    EC_HIP_TRY(ctx, hipEventSynchronize(ctx->tev[k + 1]));
+    float ms = 0.f;
+    EC_HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->tev[k], ctx->tev[k + 1]));
+    sum += ms;
+  }
+  *total_ms = sum;
+  *launches = ctx->tused / 2;
+  ctx->tused = 0;
   return EC_OK;
 }

@@ -103,6 +103,10 @@ def lib() -> C.CDLL:
     L.ec_explicit_tile_shape.restype = st
     L.ec_set_explicit_tile.argtypes = [_vp, C.c_int]
     L.ec_set_explicit_tile.restype = st
+    L.ec_step_kernel_timing.argtypes = [_vp, C.c_int, C.c_int]
+    L.ec_step_kernel_timing.restype = st
+    L.ec_step_kernel_time.argtypes = [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.ec_step_kernel_time.restype = st
     _declare_optional(L)
     _lib = L
     return L

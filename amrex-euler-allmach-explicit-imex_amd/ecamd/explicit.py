@@ -171,6 +171,21 @@ class ExplicitSolver:
               "ec_explicit_advance")
         return U_b if in_b.value else U_a
 
+    def arm_kernel_timing(self, launches: int) -> None:
+        """Bracket each of the next `launches` step kernels with HIP events."""
+        check(self._ctx, lib().ec_step_kernel_timing(self._ctx, 1, int(launches)),
+              "ec_step_kernel_timing")
+
+    def kernel_time(self) -> tuple[float, int]:
+        """(summed step-kernel milliseconds, launches) since arming."""
+        ms, n = C.c_double(), C.c_int()
+        check(self._ctx, lib().ec_step_kernel_time(self._ctx, C.byref(ms), C.byref(n)),
+              "ec_step_kernel_time")
+        return ms.value, n.value
+
+    def disarm_kernel_timing(self) -> None:
+        check(self._ctx, lib().ec_step_kernel_timing(self._ctx, 0, 0), "ec_step_kernel_timing")
+
     @staticmethod
     def status_name(run: RunState) -> str:
         return RUN_STATUS.get(run.status, str(run.status))

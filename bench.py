@@ -56,6 +56,26 @@ def cpu_baseline(n: int, mach: float, cfl: float, steps: int) -> dict:
                       "wall_time_sec"}
 
 
+# VALU wave instructions per launch of the HLLC-P O2 step at 2048^2 (16x15
+# tiles), from the committed rocprofv3 SQ counter summary.
+VALU_FILE = ROOT / "profiles" / "explicit_step_xie_hllcp_gresho2048_valu.json"
+
+
+def valu_bound(riemann: str, order: int, n: int, kernel_s: float):
+    """The FP64 VALU issue bound of the step kernel: counted VALU wave
+    instructions x 4 cycles (a wave64 instruction on a 16-lane SIMD) over the
+    1024 SIMDs at 2.4 GHz, against the measured kernel time."""
+    if not (VALU_FILE.exists() and n == 2048 and riemann == "xie_am_hllc_p" and order == 2):
+        return None
+    try:
+        d = json.loads(VALU_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    issue_s = d["valu_wave_insts_per_launch"] * 4 / (1024 * 2.4e9)
+    return {"valu_wave_insts_per_launch": d["valu_wave_insts_per_launch"],
+            "issue_bound_us": issue_s * 1e6, "frac_of_issue_bound": issue_s / kernel_s}
+
+
 def copy_calibration(dev, n: int, steps: int) -> int:
     """Device-to-device copies of one n x n state (4 planes of FP64): a plain
     copy kernel of known size, so that FETCH_SIZE / WRITE_SIZE per dispatch can
@@ -232,9 +252,11 @@ def bench_adjoint(dev, nx: int = 640, ny: int = 160, nsteps: int = 768, K: int =
             "fd_rel_err_median": sorted(errs)[len(errs) // 2]}
 
 
-def bench_ensemble(dev, riemann: str, stop_time: float = 0.1, cfl: float = 0.8) -> dict:
+def bench_ensemble(dev, riemann: str, stop_time: float = 0.1, cfl: float = 0.8,
+                   mode: str = "auto") -> dict:
     """The 64 seeded Riemann strips (tests/golden/ensemble_64.json), each run
-    to stop_time: autotuned (mode, tile) table and the best run's throughput."""
+    to stop_time: autotuned (mode, tile) table and the best run's throughput
+    (mode "graph" or "persistent": that mode with tile shape 0, no autotune)."""
     import numpy as np
     from ecamd.batch import Ensemble, riemann_strip_members, tile_shapes
     table = json.loads((ROOT / "tests" / "golden" / "ensemble_64.json").read_text())
@@ -252,7 +274,11 @@ def bench_ensemble(dev, riemann: str, stop_time: float = 0.1, cfl: float = 0.8) 
             for c in range(4):
                 U[c][:, sel] = cons[c]
         ens.set_state(k, U)
-    tuned = ens.autotune()
+    if mode == "auto":
+        tuned = ens.autotune()
+    else:
+        tuned = []
+        ens.configure(mode, 0)
     res, used = ens.run()
     cell_updates = sum(r.step * m["nx"] * 4 for r, m in zip(res, members))
     shapes = tile_shapes()
@@ -286,6 +312,8 @@ def main() -> int:
                     help="skip the IMEX, adjoint and ensemble measurements")
     ap.add_argument("--only", choices=("imex", "adjoint", "ensemble"),
                     help="print only that measurement")
+    ap.add_argument("--ensemble-mode", choices=("auto", "graph", "persistent"), default="auto",
+                    help="ensemble rounds: autotuned, or one mode with tile shape 0")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -303,7 +331,8 @@ def main() -> int:
     n = args.n
     if args.only:
         fn = {"imex": lambda: bench_imex(dev), "adjoint": lambda: bench_adjoint(dev),
-              "ensemble": lambda: {r: bench_ensemble(dev, r) for r in ("xie_am_hllc_p", "hllc")}}
+              "ensemble": lambda: {r: bench_ensemble(dev, r, mode=args.ensemble_mode)
+                                   for r in ("xie_am_hllc_p", "hllc")}}
         print(json.dumps({args.only: fn[args.only]()}))
         return 0
     if args.mode == "copy":
@@ -341,7 +370,18 @@ def main() -> int:
     per_step = gpu_s / args.steps
     cells = n * n
     value = cells * args.steps / gpu_s
-    achieved = BYTES_PER_CELL_UPDATE * cells / per_step / 1e9
+    # The step kernel's own average duration, from HIP events recorded around
+    # each launch on the solver's stream (a second pass of the same loop; the
+    # per-step finalize kernel is outside the brackets).
+    if out is B:
+        A, B = B, A
+    solver.arm_kernel_timing(args.steps)
+    out = solver.advance(A, B, args.steps, run)
+    torch.cuda.synchronize()
+    k_ms, k_n = solver.kernel_time()
+    solver.disarm_kernel_timing()
+    kernel_s = k_ms / 1e3 / max(k_n, 1)
+    achieved = BYTES_PER_CELL_UPDATE * cells / kernel_s / 1e9
     traffic = None
     if TRAFFIC_FILE.exists() and n == 2048 and args.riemann == "xie_am_hllc_p" and args.order == 2:
         try:
@@ -367,7 +407,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"explicit_step_kernel ({args.riemann}, O{args.order})",
-                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE},
+                     "kernel_us_avg": kernel_s * 1e6, "kernel_launches_timed": k_n,
+                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE,
+                     "bytes_per_launch": BYTES_PER_CELL_UPDATE * cells,
+                     "fp64_valu": valu_bound(args.riemann, args.order, n, kernel_s)},
         "wall_s": wall,
         "sim_time_advanced": run.time,
     }

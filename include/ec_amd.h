@@ -151,6 +151,15 @@ double ec_euler_dt(double max_rate, double cfl, double time, double stop_time);
 ec_status ec_explicit_advance(ec_ctx* ctx, double* U_a, double* U_b, const ec_layout* layout,
                               int nsteps, ec_run_state* io, int* result_in_b);
 
+/* Step-kernel timing (measurement hook; no reference counterpart).
+ * ec_step_kernel_timing(ctx, 1, N) arms up to N timed launches: each explicit
+ * step kernel launched afterwards is bracketed by HIP events on the
+ * context's stream (the finalize kernel is outside the bracket).
+ * ec_step_kernel_time returns the summed duration and the launch count since
+ * arming (or the last read) and rearms; enable = 0 disarms. */
+ec_status ec_step_kernel_timing(ec_ctx* ctx, int enable, int max_launches);
+ec_status ec_step_kernel_time(ec_ctx* ctx, double* total_ms, int* launches);
+
 /* Kernel tile shape of the explicit step (autotuner hook).  tile_id indexes
  * the compiled shapes; ec_explicit_tile_count() says how many exist and
  * ec_explicit_tile_shape() describes one. */
