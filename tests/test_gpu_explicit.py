@@ -189,3 +189,51 @@ def test_run_status_paths(ecor, ecamd, gpu):
     r, p, nf = ecor.diagnostics(g, bad)
     assert (d.rho_min, d.pressure_min, d.nonfinite_count) == (r, p, nf)
     assert d.max_rate == ecor.max_rate(g, bad)
+
+
+def test_headline_workload_full_size_bitwise(ecor, ecamd, gpu):
+    """The headline workload at its full size: Gresho 2048^2, M=0.01, HLLC-P
+    O2, CFL 0.4, two steps of the device loop against the oracle (about 16 s
+    of serial CPU), bit for bit, including the dt sequence."""
+    n = 2048
+    g = ecor.make_grid(n, n, 1.0 / n, 1.0 / n, riemann=3)
+    U = ecor.init_gresho(n, n, mach=0.01)
+    ref, res, dts = ecor.run_explicit(g, U, 0.4, 1.0, 2)
+    s = solver_for(ecamd, g)
+    got, run = ecamd.run_explicit(s, to_dev(U, gpu), 0.4, 1.0, 2)
+    assert run.step == 2 and res.steps == 2
+    assert run.time == res.time and run.degenerate_total == res.degenerate_total
+    assert np.array_equal(got.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("riemann", ["rusanov", "hllc", "xie_am_hllc_p"])
+def test_quarter_turn_symmetry_full_size(ecamd, gpu, riemann):
+    """Size-independent property at 2048^2: rotating a non-symmetric periodic
+    state by 90 degrees ((x, y) -> (y, -x), (mx, my) -> (my, -mx)) and stepping equals
+    stepping and then rotating, after 20 device-loop steps.  Bit for bit for
+    Rusanov; HLLC and HLLC-P see one axis's Riemann problems mirrored, and
+    their star-state arithmetic is not rounding-symmetric under mirroring, so
+    they agree to rounding level (the oracle shows the same at small n)."""
+    n = 2048
+    rng = np.random.default_rng(11)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * X) * np.cos(4 * np.pi * Y) + 0.01 * rng.standard_normal((n, n))
+    u = 0.3 * np.sin(2 * np.pi * Y)
+    v = 0.2 * np.cos(2 * np.pi * X)
+    p = 1.0 + 0.1 * np.cos(2 * np.pi * (X + Y))
+    U = np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+
+    def rot(V):  # quarter turn (x, y) -> (y, -x) of the [j][i] planes, momenta rotated
+        r = np.stack([np.rot90(V[c]) for c in range(4)])
+        return np.stack([r[0], r[2], -r[1], r[3]])
+
+    grid = ecamd.Grid(n, n, 1.0 / n, 1.0 / n, 1.4, riemann, "minmod", 2)
+    s = ecamd.Solver(grid, gpu)
+    a, ra = ecamd.run_explicit(s, to_dev(U, gpu), 0.4, 1.0, 20)
+    b, rb = ecamd.run_explicit(s, to_dev(np.ascontiguousarray(rot(U)), gpu), 0.4, 1.0, 20)
+    assert ra.step == rb.step == 20 and ra.time == rb.time
+    if riemann == "rusanov":
+        assert np.array_equal(rot(a.cpu().numpy()), b.cpu().numpy())
+    else:
+        assert np.all(rel_l1(rot(a.cpu().numpy()), b.cpu().numpy()) <= 1e-12)
