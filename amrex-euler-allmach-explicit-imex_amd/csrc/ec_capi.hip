@@ -22,6 +22,9 @@ GridParams make_grid_params(const ec_grid_desc& d)
   // numerics/explicit_update.hpp:69-70
   g.inv_dx = 1.0 / d.dx;
   g.inv_dy = 1.0 / d.dy;
+  int e;
+  g.dx_pow2 = std::frexp(d.dx, &e) == 0.5 ? 1 : 0;
+  g.dy_pow2 = std::frexp(d.dy, &e) == 0.5 ? 1 : 0;
   g.gamma = d.gamma;
   g.riemann = d.riemann;
   g.limiter = d.limiter;

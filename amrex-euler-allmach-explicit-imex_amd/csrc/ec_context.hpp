@@ -36,6 +36,7 @@ struct GridParams {
   int nx, ny;
   double dx, dy, inv_dx, inv_dy, gamma;
   int riemann, limiter, order;
+  int dx_pow2, dy_pow2;  // dx (dy) is a power of two: x / dx == x * inv_dx exactly
   int bc[4];
   double fixed[4][4];
 };

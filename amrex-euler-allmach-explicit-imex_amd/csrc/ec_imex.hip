@@ -548,7 +548,13 @@ template <typename T, typename Get>
 __device__ __forceinline__ T apply_row_g(const OpArgs<T>& A, const Get& get, int i, int j)
 {
   const ImexGrid& G = A.G;
-  auto at = [&](int ii, int jj) -> T { return fixed_side(G, ii, jj) >= 0 ? T(0) : get(pcol(G, ii, jj)); };
+  // Rows at least 2 cells from every side read their stencil directly (the
+  // boundary mapping is the identity there).
+  const bool inner = i >= 2 && j >= 2 && i < G.nx - 2 && j < G.ny - 2;
+  auto at = [&](int ii, int jj) -> T {
+    if (inner) return get((long long)jj * G.nx + ii);
+    return fixed_side(G, ii, jj) >= 0 ? T(0) : get(pcol(G, ii, jj));
+  };
   const long long k = (long long)j * G.nx + i;
   const T pc = get(k);
   const T he = A.hx[(long long)j * (G.nx + 1) + i + 1], hw = A.hx[(long long)j * (G.nx + 1) + i];

@@ -74,7 +74,10 @@ __device__ __forceinline__ void red_cell(Red& r, const Cons& u, const GridParams
   r.pmin = smin(r.pmin, bad ? -INFINITY : q.p);
   r.bad += bad ? 1 : 0;
   const double c = sound_speed(q, g.gamma);
-  const double rate = (fabs(q.u) + c) / g.dx + (fabs(q.v) + c) / g.dy;
+  // division by a power-of-two cell size is the multiplication by its exact
+  // reciprocal, bit for bit
+  const double ax = fabs(q.u) + c, ay = fabs(q.v) + c;
+  const double rate = (g.dx_pow2 ? ax * g.inv_dx : ax / g.dx) + (g.dy_pow2 ? ay * g.inv_dy : ay / g.dy);
   if (!isnan(rate)) r.rate = smax(r.rate, rate);
 }
 
@@ -226,14 +229,43 @@ struct TileStep {
     const double gamma = g.gamma;
     Red mine = red_identity();
 
-    // A. primitive variables of the tile and its 2-cell halo.
-    for (int t = tid; t < NR; t += NT) {
-      const int li = t % RX, lj = t / RX;
-      const Prim q = to_prim(load_cell(uin, rs, cs, g, i0 + li - 2, j0 + lj - 2), gamma);
-      lds[OFF_Q + t] = q.r;
-      lds[OFF_Q + NR + t] = q.u;
-      lds[OFF_Q + 2 * NR + t] = q.v;
-      lds[OFF_Q + 3 * NR + t] = q.p;
+    // Each thread's own cell of stage E (one cell per thread when the tile
+    // has no more cells than threads), loaded now so that its latency hides
+    // behind stages A-D.
+    constexpr bool ONE_CELL = TX * TY <= NT;
+    double own[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (ONE_CELL) {
+      const int ci = tid % TX, cj = tid / TX;
+      if (tid < TX * TY && ci < g.nx - i0 && cj < g.ny - j0) {
+        const long long k = (long long)(j0 + cj) * rs + (i0 + ci);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) own[c] = uin[c * cs + k];
+      }
+    }
+
+    // A. primitive variables of the tile and its 2-cell halo.  A tile whose
+    // halo lies inside the domain reads it directly (the ghost rule is the
+    // identity there); boundary tiles go through the rule.
+    const bool interior = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= g.nx && j0 + TY + 2 <= g.ny;
+    if (interior) {
+      for (int t = tid; t < NR; t += NT) {
+        const int li = t % RX, lj = t / RX;
+        const long long k = (long long)(j0 + lj - 2) * rs + (i0 + li - 2);
+        const Prim q = to_prim(Cons{uin[k], uin[cs + k], uin[2 * cs + k], uin[3 * cs + k]}, gamma);
+        lds[OFF_Q + t] = q.r;
+        lds[OFF_Q + NR + t] = q.u;
+        lds[OFF_Q + 2 * NR + t] = q.v;
+        lds[OFF_Q + 3 * NR + t] = q.p;
+      }
+    } else {
+      for (int t = tid; t < NR; t += NT) {
+        const int li = t % RX, lj = t / RX;
+        const Prim q = to_prim(load_cell(uin, rs, cs, g, i0 + li - 2, j0 + lj - 2), gamma);
+        lds[OFF_Q + t] = q.r;
+        lds[OFF_Q + NR + t] = q.u;
+        lds[OFF_Q + 2 * NR + t] = q.v;
+        lds[OFF_Q + 3 * NR + t] = q.p;
+      }
     }
     __syncthreads();
 
@@ -393,7 +425,7 @@ struct TileStep {
         for (int c = 0; c < 4; ++c) {
           const double fxl = lds[OFF_FX + c * NXF + xl], fxr = lds[OFF_FX + c * NXF + xl + 1];
           const double fyb = lds[OFF_FY + c * NYF + yb], fyt = lds[OFF_FY + c * NYF + yb + TX];
-          const double old = uin[c * cs + k];
+          const double old = ONE_CELL ? own[c] : uin[c * cs + k];
           nu[c] = old - dt * ((fxr - fxl) * inv_dx + (fyt - fyb) * inv_dy);
           uout[c * cs + k] = nu[c];
         }
