@@ -102,7 +102,10 @@ struct Cfg {
               imex_pressure_stabilization = "off", bdltv20_paper_t1_s2 = "off", final_csv,
               imex_gmres_precision = "fp64", snapshot_times = "0,0.06,0.12,0.18,0.24,0.30",
               snapshot_dir;
-  int imex_gmres_restart = 30;
+  // The reference's solve is unrestarted GMRES (imex/imex_pressure_helpers.hpp:140);
+  // the longest device restart comes closest (GMRES(30) stagnates on some
+  // transmissive-boundary systems that the unrestarted solve converges).
+  int imex_gmres_restart = 64;
   int problem_kind = EC_PROBLEM_ADVECTION_BLOB, riemann_kind = 1, limiter_kind = 0;
 };
 

@@ -31,6 +31,39 @@ BYTES_PER_CELL_UPDATE = 64     # read U^n (4 x 8 B) + write U^{n+1} (4 x 8 B)
 TRAFFIC_FILE = ROOT / "profiles" / "explicit_step_xie_hllcp_gresho2048_traffic.json"
 
 
+def roofline_sweep(dev, riemann: str, order: int, mach: float, cfl: float) -> dict:
+    """The roofline sweep of the Gresho workload at 256^2 and 1024^2 (2048^2 is
+    the main line): throughput and the step kernel's event-timed duration."""
+    import torch
+    import ecamd
+    from ecamd import problems
+    out = {}
+    for n, steps in ((256, 400), (1024, 200)):
+        grid = ecamd.Grid(n, n, 1.0 / n, 1.0 / n, 1.4, riemann, "minmod", order)
+        s = ecamd.ExplicitSolver(grid, dev, stream=torch.cuda.current_stream(dev))
+        A = torch.from_numpy(problems.gresho(n, mach)).to(dev)
+        B = torch.empty_like(A)
+        run = s.new_run(cfl, 1.0, 2**30)
+        if s.advance(A, B, 20, run) is B:
+            A, B = B, A
+        torch.cuda.synchronize()
+        s.arm_kernel_timing(steps)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        s.advance(A, B, steps, run)
+        b.record()
+        torch.cuda.synchronize()
+        k_ms, k_n = s.kernel_time()
+        s.disarm_kernel_timing()
+        per = a.elapsed_time(b) / 1e3 / steps
+        kern = k_ms / 1e3 / max(k_n, 1)
+        out[f"{n}x{n}"] = {"ms_per_step": per * 1e3, "cell_updates_per_s": n * n / per,
+                           "kernel_us_avg": kern * 1e6,
+                           "hbm_frac": BYTES_PER_CELL_UPDATE * n * n / kern / 1e9 / HBM_PEAK_GBS}
+        s.close()
+    return out
+
+
 def cpu_baseline(n: int, mach: float, cfl: float, steps: int) -> dict:
     """The oracle (serial CPU restatement) on the same workload, first `steps` steps."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -415,6 +448,7 @@ def main() -> int:
         "sim_time_advanced": run.time,
     }
     if not args.no_extras:
+        line["roofline_sweep"] = roofline_sweep(dev, args.riemann, args.order, args.mach, args.cfl)
         line["imex"] = bench_imex(dev)
         line["adjoint"] = bench_adjoint(dev)
         line["ensemble"] = {r: bench_ensemble(dev, r) for r in ("xie_am_hllc_p", "hllc")}

@@ -148,3 +148,43 @@ def test_autotune_restores_states(gpu):
         assert np.array_equal(ens.get_state(k), ens2.get_state(k))
     ens.close()
     ens2.close()
+
+
+@pytest.mark.parametrize("mode", ["graph", "persistent"])
+@pytest.mark.parametrize("tile", [0, 2])
+def test_mixed_2d_and_strip_members(ecor, gpu, mode, tile):
+    """Members of different kinds in one batch: Gresho boxes of several sizes
+    (periodic), a strip and a shock-bubble-rule box (fixed / outflow /
+    reflect / outflow), HLLC-P O2, 15 capped steps; each member bit-identical
+    to the oracle's driver loop."""
+    from ecamd.batch import Ensemble, Member
+    specs = [(64, 64, (0, 0, 0, 0)), (48, 40, (0, 0, 0, 0)), (100, 30, (0, 0, 0, 0)),
+             (200, 4, (1, 1, 0, 0)), (36, 28, (3, 1, 2, 1))]
+    members, states, grids = [], [], []
+    for k, (nx, ny, bc) in enumerate(specs):
+        if bc[0] == 3:
+            U = ecor.init_gresho(nx, ny, mach=0.3)
+            fixed = (tuple(U[:, ny // 2, 0] * np.array([1.05, 1.2, 1.0, 1.1])), None, None, None)
+        elif ny == 4:
+            U = ecor.init_toro(nx, ny, [1, 0.2, 0, 1], [0.3, -0.1, 0, 0.4])
+            fixed = None
+        else:
+            U = ecor.init_gresho(nx, ny, mach=0.05)
+            fixed = None
+        g = ecor.make_grid(nx, ny, 1.0 / nx, 1.0 / ny, riemann=3, bc=bc, fixed=fixed)
+        members.append(Member(nx=nx, ny=ny, dx=1.0 / nx, dy=1.0 / ny, bc=bc,
+                              fixed=fixed if fixed else (None,) * 4, cfl=0.4, stop_time=1.0,
+                              max_step=15))
+        states.append(np.ascontiguousarray(U))
+        grids.append(g)
+    ens = Ensemble(members, gamma=1.4, riemann="xie_am_hllc_p", device=gpu)
+    for k, U in enumerate(states):
+        ens.set_state(k, U)
+    ens.configure(mode, tile, 4)
+    res, _ = ens.run()
+    for k, (U, g) in enumerate(zip(states, grids)):
+        ref, r, _ = ecor.run_explicit(g, U, 0.4, 1.0, 15)
+        assert res[k].step == r.steps == 15, k
+        assert res[k].time == r.time and res[k].degenerate_total == r.degenerate_total
+        assert np.array_equal(ens.get_state(k), ref), f"member {k}"
+    ens.close()

@@ -100,3 +100,57 @@ def test_periodic_conservation_and_failure_status(ecor, ecamd, gpu):
     d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-14, max_iter=3))
     assert ecamd.IMEX_STATUS[d.status] == "linear_solve_not_converged"
     assert np.array_equal(Ud.cpu().numpy(), U)
+
+
+def smooth_low_mach(nx, ny, mach=0.01):
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    rho = 1.0 + 0.1 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    u = 0.5 * np.sin(np.pi * Y) * np.cos(np.pi * X)
+    v = 0.3 * np.cos(np.pi * Y) * np.sin(2 * np.pi * X)
+    p = 1.0 / (1.4 * mach * mach) + np.cos(2 * np.pi * X)
+    return np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("bcs", [("outflow", "outflow", "periodic", "periodic"),
+                                 ("fixed", "outflow", "reflect", "outflow")])
+def test_non_periodic_pressure_solves_the_reference_rows(ecor, ecamd, gpu, precision, bcs):
+    """Non-periodic boundaries (transmissive strips; the shock-bubble rule:
+    fixed inflow, outflow, reflecting bottom): with one Picard iteration the
+    pressure system depends on U^n alone, so the device pressure must satisfy
+    the oracle's assembled rows (the reference's triplet assembly) to the
+    solver tolerance.  With the shock-bubble rule this system is so badly
+    conditioned that solutions at equal residual differ by ~1e-4 in
+    momentum, so the residual, not the state, is the parity measure here.
+    The reference's solve is unrestarted; GMRES(30) stagnates near 5e-9 on
+    the transmissive strip's system, so the test runs GMRES(64), the longest
+    restart."""
+    nx, ny = 40, 24
+    U = smooth_low_mach(nx, ny)
+    code = {"periodic": 0, "outflow": 1, "reflect": 2, "fixed": 3}
+    fixed = (tuple(U[:, ny // 2, 0] * np.array([1.02, 1.0, 1.0, 1.0])), None, None, None) \
+        if bcs[0] == "fixed" else None
+    g = ecor.make_grid(nx, ny, 1 / nx, 1 / ny, bc=tuple(code[b] for b in bcs), fixed=fixed)
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    S = ecor.imex_star(g, U, dt)
+    p0 = 0.4 * (U[3] - 0.5 * (U[1] ** 2 + U[2] ** 2) / U[0])
+    rp, cols, vals, rhs = ecor.imex_assemble(g, S, p0, np.ascontiguousarray(S[1:3]), dt)
+    s = ecamd.Solver(ecamd.Grid(nx, ny, 1 / nx, 1 / ny, 1.4, "hllc", "minmod", 2, bcs,
+                                fixed if fixed else (None,) * 4), gpu)
+    Ud = torch.from_numpy(U).to(gpu)
+    p = torch.empty(nx * ny, dtype=torch.float64, device=gpu)
+    d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-10, max_iter=3000, precision=precision,
+                                        picard_iterations=1, restart=64), p)
+    assert ecamd.IMEX_STATUS[d.status] in ("converged_absolute", "converged_relative"), d.status
+    pd = p.cpu().numpy()
+    Ax = np.zeros(nx * ny)
+    for r in range(nx * ny):
+        Ax[r] = np.dot(vals[rp[r]:rp[r + 1]], pd[cols[rp[r]:rp[r + 1]]])
+    rel = np.abs(Ax - rhs).max() / max(np.abs(rhs).max(), 1.0)
+    print(precision, bcs, d.iterations, d.relative_residual_linf, rel)
+    assert rel <= 2e-10, rel
+    ref, pref, r = ecor.imex_step(g, U, dt, picard=1, tol=1e-10, max_iter=3000)
+    assert r.status in (0, 1)
+    assert np.array_equal(Ud.cpu().numpy()[0], ref[0])   # the star density, no solver error
