@@ -237,3 +237,59 @@ def test_quarter_turn_symmetry_full_size(ecamd, gpu, riemann):
         assert np.array_equal(rot(a.cpu().numpy()), b.cpu().numpy())
     else:
         assert np.all(rel_l1(rot(a.cpu().numpy()), b.cpu().numpy()) <= 1e-12)
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (1, 5), (2, 3), (3, 2), (5, 1), (7, 3)])
+@pytest.mark.parametrize("bcs", [(0, 0, 0, 0), (1, 1, 2, 2), (3, 1, 2, 1)])
+def test_tiny_and_ragged_grids(ecor, ecamd, gpu, nx, ny, bcs):
+    """Grids smaller than the 2-cell halo (the ghost rule wraps, clamps or
+    mirrors more than once) and than one tile, every Riemann solver, 10 steps
+    of the device loop, bit for bit against the oracle."""
+    rng = np.random.default_rng(nx * 10 + ny)
+    q = np.stack([1 + 0.2 * rng.random((ny, nx)), 0.3 * rng.standard_normal((ny, nx)),
+                  0.3 * rng.standard_normal((ny, nx)), 1 + 0.2 * rng.random((ny, nx))])
+    U = np.ascontiguousarray(np.stack([q[0], q[0] * q[1], q[0] * q[2],
+                                       q[3] / 0.4 + 0.5 * q[0] * (q[1] ** 2 + q[2] ** 2)]))
+    fixed = (tuple(ecor.to_conserved([1.1, 0.2, 0.0, 1.2])), None, None, None) if bcs[0] == 3 else None
+    for riemann in range(4):
+        g = ecor.make_grid(nx, ny, 1.0 / max(nx, 1), 1.0 / max(ny, 1), riemann=riemann, bc=bcs,
+                           fixed=fixed)
+        ref, res, _ = ecor.run_explicit(g, U, 0.4, 1.0, 10)
+        s = solver_for(ecamd, g)
+        got, run = ecamd.run_explicit(s, to_dev(U, gpu), 0.4, 1.0, 10)
+        assert run.step == res.steps and run.status == res.status, (riemann, run.status, res.status)
+        assert np.array_equal(got.cpu().numpy(), ref), riemann
+
+
+def test_largest_size_properties(ecamd, gpu):
+    """8192^2 (2.1 GB per state, 280k tiles), Rusanov O2, 5 device-loop
+    steps of a non-symmetric periodic state: the quarter-turn commutes with
+    the step bit for bit, and mass, momentum and energy are conserved to
+    rounding."""
+    n = 8192
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * X) * np.cos(4 * np.pi * Y)
+    u = 0.3 * np.sin(2 * np.pi * Y)
+    v = 0.2 * np.cos(2 * np.pi * X)
+    p = 1.0 + 0.1 * np.cos(2 * np.pi * (X + Y))
+    U = np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+    del X, Y, rho, u, v, p
+
+    def rot(V):
+        r = np.stack([np.rot90(V[c]) for c in range(4)])
+        return np.ascontiguousarray(np.stack([r[0], r[2], -r[1], r[3]]))
+
+    grid = ecamd.Grid(n, n, 1.0 / n, 1.0 / n, 1.4, "rusanov", "minmod", 2)
+    s = ecamd.Solver(grid, gpu)
+    a, ra = ecamd.run_explicit(s, to_dev(U, gpu), 0.4, 1.0, 5)
+    A = a.cpu().numpy()
+    del a
+    b, rb = ecamd.run_explicit(s, to_dev(rot(U), gpu), 0.4, 1.0, 5)
+    assert ra.step == rb.step == 5 and ra.time == rb.time and ra.status == 0
+    assert np.array_equal(rot(A), b.cpu().numpy())
+    del b
+    s0 = U.reshape(4, -1).sum(axis=1)
+    s1 = A.reshape(4, -1).sum(axis=1)
+    scale = np.abs(U).reshape(4, -1).sum(axis=1)
+    assert np.all(np.abs(s1 - s0) <= 1e-12 * scale), (s1 - s0) / scale
