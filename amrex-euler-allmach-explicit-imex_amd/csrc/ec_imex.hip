@@ -29,6 +29,7 @@
 #include <cstddef>
 
 #include "ec_context.hpp"
+#include "ec_wave.hpp"
 #include "ec_ghost.hpp"
 #include "ec_state.hpp"
 
@@ -239,10 +240,8 @@ __device__ void reduce_sum_store(DSum<K> s, double* partials, int kactive)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (k < kactive) {
-      double v = s.v[k];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0) sh[wave][k] = v;
+      const double v = wave_sum_lane63(s.v[k]);
+      if (lane == 63) sh[wave][k] = v;
     }
   }
   __syncthreads();
@@ -261,17 +260,15 @@ __device__ __forceinline__ void rowdots_store(const F& prod, int k, double* part
 {
   __shared__ double sh[RB / 64][MAXK];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // Eight products (and their basis loads) in flight at a time.
+  // Eight products (and their basis loads) in flight at a time; each summed
+  // over the wavefront with DPP moves (total in lane 63).
   for (int q0 = 0; q0 < k; q0 += 8) {
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) v[u] = q0 + u < k ? prod(q0 + u) : 0.0;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] += __shfl_xor(v[u], off);
-    }
-    if (lane == 0) {
+    for (int u = 0; u < 8; ++u) v[u] = wave_sum_lane63(v[u]);
+    if (lane == 63) {
 #pragma unroll
       for (int u = 0; u < 8; ++u)
         if (q0 + u < k) sh[wave][q0 + u] = v[u];
@@ -621,7 +618,14 @@ __global__ void __launch_bounds__(RB) update_dot_kernel(const T* __restrict__ V,
   double wd = 0.0;
   if (t < n) {
     double wt = (double)w[t];
-    for (int q = 0; q < k; ++q) wt -= h[q] * (double)V[q * n + t];
+    for (int q0 = 0; q0 < k; q0 += 8) {   // eight basis loads in flight
+      double vb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) vb[u] = q0 + u < k ? (double)V[(q0 + u) * n + t] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < k) wt -= h[q0 + u] * vb[u];
+    }
     const T wn = (T)wt;
     w[t] = wn;
     wd = (double)wn;
@@ -644,7 +648,14 @@ __global__ void __launch_bounds__(RB) update_norm_kernel(const T* __restrict__ V
   s.v[0] = 0.0;
   for (long long t = (long long)blockIdx.x * RB + threadIdx.x; t < n; t += (long long)gridDim.x * RB) {
     double wt = (double)w[t];
-    for (int q = 0; q < k; ++q) wt -= h[q] * (double)V[q * n + t];
+    for (int q0 = 0; q0 < k; q0 += 8) {   // eight basis loads in flight
+      double vb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) vb[u] = q0 + u < k ? (double)V[(q0 + u) * n + t] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < k) wt -= h[q0 + u] * vb[u];
+    }
     const T wn = (T)wt;
     w[t] = wn;
     s.v[0] += (double)wn * (double)wn;
@@ -692,9 +703,8 @@ __global__ void __launch_bounds__(RB) arnoldi_kernel(const OpArgs<T>* __restrict
 __device__ __forceinline__ double block_sum_fixed(double v, double* sh)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  if (lane == 0) sh[wave] = v;
+  v = wave_sum_lane63(v);
+  if (lane == 63) sh[wave] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     v = sh[0];
@@ -806,7 +816,14 @@ __global__ void __launch_bounds__(256) xupdate_kernel(const T* __restrict__ V, l
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
     double s = 0.0;
-    for (int q = 0; q < kk; ++q) s += y[q] * (double)V[q * n + t];
+    for (int q0 = 0; q0 < kk; q0 += 8) {
+      double vb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) vb[u] = q0 + u < kk ? (double)V[(q0 + u) * n + t] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < kk) s += y[q0 + u] * vb[u];
+    }
     x[t] = (X)((double)x[t] + scale * s);
   }
 }

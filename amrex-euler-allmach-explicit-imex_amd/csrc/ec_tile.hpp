@@ -22,6 +22,7 @@
 #include "ec_context.hpp"
 #include "ec_ghost.hpp"
 #include "ec_state.hpp"
+#include "ec_wave.hpp"
 
 namespace ec {
 
@@ -81,18 +82,32 @@ __device__ __forceinline__ void red_cell(Red& r, const Cons& u, const GridParams
   if (!isnan(rate)) r.rate = smax(r.rate, rate);
 }
 
+// One step of the wavefront tree: combine with the value DPP control CTRL
+// brings in (rows outside ROW_MASK bring the identity).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void red_dpp_step(Red& r)
+{
+  Red o;
+  o.rate = dpp_f64_id<CTRL, ROW_MASK>(r.rate, -INFINITY);
+  o.rmin = dpp_f64_id<CTRL, ROW_MASK>(r.rmin, INFINITY);
+  o.pmin = dpp_f64_id<CTRL, ROW_MASK>(r.pmin, INFINITY);
+  o.bad = dpp_i32<CTRL, ROW_MASK>(r.bad);
+  o.deg = dpp_i32<CTRL, ROW_MASK>(r.deg);
+  red_combine(r, o);
+}
+
+// Wavefront reduction (result in lane 63): quad swaps, half-row and row
+// mirrors, then the row broadcasts of lanes 15 and 31, all DPP moves.  Every
+// combine is order independent (max, min without NaN, integer sums), so the
+// result equals any other reduction order's bit for bit.
 __device__ __forceinline__ Red wave_reduce(Red r)
 {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    Red o;
-    o.rate = __shfl_xor(r.rate, off);
-    o.rmin = __shfl_xor(r.rmin, off);
-    o.pmin = __shfl_xor(r.pmin, off);
-    o.bad = __shfl_xor(r.bad, off);
-    o.deg = __shfl_xor(r.deg, off);
-    red_combine(r, o);
-  }
+  red_dpp_step<0xB1, 0xF>(r);
+  red_dpp_step<0x4E, 0xF>(r);
+  red_dpp_step<0x141, 0xF>(r);
+  red_dpp_step<0x140, 0xF>(r);
+  red_dpp_step<0x142, 0xA>(r);
+  red_dpp_step<0x143, 0xC>(r);
   return r;
 }
 
@@ -102,9 +117,10 @@ __device__ __forceinline__ Red block_reduce(Red r, Red* scratch)
 {
   r = wave_reduce(r);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) scratch[wave] = r;
+  if (lane == 63) scratch[wave] = r;
   __syncthreads();
   if (threadIdx.x == 0) {
+    r = scratch[0];
     for (int w = 1; w < NT / 64; ++w) red_combine(r, scratch[w]);
   }
   return r;

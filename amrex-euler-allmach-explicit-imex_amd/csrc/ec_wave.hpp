@@ -1,0 +1,55 @@
+// ec_wave.hpp -- wavefront sums of doubles with DPP lane moves (VALU only,
+// no LDS traffic), in a fixed tree: two quad swaps, the half-row and row
+// mirrors, then the broadcasts of lanes 15 and 31 into the following rows.
+// The total ends in lane 63.  Every lane of the wavefront must be active.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ec {
+
+// The value of lane sel(i) for each lane i under DPP control CTRL; rows
+// outside ROW_MASK read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)(unsigned)(b & 0xffffffffLL);
+  const int hi = (int)(b >> 32);
+  const int lo2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi2 << 32) | (long long)(unsigned)lo2);
+}
+
+// The same move for a double whose rows outside ROW_MASK read `id`, and for
+// a 32-bit integer.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64_id(double v, double id)
+{
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(id);
+  const int lo2 = __builtin_amdgcn_update_dpp((int)(unsigned)(o & 0xffffffffLL),
+                                              (int)(unsigned)(b & 0xffffffffLL), CTRL, ROW_MASK,
+                                              0xf, false);
+  const int hi2 = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROW_MASK, 0xf,
+                                              false);
+  return __longlong_as_double(((long long)hi2 << 32) | (long long)(unsigned)lo2);
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ __forceinline__ double wave_sum_lane63(double v)
+{
+  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]: pairs
+  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]: quads
+  v += dpp_f64<0x141, 0xF>(v);  // row_half_mirror: 8-lane halves
+  v += dpp_f64<0x140, 0xF>(v);  // row_mirror: 16-lane rows
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+}  // namespace ec
