@@ -47,8 +47,16 @@ struct MemberDev {
   int pad;
 };
 
+// Everything a round needs about one tile, in one record (one dependent load).
+struct TileInfo {
+  MemberDev md;
+  int m;   // member
+  int lt;  // member-local tile index
+};
+
 struct BatchArgs {
   const MemberDev* members;
+  const TileInfo* tiles;   // per global tile
   const int* tile_member;  // global tile -> member
   double* ua;
   double* ub;
@@ -88,17 +96,18 @@ __device__ Red tile_state_reduce(Red* scratch, const double* __restrict__ u, lon
 // of it follows).
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
 __device__ bool member_round(typename TileStep<TX, TY, NT, RIEMANN, ORDER>::Smem& sm,
-                             RoundShared& rsh, const BatchArgs& A, int m, int ta, int tb, int p)
+                             RoundShared& rsh, const BatchArgs& A, const MemberDev& md, int m, int ta,
+                             int tb, int p)
 {
   using T = TileStep<TX, TY, NT, RIEMANN, ORDER>;
-  const MemberDev& md = A.members[m];
   const int tid = threadIdx.x;
   const DevCtrl* cin = A.ctrl + (long long)p * A.nm + m;
-  const int pending = cin->counter;
 
-  // 1. fold the previous round's partials of every tile of the member
+  // 1. fold the previous round's partials of every tile of the member.  The
+  // partials are read whether or not a fold is pending (the buffer always
+  // holds finite data), so their loads overlap the control block's.
   Red r = red_identity();
-  if (pending != 0) {
+  {
     const Partial* prev = A.part + (long long)(p ^ 1) * A.ntiles + md.tile0;
     for (int q = tid; q < md.ntiles; q += NT) {
       const Partial v = prev[q];
@@ -108,6 +117,7 @@ __device__ bool member_round(typename TileStep<TX, TY, NT, RIEMANN, ORDER>::Smem
   r = block_reduce<NT>(r, sm.red_scratch);
   if (tid == 0) {
     DevCtrl c = *cin;
+    const int pending = c.counter;
     if (pending != 0) ctrl_fold(c, r, pending, nullptr);
     double dt;
     bool active, dt_bad;
@@ -168,10 +178,8 @@ __global__ void __launch_bounds__(NT) batch_round_kernel(BatchArgs A, int p)
   using T = TileStep<TX, TY, NT, RIEMANN, ORDER>;
   __shared__ typename T::Smem sm;
   __shared__ RoundShared rsh;
-  const int t = blockIdx.x;
-  const int m = A.tile_member[t];
-  const int lt = t - A.members[m].tile0;
-  (void)member_round<TX, TY, NT, RIEMANN, ORDER>(sm, rsh, A, m, lt, lt + 1, p);
+  const TileInfo& ti = A.tiles[blockIdx.x];
+  (void)member_round<TX, TY, NT, RIEMANN, ORDER>(sm, rsh, A, ti.md, ti.m, ti.lt, ti.lt + 1, p);
 }
 
 // Waits until the member's arrival counter reaches target.  Bounded: after
@@ -228,8 +236,8 @@ __global__ void __launch_bounds__(NT) batch_persistent_kernel(BatchArgs A, const
       const int tb = min(range.y, md.tile0 + md.ntiles);
       const int slot = m - first_member;
       if (!done[slot]) {
-        const bool stepped =
-            member_round<TX, TY, NT, RIEMANN, ORDER>(sm, rsh, A, m, t - md.tile0, tb - md.tile0, p);
+        const bool stepped = member_round<TX, TY, NT, RIEMANN, ORDER>(sm, rsh, A, md, m, t - md.tile0,
+                                                                      tb - md.tile0, p);
         if (stepped) {
           any = true;
           // Steps taken so far by the member, counting this one.
@@ -275,7 +283,10 @@ __global__ void batch_final_ctrl_kernel(const DevCtrl* ctrl, int nm, DevCtrl* ou
 struct BatchShape {
   int tx, ty, nt;
 };
-static const BatchShape kBatchShapes[] = {{60, 4, 256}, {124, 4, 512}, {16, 15, 256}};
+// 48x4: both face sets of a 4-row strip tile in one pass of 256 threads
+// (196 x-faces, 240 y-faces); 60x4 needs a second y pass but has fewer halo
+// columns per cell; 16x15 suits 2D members.
+static const BatchShape kBatchShapes[] = {{60, 4, 256}, {124, 4, 512}, {16, 15, 256}, {48, 4, 256}};
 constexpr int kNumBatchShapes = sizeof(kBatchShapes) / sizeof(kBatchShapes[0]);
 
 struct BatchFns {
@@ -313,6 +324,7 @@ static BatchFns batch_fns(int shape, int riemann, int order)
   switch (shape) {
     case 1: return fns_for_shape<124, 4, 512>(riemann, order);
     case 2: return fns_for_shape<16, 15, 256>(riemann, order);
+    case 3: return fns_for_shape<48, 4, 256>(riemann, order);
     default: return fns_for_shape<60, 4, 256>(riemann, order);
   }
 }
@@ -344,6 +356,7 @@ struct ec_batch {
   unsigned* arrive = nullptr;
   MemberDev* members = nullptr;
   int* tile_member = nullptr;
+  TileInfo* tiles = nullptr;
   int2* ranges = nullptr;
   int ranges_cap = 0;
   int mode = EC_BATCH_AUTO;
@@ -428,11 +441,22 @@ ec_status configure_tables(ec_batch* b, int shape)
   }
   if (b->tile_member) (void)hipFree(b->tile_member);
   b->tile_member = nullptr;
+  if (b->tiles) (void)hipFree(b->tiles);
+  b->tiles = nullptr;
   BT_TRY(b, hipMalloc(&b->tile_member, sizeof(int) * (size_t)nt));
+  BT_TRY(b, hipMalloc(&b->tiles, sizeof(TileInfo) * (size_t)nt));
+  std::vector<TileInfo> ti(nt);
+  for (int t = 0; t < nt; ++t) {
+    ti[t].md = md[tm[t]];
+    ti[t].m = tm[t];
+    ti[t].lt = t - md[tm[t]].tile0;
+  }
   BT_TRY(b, hipMemcpyAsync(b->members, md.data(), sizeof(MemberDev) * (size_t)b->nm,
                            hipMemcpyHostToDevice, b->stream));
   BT_TRY(b, hipMemcpyAsync(b->tile_member, tm.data(), sizeof(int) * (size_t)nt,
                            hipMemcpyHostToDevice, b->stream));
+  BT_TRY(b, hipMemcpyAsync(b->tiles, ti.data(), sizeof(TileInfo) * (size_t)nt, hipMemcpyHostToDevice,
+                           b->stream));
   BT_TRY(b, hipStreamSynchronize(b->stream));
   b->ntiles = nt;
   b->cfg_shape = shape;
@@ -448,6 +472,7 @@ BatchArgs make_args(ec_batch* b)
 {
   BatchArgs a;
   a.members = b->members;
+  a.tiles = b->tiles;
   a.tile_member = b->tile_member;
   a.ua = b->ua;
   a.ub = b->ub;
@@ -716,6 +741,7 @@ extern "C" ec_status ec_batch_create(const ec_member_desc* members, int n_member
   }
   if (hipMemsetAsync(b->ua, 0, sb, b->stream) != hipSuccess ||
       hipMemsetAsync(b->ub, 0, sb, b->stream) != hipSuccess ||
+      hipMemsetAsync(b->part, 0, sizeof(Partial) * 2 * (size_t)cap, b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess) {
     return bad(EC_ERR_HIP);
   }
@@ -740,6 +766,7 @@ extern "C" ec_status ec_batch_destroy(ec_batch* b)
   if (b->arrive) (void)hipFree(b->arrive);
   if (b->members) (void)hipFree(b->members);
   if (b->tile_member) (void)hipFree(b->tile_member);
+  if (b->tiles) (void)hipFree(b->tiles);
   if (b->ranges) (void)hipFree(b->ranges);
   if (b->ev_user) (void)hipEventDestroy(b->ev_user);
   if (b->ev_a) (void)hipEventDestroy(b->ev_a);

@@ -75,7 +75,7 @@ def oracle_capped(ecor):
 
 @pytest.mark.parametrize("riemann", list(RIEMANN_ID))
 @pytest.mark.parametrize("mode", ["graph", "persistent"])
-@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
 def test_capped_members_match_oracle(gpu, oracle_capped, riemann, mode, tile):
     print(f"ensemble capped {riemann} {mode} tile {tile}", flush=True)
     ens = build(gpu, riemann, 0.1, 10)
@@ -131,10 +131,11 @@ def test_failing_member_stops_alone(gpu):
 
 
 def test_autotune_restores_states(gpu):
+    from ecamd.batch import tile_shapes
     ens = build(gpu, "xie_am_hllc_p", 0.01, 100000)
     before = [ens.get_state(k) for k in range(len(table()))]
     timings = ens.autotune()
-    assert len(timings) == 6 and all(t.ms > 0 for t in timings)
+    assert len(timings) == 2 * len(tile_shapes()) and all(t.ms > 0 for t in timings)
     for k in range(len(table())):
         assert np.array_equal(ens.get_state(k), before[k])
     res, used = ens.run()
