@@ -72,11 +72,12 @@ struct ImexWork {
   double* w = nullptr;
   float* w32 = nullptr;
   double* R = nullptr;      // (m+1) x m Hessenberg / triangular factor
-  double* g = nullptr;      // m+1
+  double* g = nullptr;      // 2(m+1): running entries, then the rotated ones
   double* cs = nullptr;
   double* sn = nullptr;
   double* y = nullptr;
   double* partials = nullptr;  // reduction partials
+  double* npart = nullptr;     // |w|^2 partials of the Arnoldi norm pass
   int nparts = 0;
   GmCtl* ctl = nullptr;
   GmCtl* ctl_host = nullptr;
@@ -99,6 +100,7 @@ void imex_release(ec_ctx* ctx)
   void* ptrs[] = {w->star, w->p,   w->x,    w->mom, w->hx,       w->hy,      w->fx, w->fy,
                   w->hx32, w->hy32, w->b,   w->r,   w->V,        w->V32,     w->w,  w->w32,
                   w->R,    w->g,    w->cs,   w->sn,  w->y,        w->partials, w->ctl,
+                  w->npart,
                   w->dA64, w->dA32};
   for (void* p : ptrs) {
     if (p) (void)hipFree(p);
@@ -663,42 +665,6 @@ __global__ void __launch_bounds__(RB) update_norm_kernel(const T* __restrict__ V
   reduce_sum_store<1>(s, partials, 1);
 }
 
-// Arnoldi iteration j, first half: v_j = w_prev / |w_prev| (for j > 0; v_0
-// is already in V) is formed on the fly and stored, y = A v_j, and the first
-// Gram-Schmidt pass's dots V_q . y (q <= j) are accumulated in the same sweep,
-// so the new column and the basis are each read once.
-template <typename T>
-__global__ void __launch_bounds__(RB) arnoldi_kernel(const OpArgs<T>* __restrict__ Ap, T* __restrict__ V,
-                                                     int j, const T* __restrict__ wprev,
-                                                     T* __restrict__ y, double* partials, GmCtl* ctl)
-{
-  if (ctl->done) return;
-  const OpArgs<T> A = *Ap;
-  const long long n = (long long)A.G.nx * A.G.ny;
-  const double s = ctl->inv_norm;
-  T* vj = V + (long long)j * n;
-  const long long t = (long long)blockIdx.x * RB + threadIdx.x;
-  double yd = 0.0, vd = 0.0;
-  if (t < n) {
-    const int i = (int)(t % A.G.nx), jj = (int)(t / A.G.nx);
-    T yt, vt;
-    if (j == 0) {
-      yt = apply_row(A, vj, i, jj);
-      vt = vj[t];
-    } else {
-      const auto get = [wprev, s](long long k) -> T { return (T)((double)wprev[k] * s); };
-      yt = apply_row_g(A, get, i, jj);
-      vt = get(t);
-      vj[t] = vt;
-    }
-    y[t] = yt;
-    yd = (double)yt;
-    vd = (double)vt;
-  }
-  rowdots_store([&](int q) { return t >= n ? 0.0 : (q < j ? (double)V[q * n + t] : vd) * yd; }, j + 1,
-                partials);
-}
-
 // Fixed-order workgroup sum (valid in thread 0).
 __device__ __forceinline__ double block_sum_fixed(double v, double* sh)
 {
@@ -727,19 +693,68 @@ __global__ void __launch_bounds__(RB) finalize_dots_kernel(const double* partial
   if (threadIdx.x == 0) out[q] = v;
 }
 
-// Arnoldi iteration j, closing half: |w|^2 from the partials, then column j
-// of the Hessenberg matrix, its Givens rotations, the residual estimate and
-// the early-exit decision (the column and rotations staged in LDS).
-__global__ void __launch_bounds__(RB) norm_givens_kernel(const double* partials, int nparts, double* R,
-                                                         double* g, double* cs, double* sn, int j,
-                                                         int m, GmCtl* ctl, int two_pass)
+// Column j's Givens step (thread 0, on the LDS-staged column and rotations):
+// the column's earlier rotations, the new rotation, the residual estimate
+// and the early-exit decision, from |w|^2 = v.  With `write` the results are
+// committed (R column, rotation, g, control fields); every caller computes
+// the same values, so only one workgroup needs to write.
+struct GivensOut {
+  double inv_norm;
+  int done;
+};
+
+__device__ GivensOut givens_step(double v, double* col, const double* c, const double* sv, double* R,
+                                 double* g, double* cs, double* sn, int j, int m, GmCtl* ctl,
+                                 bool write)
 {
-  __shared__ double sh[RB / 64];
-  __shared__ double col[MAXK + 1], c[MAXK], sv[MAXK];
-  if (ctl->done) return;
+  // breakdown tolerance 100 eps max(beta, 1) (imex_pressure_helpers.hpp:167-169)
+  const double breakdown_tol = 2.220446049250313e-16 * 100.0 * smax(ctl->beta, 1.0);
+  const double hn = sqrt(v);
+  col[j + 1] = hn;
+  for (int i = 0; i < j; ++i) {
+    const double a = col[i], b = col[i + 1];
+    col[i] = c[i] * a + sv[i] * b;
+    col[i + 1] = -sv[i] * a + c[i] * b;
+  }
+  const double a = col[j], b = col[j + 1];
+  const double rr = hypot(a, b);
+  const double cj = rr == 0.0 ? 1.0 : a / rr;
+  const double sj = rr == 0.0 ? 0.0 : b / rr;
+  col[j] = rr;
+  col[j + 1] = 0.0;
+  const double gj = g[j];
+  const double gnext = -sj * gj;
+  const double est = fabs(gnext);
+  const int bad = (!isfinite(hn) || !isfinite(est) || !isfinite(rr)) ? 1 : ctl->bad;
+  const int brk = hn <= breakdown_tol ? 1 : ctl->breakdown;
+  const int done = (est <= ctl->thr || j + 1 >= ctl->mlim || bad || brk) ? 1 : 0;
+  if (write) {
+    cs[j] = cj;
+    sn[j] = sj;
+    g[j + 1] = gnext;   // the running last entry, read by column j + 1
+    g[m + 1 + j] = cj * gj;  // the rotated entry, kept apart so g[j] stays readable
+    ctl->nrm2 = v;
+    ctl->est = est;
+    ctl->kk = j + 1;
+    ctl->inv_norm = 1.0 / hn;
+    ctl->bad = bad;
+    ctl->breakdown = brk;
+    if (done) ctl->done = 1;
+    double* rc = R + (long long)j * (m + 1);  // column-major (m+1) x m
+    for (int i = 0; i <= j + 1; ++i) rc[i] = col[i];
+  }
+  return GivensOut{1.0 / hn, done};
+}
+
+// Stages column j (first- plus, with two passes, second-pass dots) and the
+// earlier rotations in LDS and sums |w|^2 from the norm partials (fixed
+// order); result valid in thread 0 after the barrier inside.
+__device__ double givens_stage(const double* npart, int nparts, const double* cs, const double* sn,
+                               int j, const GmCtl* ctl, int two_pass, double* sh, double* col,
+                               double* c, double* sv)
+{
   double v = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += RB) v += partials[b];
-  v = block_sum_fixed(v, sh);
+  for (int b = threadIdx.x; b < nparts; b += RB) v += npart[b];
   for (int i = threadIdx.x; i <= j; i += RB) {
     col[i] = two_pass ? ctl->dots1[i] + ctl->dots2[i] : ctl->dots1[i];
     if (i < j) {
@@ -747,38 +762,83 @@ __global__ void __launch_bounds__(RB) norm_givens_kernel(const double* partials,
       sv[i] = sn[i];
     }
   }
+  return block_sum_fixed(v, sh);  // its barrier also publishes col, c, sv
+}
+
+// The Givens step of the cycle's last column (every other column's step runs
+// inside the next Arnoldi kernel).
+__global__ void __launch_bounds__(RB) norm_givens_kernel(const double* npart, int nparts, double* R,
+                                                         double* g, double* cs, double* sn, int j,
+                                                         int m, GmCtl* ctl, int two_pass)
+{
+  __shared__ double sh[RB / 64];
+  __shared__ double col[MAXK + 1], c[MAXK], sv[MAXK];
+  if (ctl->done) return;
+  const double v = givens_stage(npart, nparts, cs, sn, j, ctl, two_pass, sh, col, c, sv);
+  if (threadIdx.x == 0) (void)givens_step(v, col, c, sv, R, g, cs, sn, j, m, ctl, true);
+}
+
+// Arnoldi iteration j, first half.  For j > 0 every workgroup first closes
+// column j - 1 (its Givens step, from the norm partials of the previous
+// iteration; workgroup 0 commits it) and exits if that ends the cycle.  Then
+// v_j = w_prev / |w_prev| (v_0 is already in V) is formed on the fly and
+// stored, y = A v_j, and the first Gram-Schmidt pass's dots V_q . y (q <= j)
+// are accumulated in the same sweep, so the new column and the basis are each
+// read once.
+template <typename T>
+__global__ void __launch_bounds__(RB) arnoldi_kernel(const OpArgs<T>* __restrict__ Ap, T* __restrict__ V,
+                                                     int j, const T* __restrict__ wprev,
+                                                     T* __restrict__ y, double* partials, GmCtl* ctl,
+                                                     const double* npart, int nparts, double* R,
+                                                     double* g, double* cs, double* sn, int m,
+                                                     int two_pass)
+{
+  __shared__ double gsh[RB / 64];
+  __shared__ double col[MAXK + 1], gc[MAXK], gsv[MAXK];
+  __shared__ double s_inv;
+  __shared__ int s_done;
+  // workgroup 0 may set ctl->done during this kernel: read it once per
+  // workgroup so every wave takes the same exit
+  if (threadIdx.x == 0) s_done = ctl->done;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    // breakdown tolerance 100 eps max(beta, 1) (imex_pressure_helpers.hpp:167-169)
-    const double breakdown_tol = 2.220446049250313e-16 * 100.0 * smax(ctl->beta, 1.0);
-    ctl->nrm2 = v;
-    const double hn = sqrt(v);
-    col[j + 1] = hn;
-    for (int i = 0; i < j; ++i) {
-      const double a = col[i], b = col[i + 1];
-      col[i] = c[i] * a + sv[i] * b;
-      col[i + 1] = -sv[i] * a + c[i] * b;
+  if (s_done) return;
+  double s = 1.0;
+  if (j > 0) {
+    // Column j-1's Givens step, redundantly in every workgroup (workgroup 0
+    // commits it), then v_j = w / |w| below.
+    const double v = givens_stage(npart, nparts, cs, sn, j - 1, ctl, two_pass, gsh, col, gc, gsv);
+    if (threadIdx.x == 0) {
+      const GivensOut o = givens_step(v, col, gc, gsv, R, g, cs, sn, j - 1, m, ctl, blockIdx.x == 0);
+      s_inv = o.inv_norm;
+      s_done = o.done;
     }
-    const double a = col[j], b = col[j + 1];
-    const double rr = hypot(a, b);
-    const double cj = rr == 0.0 ? 1.0 : a / rr;
-    const double sj = rr == 0.0 ? 0.0 : b / rr;
-    cs[j] = cj;
-    sn[j] = sj;
-    col[j] = rr;
-    col[j + 1] = 0.0;
-    g[j + 1] = -sj * g[j];
-    g[j] = cj * g[j];
-    ctl->est = fabs(g[j + 1]);
-    ctl->kk = j + 1;
-    ctl->inv_norm = 1.0 / hn;
-    if (!isfinite(hn) || !isfinite(ctl->est) || !isfinite(rr)) ctl->bad = 1;
-    if (hn <= breakdown_tol) ctl->breakdown = 1;
-    if (ctl->est <= ctl->thr || j + 1 >= ctl->mlim || ctl->bad || ctl->breakdown) ctl->done = 1;
+    __syncthreads();
+    if (s_done) return;
+    s = s_inv;
   }
-  __syncthreads();
-  double* rc = R + (long long)j * (m + 1);  // column-major (m+1) x m
-  for (int i = threadIdx.x; i <= j + 1; i += RB) rc[i] = col[i];
+  const OpArgs<T> A = *Ap;
+  const long long n = (long long)A.G.nx * A.G.ny;
+  T* vj = V + (long long)j * n;
+  const long long t = (long long)blockIdx.x * RB + threadIdx.x;
+  double yd = 0.0, vd = 0.0;
+  if (t < n) {
+    const int i = (int)(t % A.G.nx), jj = (int)(t / A.G.nx);
+    T yt, vt;
+    if (j == 0) {
+      yt = apply_row(A, vj, i, jj);
+      vt = vj[t];
+    } else {
+      const auto get = [wprev, s](long long k) -> T { return (T)((double)wprev[k] * s); };
+      yt = apply_row_g(A, get, i, jj);
+      vt = get(t);
+      vj[t] = vt;
+    }
+    y[t] = yt;
+    yd = (double)yt;
+    vd = (double)vt;
+  }
+  rowdots_store([&](int q) { return t >= n ? 0.0 : (q < j ? (double)V[q * n + t] : vd) * yd; }, j + 1,
+                partials);
 }
 
 // Start a cycle: v0 = r / beta, g = beta e1.
@@ -801,7 +861,7 @@ __global__ void backsolve_kernel(const double* R, const double* g, double* y, in
   if (threadIdx.x != 0) return;
   const int kk = ctl->kk;
   for (int i = kk - 1; i >= 0; --i) {
-    double s = g[i];
+    double s = g[m + 1 + i];  // rotated right-hand side
     for (int q = i + 1; q < kk; ++q) s -= R[(long long)q * (m + 1) + i] * y[q];
     y[i] = s / R[(long long)i * (m + 1) + i];
     if (!isfinite(y[i])) ctl->bad = 1;
@@ -966,11 +1026,12 @@ static ec_status imex_workspace(ec_ctx* ctx, int m)
   A((void**)&w->dA64, sizeof(OpArgs<double>));
   A((void**)&w->dA32, sizeof(OpArgs<float>));
   A((void**)&w->R, (size_t)(m + 1) * m * sizeof(double));
-  A((void**)&w->g, (size_t)(m + 1) * sizeof(double));
+  A((void**)&w->g, (size_t)2 * (m + 1) * sizeof(double));
   A((void**)&w->cs, (size_t)m * sizeof(double));
   A((void**)&w->sn, (size_t)m * sizeof(double));
   A((void**)&w->y, (size_t)m * sizeof(double));
   A((void**)&w->partials, (size_t)w->nparts * MAXK * sizeof(double));
+  A((void**)&w->npart, (size_t)w->nparts * sizeof(double));
   A((void**)&w->ctl, sizeof(GmCtl));
   ok = ok && hipHostMalloc((void**)&w->ctl_host, sizeof(GmCtl)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&w->hA64, sizeof(OpArgs<double>)) == hipSuccess;
@@ -1028,7 +1089,7 @@ static void enqueue_cycle(ImexWork* w, hipStream_t st, const OpArgs<T>* dA, T* V
     T* wj = (j & 1) ? wv + n : wv;
     const T* wp = (j & 1) ? wv : wv + n;
     hipLaunchKernelGGL((arnoldi_kernel<T>), dim3(nb), dim3(RB), 0, st, dA, V, j, wp, wj, w->partials,
-                       w->ctl);
+                       w->ctl, w->npart, nb, w->R, w->g, w->cs, w->sn, m, two_pass);
     hipLaunchKernelGGL(finalize_dots_kernel, dim3(j + 1), dim3(RB), 0, st, w->partials, nb, MAXK,
                        w->ctl->dots1, w->ctl);
     if (two_pass) {
@@ -1038,10 +1099,11 @@ static void enqueue_cycle(ImexWork* w, hipStream_t st, const OpArgs<T>* dA, T* V
                          w->ctl->dots2, w->ctl);
     }
     hipLaunchKernelGGL((update_norm_kernel<T>), dim3(nb), dim3(RB), 0, st, V, n, j + 1, wj,
-                       w->partials, w->ctl, two_pass ? 2 : 1);
-    hipLaunchKernelGGL(norm_givens_kernel, dim3(1), dim3(RB), 0, st, w->partials, nb, w->R, w->g,
-                       w->cs, w->sn, j, m, w->ctl, two_pass);
+                       w->npart, w->ctl, two_pass ? 2 : 1);
   }
+  // the last column's Givens step (the others ran inside the next Arnoldi kernel)
+  hipLaunchKernelGGL(norm_givens_kernel, dim3(1), dim3(RB), 0, st, w->npart, nb, w->R, w->g, w->cs,
+                     w->sn, m - 1, m, w->ctl, two_pass);
   hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, st, w->R, w->g, w->y, m, w->ctl);
   hipLaunchKernelGGL((xupdate_kernel<T, double>), dim3(nb), dim3(RB), 0, st, V, n, w->y, w->x, 1.0,
                      w->ctl);
