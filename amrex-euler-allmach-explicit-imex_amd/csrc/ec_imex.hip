@@ -32,6 +32,7 @@
 #include "ec_wave.hpp"
 #include "ec_ghost.hpp"
 #include "ec_state.hpp"
+#include "ec_tile.hpp"
 
 namespace ec {
 
@@ -41,9 +42,11 @@ constexpr int MAXK = 64;       // largest GMRES restart length
 // ------------------------------------------------------------ device control
 struct GmCtl {
   int done, kk, breakdown, bad;
-  int mlim, pad0;              // iterations allowed in this cycle
+  int mlim;                    // iterations allowed in this cycle
+  int inner_its;               // flexible outer solver: FP32 iterations of this cycle
   double thr, est, beta, inv_norm;
   double rinf, r2;             // true residual norms
+  double inner_thr;            // flexible outer solver: inner relative tolerance
   double dots1[MAXK], dots2[MAXK], nrm2;
   unsigned int counter;
   int floor_count, nonfinite, guard_count, mom_bad;
@@ -79,6 +82,22 @@ struct ImexWork {
   double* partials = nullptr;  // reduction partials
   double* npart = nullptr;     // |w|^2 partials of the Arnoldi norm pass
   int nparts = 0;
+  // flexible outer solver of the mixed form (FP64), allocated on first use
+  double* Vo = nullptr;      // (MO+1) n outer basis
+  double* Zo = nullptr;      // MO n preconditioned directions
+  double* Ro = nullptr;      // (MO+1) x MO
+  double* go = nullptr;      // 2(MO+1)
+  double* cso = nullptr;
+  double* sno = nullptr;
+  double* yo = nullptr;
+  GmCtl* ctlo = nullptr;
+  GmCtl* ctlo_host = nullptr;
+  std::vector<hipGraphExec_t> exec_outer;  // MO step graphs, then the closing graph
+  // one-reduction cycle (FP32 inner solver)
+  double* opart = nullptr;   // tile partials, otiles x KP
+  double* hraw = nullptr;    // unrotated Hessenberg columns, (m+1) x m
+  double* inv_nu = nullptr;  // 1 / |u_q|, m+1
+  int otiles = 0;
   GmCtl* ctl = nullptr;
   GmCtl* ctl_host = nullptr;
   // Restart cycles run as hipGraphs (one per precision) on a private stream;
@@ -90,7 +109,6 @@ struct ImexWork {
   hipStream_t gs = nullptr;
   hipEvent_t ev = nullptr;
   hipGraphExec_t exec64 = nullptr;
-  hipGraphExec_t exec32 = nullptr;
 };
 
 void imex_release(ec_ctx* ctx)
@@ -100,16 +118,20 @@ void imex_release(ec_ctx* ctx)
   void* ptrs[] = {w->star, w->p,   w->x,    w->mom, w->hx,       w->hy,      w->fx, w->fy,
                   w->hx32, w->hy32, w->b,   w->r,   w->V,        w->V32,     w->w,  w->w32,
                   w->R,    w->g,    w->cs,   w->sn,  w->y,        w->partials, w->ctl,
-                  w->npart,
+                  w->npart, w->opart,  w->hraw,  w->inv_nu,
+                  w->Vo,   w->Zo,   w->Ro,   w->go,  w->cso,      w->sno,     w->yo, w->ctlo,
                   w->dA64, w->dA32};
   for (void* p : ptrs) {
     if (p) (void)hipFree(p);
   }
   if (w->ctl_host) (void)hipHostFree(w->ctl_host);
+  if (w->ctlo_host) (void)hipHostFree(w->ctlo_host);
+  for (hipGraphExec_t e : w->exec_outer) {
+    if (e) (void)hipGraphExecDestroy(e);
+  }
   if (w->hA64) (void)hipHostFree(w->hA64);
   if (w->hA32) (void)hipHostFree(w->hA32);
   if (w->exec64) (void)hipGraphExecDestroy(w->exec64);
-  if (w->exec32) (void)hipGraphExecDestroy(w->exec32);
   if (w->ev) (void)hipEventDestroy(w->ev);
   if (w->gs) (void)hipStreamDestroy(w->gs);
   delete w;
@@ -778,6 +800,380 @@ __global__ void __launch_bounds__(RB) norm_givens_kernel(const double* npart, in
   if (threadIdx.x == 0) (void)givens_step(v, col, c, sv, R, g, cs, sn, j, m, ctl, true);
 }
 
+// ------------------------------------------- one-reduction GMRES cycle (FP32)
+// Classical Gram-Schmidt with lagged normalisation: one global reduction per
+// Arnoldi step (the one-synchronisation CGS of Swirydowicz, Langou,
+// Ananthan, Yang and Thomas, "Low synchronization Gram-Schmidt and GMRES
+// algorithms", Numer. Linear Algebra Appl. 28 (2021) e2343).  The basis is
+// kept unnormalised, u_q = nu_q v_q with nu_q = |u_q| (nu_0 = 1: u_0 = r /
+// beta), next to z = A u of the newest direction.  Kernel k:
+//   0. (k > 0) sums the previous kernel's tile partials in a fixed order, in
+//      every workgroup: nu_{k-1} = |u_{k-1}|, h_{q,k-1} = (u_q . z_{k-1}) /
+//      (nu_q nu_{k-1}), and column k-2's Givens step with its subdiagonal
+//      entry nu_{k-1} (workgroup 0 commits H, the scales and the rotation);
+//      every workgroup exits on the same early-exit decision;
+//   1. forms u_k = z_{k-1} / nu_{k-1} - sum_q h_{q,k-1} u_q / nu_q on its
+//      tile and the operator's cross-shaped 2-cell halo, in LDS (FP64
+//      accumulation, stored in T), and stores the tile's cells;
+//   2. applies the operator from LDS, z_k = A u_k;
+//   3. reduces u_q . z_k (q <= k) and u_k . u_k per tile.
+// The mathematics is that of CGS1 (the FP32 inner solver's
+// orthogonalisation), in one kernel per Arnoldi step instead of three.
+constexpr int OTX = 32, OTY = 32, ONT = OTX * OTY;  // one cell per thread
+constexpr int OHX = OTX + 4, OHY = OTY + 4, ONR = OHX * OHY;
+constexpr int KP = MAXK + 2;  // partial entries per tile: MAXK + 1 dots and one norm
+
+static inline int onesync_tiles(int nx, int ny)
+{
+  return ((nx + OTX - 1) / OTX) * ((ny + OTY - 1) / OTY);
+}
+
+// tot[e] = sum over tiles of partials[b*KP + e] for e in [e0, e1), in a
+// fixed order (lanes over tiles, then the wavefront): every workgroup gets
+// the same bits.
+__device__ __forceinline__ void sum_tile_entries(const double* partials, int nparts, int e0, int e1,
+                                                 double* tot)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int e = e0 + wave; e < e1; e += ONT / 64) {
+    double v = 0.0;
+    for (int b = lane; b < nparts; b += 64) v += partials[(long long)b * KP + e];
+    v = wave_sum_lane63(v);
+    if (lane == 63) tot[e] = v;
+  }
+}
+
+// Tile-local (li, lj) of halo slot h < 256 of the 32 x 32 tile: the two
+// rows above, the two below, the two columns left and the two right.
+__device__ __forceinline__ void halo_slot(int h, int& li, int& lj)
+{
+  const int seg = h >> 6, r = h & 63;
+  if (seg < 2) {
+    li = 2 + (r & (OTX - 1));
+    lj = (seg == 0 ? 0 : OTY + 2) + (r >> 5);
+  } else {
+    li = (seg == 2 ? 0 : OTX + 2) + (r & 1);
+    lj = 2 + (r >> 1);
+  }
+}
+static_assert(OTX == 32 && OTY == 32, "halo_slot enumerates the 256 halo cells of a 32 x 32 tile");
+
+// u_k at column c: z_{k-1} / nu_{k-1} - sum_q c_q u_q (FP64 accumulation),
+// with the first QB basis values already loaded (uv) and the rest read here;
+// for k = 0, z is u_0 itself.
+template <typename T, int QB>
+__device__ __forceinline__ T form_value(const T (&uv)[QB], T z, const T* __restrict__ U, long long n,
+                                        long long c, const double* coef, double cz, int k)
+{
+  if (k == 0) return z;
+  double sum = (double)z * cz;
+#pragma unroll
+  for (int q = 0; q < QB; ++q)
+    if (q < k) sum -= coef[q] * (double)uv[q];
+  for (int q0 = QB; q0 < k; q0 += 8) {  // basis vectors beyond the held ones
+    T ub[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ub[u] = U[(long long)(q0 + u < k ? q0 + u : 0) * n + c];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (q0 + u < k) sum -= coef[q0 + u] * (double)ub[u];
+  }
+  return (T)sum;
+}
+
+template <typename T, int QB>
+__global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restrict__ Ap, T* __restrict__ U,
+                                                      T* __restrict__ Z, int k, int m, double* partials,
+                                                      int nparts, GmCtl* ctl, double* hraw,
+                                                      double* inv_nu, double* R, double* g, double* cs,
+                                                      double* sn)
+{
+  __shared__ T tile[ONR];
+  __shared__ double tot[KP];
+  __shared__ double coef[MAXK];
+  __shared__ double col[MAXK + 1], gc[MAXK], gsv[MAXK];
+  __shared__ double sh[ONT / 64][KP];
+  __shared__ int s_done;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  // workgroup 0 may set ctl->done during this kernel: read it once per
+  // workgroup so every wave takes the same exit
+  if (tid == 0) s_done = ctl->done;
+  __syncthreads();
+  if (s_done) return;
+
+  const OpArgs<T> A = *Ap;
+  const int nx = A.G.nx, ny = A.G.ny;
+  const long long n = (long long)nx * ny;
+  const int tiles_x = (nx + OTX - 1) / OTX;
+  const int tile_id = xcd_banded(blockIdx.x, gridDim.x);
+  const int i0 = (tile_id % tiles_x) * OTX, j0 = (tile_id / tiles_x) * OTY;
+  const int ci = tid % OTX, cj = tid / OTX;
+  const int gi = i0 + ci, gj = j0 + cj;
+  const bool own = gi < nx && gj < ny;
+  const long long kown = (long long)gj * nx + gi;
+  const bool apply = k < m;  // the last kernel only needs |u_m|
+
+  // Column read for tile slot (li, lj): the cell itself, its mapped column
+  // outside the domain, or false for a FIXED side's zero.
+  auto slot_col = [&](int li, int lj, long long& c) -> bool {
+    const int i = i0 + li - 2, j = j0 + lj - 2;
+    c = 0;
+    if (i >= 0 && j >= 0 && i < nx && j < ny) {
+      c = (long long)j * nx + i;
+      return true;
+    }
+    if (fixed_side(A.G, i, j) >= 0) return false;
+    c = pcol(A.G, i, j);
+    return true;
+  };
+  const int oslot = (cj + 2) * OHX + (ci + 2);
+  long long co, ch = 0;
+  const bool vo = slot_col(ci + 2, cj + 2, co);
+  const bool has_h = tid < 256;
+  int hslot = 0;
+  bool vh = false;
+  if (has_h) {
+    int hli, hlj;
+    halo_slot(tid, hli, hlj);
+    hslot = hlj * OHX + hli;
+    vh = slot_col(hli, hlj, ch);
+  }
+
+  // Every global read that does not depend on the reductions, issued up
+  // front and unconditionally (indices clamped into the arrays, unused values
+  // selected away), so that none waits for another: the own row's face
+  // coefficients, z_{k-1} and the first QB basis vectors at the own and halo
+  // slots, and the previous kernel's partials.
+  const int hi = gi < nx ? gi : nx - 1, hj = gj < ny ? gj : ny - 1;
+  const T he = A.hx[(long long)hj * (nx + 1) + hi + 1];
+  const T hw = A.hx[(long long)hj * (nx + 1) + hi];
+  const T hn = A.hy[(long long)(hj + 1) * nx + hi];
+  const T hs = A.hy[(long long)hj * nx + hi];
+  const T* zsrc = k == 0 ? U : Z + (long long)((k + 1) & 1) * n;  // u_0, or z_{k-1}
+  const T zo = zsrc[co], zh = zsrc[ch];
+  T uo[QB], uh[QB];
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const long long qn = (long long)(q < k ? q : 0) * n;
+    uo[q] = U[qn + co];
+    uh[q] = U[qn + ch];
+  }
+  double pv[2][4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int e = wave + 16 * r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int b = lane + 64 * i;
+      pv[r][i] = partials[(long long)(b < nparts ? b : 0) * KP + (e < KP ? e : 0)];
+    }
+  }
+
+  // 0. the previous kernel's reductions and column k-2's Givens step
+  double cz = 1.0;
+  if (k > 0) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int e = wave + 16 * r;
+      if (e <= k) {
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v += lane + 64 * i < nparts ? pv[r][i] : 0.0;
+        for (int b = lane + 256; b < nparts; b += 64) v += partials[(long long)b * KP + e];
+        v = wave_sum_lane63(v);
+        if (lane == 63) tot[e] = v;
+      }
+    }
+    sum_tile_entries(partials, nparts, 32, k + 1, tot);
+    if (k >= 2) {
+      for (int i = tid; i <= k - 2; i += ONT) {
+        col[i] = hraw[(long long)(k - 2) * (m + 1) + i];
+        if (i < k - 2) {
+          gc[i] = cs[i];
+          gsv[i] = sn[i];
+        }
+      }
+    }
+    __syncthreads();
+    const double nrm2 = tot[k];  // |u_{k-1}|^2
+    const double inv_prev = k == 1 ? 1.0 : 1.0 / sqrt(nrm2);
+    for (int q = tid; q < k; q += ONT) {
+      const double iq = q == k - 1 ? inv_prev : inv_nu[q];
+      const double h = tot[q] * iq * inv_prev;
+      coef[q] = h * iq;
+      if (blockIdx.x == 0) hraw[(long long)(k - 1) * (m + 1) + q] = h;
+    }
+    if (tid == 0) {
+      if (blockIdx.x == 0) inv_nu[k - 1] = inv_prev;
+      int done = 0;
+      if (k >= 2) {
+        done = givens_step(nrm2, col, gc, gsv, R, g, cs, sn, k - 2, m, ctl, blockIdx.x == 0).done;
+      }
+      s_done = done;
+    }
+    __syncthreads();
+    if (s_done) return;
+    cz = inv_prev;
+  } else if (blockIdx.x == 0 && tid == 0) {
+    inv_nu[0] = 1.0;
+  }
+
+  // 1. u_k at the own and halo slots (FP64 accumulation, stored in T)
+  if (has_h) tile[hslot] = vh ? form_value<T, QB>(uh, zh, U, n, ch, coef, cz, k) : T(0);
+  tile[oslot] = vo ? form_value<T, QB>(uo, zo, U, n, co, coef, cz, k) : T(0);
+  __syncthreads();
+
+  // 2. z_k = A u_k (the operator row of apply_row_g, from LDS)
+  double ud = 0.0, zd = 0.0;
+  if (own) {
+    const T pc = tile[oslot];
+    if (apply) {
+      const T xe = tile[oslot + 1], xw = tile[oslot - 1];
+      const T xee = tile[oslot + 2], xww = tile[oslot - 2];
+      const T yn = tile[oslot + OHX], ys = tile[oslot - OHX];
+      const T ynn = tile[oslot + 2 * OHX], yss = tile[oslot - 2 * OHX];
+      const T cxd = xe - xw, cyd = yn - ys;
+      const T xpart = he * ((xee - pc) + cxd) - hw * ((pc - xww) + cxd);
+      const T ypart = hn * ((ynn - pc) + cyd) - hs * ((pc - yss) + cyd);
+      const T zt = A.diag * pc - A.cx * xpart - A.cy * ypart;
+      Z[(long long)(k & 1) * n + kown] = zt;
+      if (k > 0) U[(long long)k * n + kown] = pc;
+      zd = (double)zt;
+    }
+    ud = (double)pc;
+  }
+
+  // 3. tile partials: u_q . z_k (q < k from the held values), u_k . z_k,
+  // u_k . u_k
+  const bool dots = own && apply;
+#pragma unroll
+  for (int e0 = 0; e0 < QB; e0 += 8) {
+    if (e0 < k) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (dots && e0 + u < k) ? (double)uo[e0 + u] * zd : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = wave_sum_lane63(v[u]);
+      if (lane == 63) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (e0 + u < k) sh[wave][e0 + u] = v[u];
+      }
+    }
+  }
+  for (int e0 = QB; e0 < k; e0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double x = (double)U[(long long)(e0 + u < k ? e0 + u : 0) * n + co];
+      v[u] = (dots && e0 + u < k) ? x * zd : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = wave_sum_lane63(v[u]);
+    if (lane == 63) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (e0 + u < k) sh[wave][e0 + u] = v[u];
+    }
+  }
+  {
+    const double a = wave_sum_lane63(own ? ud * zd : 0.0);
+    const double b2 = wave_sum_lane63(own ? ud * ud : 0.0);
+    if (lane == 63) {
+      sh[wave][k] = a;
+      sh[wave][k + 1] = b2;
+    }
+  }
+  __syncthreads();
+  const int ne = k + 2;
+  if (tid < ne) {
+    double v = sh[0][tid];
+    for (int w = 1; w < ONT / 64; ++w) v += sh[w][tid];
+    partials[(long long)blockIdx.x * KP + tid] = v;
+  }
+}
+
+// Closes a one-reduction cycle: |u_m| from the last kernel's partials and
+// column m-1's Givens step.
+__global__ void __launch_bounds__(ONT) onesync_close_kernel(const double* partials, int nparts, int m,
+                                                            GmCtl* ctl, const double* hraw, double* R,
+                                                            double* g, double* cs, double* sn)
+{
+  __shared__ double tot[KP];
+  __shared__ double col[MAXK + 1], gc[MAXK], gsv[MAXK];
+  if (ctl->done) return;  // no other workgroup writes ctl during this kernel
+  const int tid = threadIdx.x;
+  sum_tile_entries(partials, nparts, m + 1, m + 2, tot);
+  for (int i = tid; i <= m - 1; i += ONT) {
+    col[i] = hraw[(long long)(m - 1) * (m + 1) + i];
+    if (i < m - 1) {
+      gc[i] = cs[i];
+      gsv[i] = sn[i];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) (void)givens_step(tot[m + 1], col, gc, gsv, R, g, cs, sn, m - 1, m, ctl, true);
+}
+
+// ------------------------------------------------ mixed form: FGMRES-IR
+// The mixed form is iterative refinement accelerated by a flexible outer
+// Krylov solver in FP64 (FGMRES, Saad, SIAM J. Sci. Comput. 14 (1993)
+// 461-469) whose preconditioner is the FP32 inner GMRES(m) (Arioli and
+// Duff, "Using FGMRES to obtain backward stability in mixed precision",
+// Electron. Trans. Numer. Anal. 33 (2009) 31-44).  Each outer step solves
+// A z_j = v_j approximately in FP32; the outer basis, its orthogonalisation,
+// the update x += Z y and the true residual are FP64, so the attainable
+// accuracy is FP64's while the Krylov work is FP32.
+constexpr int MO = 20;  // outer restart length
+
+// A fresh inner cycle on a unit right-hand side, or none once the outer
+// cycle is done.
+__global__ void inner_reset_kernel(GmCtl* ci, const GmCtl* co, int mlim)
+{
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  ci->done = co->done;
+  ci->kk = 0;
+  ci->bad = 0;
+  ci->breakdown = 0;
+  ci->mlim = mlim;
+  ci->beta = 1.0;
+  ci->thr = co->inner_thr;
+}
+
+// Outer step j: y = A z_j (FP64) into w and the first Gram-Schmidt pass's
+// dots V_q . y (q <= j).
+__global__ void __launch_bounds__(RB) outer_arnoldi_kernel(const OpArgs<double>* __restrict__ Ap,
+                                                           const double* __restrict__ V,
+                                                           const double* __restrict__ z, int j,
+                                                           double* __restrict__ w, double* partials,
+                                                           const GmCtl* ctl)
+{
+  if (ctl->done) return;  // nothing writes ctl during this kernel
+  const OpArgs<double> A = *Ap;
+  const long long n = (long long)A.G.nx * A.G.ny;
+  const long long t = (long long)blockIdx.x * RB + threadIdx.x;
+  double yd = 0.0;
+  if (t < n) {
+    yd = apply_row(A, z, (int)(t % A.G.nx), (int)(t / A.G.nx));
+    w[t] = yd;
+  }
+  rowdots_store([&](int q) { return t < n ? V[q * n + t] * yd : 0.0; }, j + 1, partials);
+}
+
+// V_{j+1} = w / |w|, and the inner iterations of step j counted into the
+// outer control block.
+__global__ void __launch_bounds__(RB) outer_next_kernel(const double* __restrict__ w,
+                                                        double* __restrict__ vnext, long long n,
+                                                        GmCtl* co, const GmCtl* ci)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) co->inner_its += ci->kk;
+  if (co->done) return;
+  const double s = co->inv_norm;
+  const long long t = (long long)blockIdx.x * RB + threadIdx.x;
+  if (t < n) vnext[t] = w[t] * s;
+}
+
 // Arnoldi iteration j, first half.  For j > 0 every workgroup first closes
 // column j - 1 (its Givens step, from the norm partials of the previous
 // iteration; workgroup 0 commits it) and exits if that ends the cycle.  Then
@@ -855,22 +1251,34 @@ __global__ void __launch_bounds__(256) cycle_start_kernel(const double* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) g[0] = beta;
 }
 
-// y = R^{-1} g (kk x kk upper triangular), then x += V y (x in FP64; V in T).
-__global__ void backsolve_kernel(const double* R, const double* g, double* y, int m, GmCtl* ctl)
+// y = R^{-1} g (kk x kk upper triangular; column-oriented back substitution,
+// one lane per row with the columns staged in LDS), each y_q times scale[q]
+// when a scale is given (the one-reduction cycle's unnormalised basis); then
+// x += V y (x in FP64; V in T).
+__global__ void __launch_bounds__(64) backsolve_kernel(const double* R, const double* g, double* y,
+                                                       int m, GmCtl* ctl, const double* scale)
 {
-  if (threadIdx.x != 0) return;
+  __shared__ double Rs[MAXK * MAXK];
   const int kk = ctl->kk;
+  const int r = threadIdx.x;
+  for (int q = 0; q < kk; ++q) Rs[q * MAXK + r] = r <= q ? R[(long long)q * (m + 1) + r] : 0.0;
+  double s = r < kk ? g[m + 1 + r] : 0.0;  // rotated right-hand side
+  __syncthreads();
+  double yr = 0.0;
   for (int i = kk - 1; i >= 0; --i) {
-    double s = g[m + 1 + i];  // rotated right-hand side
-    for (int q = i + 1; q < kk; ++q) s -= R[(long long)q * (m + 1) + i] * y[q];
-    y[i] = s / R[(long long)i * (m + 1) + i];
-    if (!isfinite(y[i])) ctl->bad = 1;
+    const double yi = __shfl(s, i) / Rs[i * MAXK + i];
+    if (r == i) yr = yi;
+    if (r < i) s -= Rs[i * MAXK + r] * yi;
+  }
+  if (r < kk) {
+    if (!isfinite(yr)) ctl->bad = 1;
+    y[r] = scale ? yr * scale[r] : yr;
   }
 }
 
 template <typename T, typename X>
 __global__ void __launch_bounds__(256) xupdate_kernel(const T* __restrict__ V, long long n, const double* __restrict__ y,
-                               X* __restrict__ x, double scale, const GmCtl* ctl)
+                               X* __restrict__ x, double scale, const GmCtl* ctl, int assign)
 {
   const int kk = ctl->kk;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -884,7 +1292,7 @@ __global__ void __launch_bounds__(256) xupdate_kernel(const T* __restrict__ V, l
       for (int u = 0; u < 8; ++u)
         if (q0 + u < kk) s += y[q0 + u] * vb[u];
     }
-    x[t] = (X)((double)x[t] + scale * s);
+    x[t] = assign ? (X)(scale * s) : (X)((double)x[t] + scale * s);
   }
 }
 
@@ -1032,6 +1440,10 @@ static ec_status imex_workspace(ec_ctx* ctx, int m)
   A((void**)&w->y, (size_t)m * sizeof(double));
   A((void**)&w->partials, (size_t)w->nparts * MAXK * sizeof(double));
   A((void**)&w->npart, (size_t)w->nparts * sizeof(double));
+  w->otiles = onesync_tiles(w->nx, w->ny);
+  A((void**)&w->opart, (size_t)w->otiles * KP * sizeof(double));
+  A((void**)&w->hraw, (size_t)(m + 1) * m * sizeof(double));
+  A((void**)&w->inv_nu, (size_t)(m + 1) * sizeof(double));
   A((void**)&w->ctl, sizeof(GmCtl));
   ok = ok && hipHostMalloc((void**)&w->ctl_host, sizeof(GmCtl)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&w->hA64, sizeof(OpArgs<double>)) == hipSuccess;
@@ -1104,9 +1516,10 @@ static void enqueue_cycle(ImexWork* w, hipStream_t st, const OpArgs<T>* dA, T* V
   // the last column's Givens step (the others ran inside the next Arnoldi kernel)
   hipLaunchKernelGGL(norm_givens_kernel, dim3(1), dim3(RB), 0, st, w->npart, nb, w->R, w->g, w->cs,
                      w->sn, m - 1, m, w->ctl, two_pass);
-  hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, st, w->R, w->g, w->y, m, w->ctl);
+  hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, st, w->R, w->g, w->y, m, w->ctl,
+                     (const double*)nullptr);
   hipLaunchKernelGGL((xupdate_kernel<T, double>), dim3(nb), dim3(RB), 0, st, V, n, w->y, w->x, 1.0,
-                     w->ctl);
+                     w->ctl, 0);
   hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(RB), 0, st, w->dA64, w->x, w->b, w->r,
                      w->partials, w->ctl);
   hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, st, w->partials, nb, 0,
@@ -1114,14 +1527,75 @@ static void enqueue_cycle(ImexWork* w, hipStream_t st, const OpArgs<T>* dA, T* V
   (void)hipMemcpyAsync(w->ctl_host, w->ctl, sizeof(GmCtl), hipMemcpyDeviceToHost, st);
 }
 
-template <typename T>
-static ec_status cycle_graph(ec_ctx* ctx, ImexWork* w, const OpArgs<T>* dA, T* V, T* wv, int two_pass,
-                             hipGraphExec_t* exec)
+// Outer step j of the mixed form: the FP32 inner GMRES(m) solve of
+// A z_j = v_j (one-reduction kernels, unit right-hand side, relative
+// tolerance inner_thr), z_j in FP64, then w = A z_j orthogonalised against
+// V_0..V_j in FP64 (CGS2), column j's Givens step and V_{j+1} = w / |w|.
+// Every kernel returns at once when the outer cycle is done.
+static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
+{
+  const long long n = (long long)w->nx * w->ny;
+  const int nb = grid_blocks(n);
+  const int m = w->m;
+  double* vj = w->Vo + (long long)j * n;
+  double* zj = w->Zo + (long long)j * n;
+  hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(64), 0, st, w->ctl, w->ctlo, m);
+  hipLaunchKernelGGL((cycle_start_kernel<float>), dim3(nb), dim3(RB), 0, st, vj, w->V32, n, w->ctl,
+                     w->g);
+  for (int k = 0; k <= m; ++k) {
+    // the smallest instantiation that holds the k basis values in registers
+    auto kern = k <= 8 ? onesync_kernel<float, 8> : k <= 16 ? onesync_kernel<float, 16>
+                                                            : onesync_kernel<float, 32>;
+    hipLaunchKernelGGL(kern, dim3(w->otiles), dim3(ONT), 0, st, w->dA32, w->V32, w->w32, k, m, w->opart,
+                       w->otiles, w->ctl, w->hraw, w->inv_nu, w->R, w->g, w->cs, w->sn);
+  }
+  hipLaunchKernelGGL(onesync_close_kernel, dim3(1), dim3(ONT), 0, st, w->opart, w->otiles, m, w->ctl,
+                     w->hraw, w->R, w->g, w->cs, w->sn);
+  hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, st, w->R, w->g, w->y, m, w->ctl,
+                     (const double*)w->inv_nu);
+  hipLaunchKernelGGL((xupdate_kernel<float, double>), dim3(nb), dim3(RB), 0, st, w->V32, n, w->y, zj,
+                     1.0, w->ctl, 1);
+  hipLaunchKernelGGL(outer_arnoldi_kernel, dim3(nb), dim3(RB), 0, st, w->dA64, w->Vo, zj, j, w->w,
+                     w->partials, w->ctlo);
+  hipLaunchKernelGGL(finalize_dots_kernel, dim3(j + 1), dim3(RB), 0, st, w->partials, nb, MAXK,
+                     w->ctlo->dots1, w->ctlo);
+  hipLaunchKernelGGL((update_dot_kernel<double>), dim3(nb), dim3(RB), 0, st, w->Vo, n, j + 1, w->w,
+                     w->partials, w->ctlo);
+  hipLaunchKernelGGL(finalize_dots_kernel, dim3(j + 1), dim3(RB), 0, st, w->partials, nb, MAXK,
+                     w->ctlo->dots2, w->ctlo);
+  hipLaunchKernelGGL((update_norm_kernel<double>), dim3(nb), dim3(RB), 0, st, w->Vo, n, j + 1, w->w,
+                     w->npart, w->ctlo, 2);
+  hipLaunchKernelGGL(norm_givens_kernel, dim3(1), dim3(RB), 0, st, w->npart, nb, w->Ro, w->go, w->cso,
+                     w->sno, j, MO, w->ctlo, 1);
+  hipLaunchKernelGGL(outer_next_kernel, dim3(nb), dim3(RB), 0, st, w->w, w->Vo + (long long)(j + 1) * n,
+                     n, w->ctlo, w->ctl);
+}
+
+// Closes an outer cycle: y = R^{-1} g, x += Z y (FP64), the true residual
+// and its norms, and the outer control block back to the host.
+static void enqueue_outer_close(ImexWork* w, hipStream_t st)
+{
+  const long long n = (long long)w->nx * w->ny;
+  const int nb = grid_blocks(n);
+  hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, st, w->Ro, w->go, w->yo, MO, w->ctlo,
+                     (const double*)nullptr);
+  hipLaunchKernelGGL((xupdate_kernel<double, double>), dim3(nb), dim3(RB), 0, st, w->Zo, n, w->yo, w->x,
+                     1.0, w->ctlo, 0);
+  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(RB), 0, st, w->dA64, w->x, w->b, w->r,
+                     w->partials, w->ctlo);
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, st, w->partials, nb, 0,
+                     &w->ctlo->rinf, &w->ctlo->r2);
+  (void)hipMemcpyAsync(w->ctlo_host, w->ctlo, sizeof(GmCtl), hipMemcpyDeviceToHost, st);
+}
+
+// Captures one restart cycle (enqueue(stream)) into a hipGraph, once.
+template <typename F>
+static ec_status cycle_graph(ec_ctx* ctx, ImexWork* w, hipGraphExec_t* exec, const F& enqueue)
 {
   if (*exec) return EC_OK;
   hipGraph_t g = nullptr;
   EC_HIP_TRY(ctx, hipStreamBeginCapture(w->gs, hipStreamCaptureModeThreadLocal));
-  enqueue_cycle<T>(w, w->gs, dA, V, wv, two_pass);
+  enqueue(w->gs);
   const hipError_t e = hipStreamEndCapture(w->gs, &g);
   if (e != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
@@ -1160,6 +1634,91 @@ static ec_status true_residual(ec_ctx* ctx, ImexWork* w, hipStream_t st, double*
   return EC_OK;
 }
 
+// The mixed form's FP64 outer buffers and step graphs (first use only).
+static ec_status outer_setup(ec_ctx* ctx, ImexWork* w)
+{
+  if (w->Vo && (int)w->exec_outer.size() == MO + 1) return EC_OK;
+  const size_t n = (size_t)w->nx * w->ny;
+  if (!w->Vo) {
+    bool ok = true;
+    auto A = [&](void** p, size_t bytes) { ok = ok && hipMalloc(p, bytes) == hipSuccess; };
+    A((void**)&w->Vo, (size_t)(MO + 1) * n * sizeof(double));
+    A((void**)&w->Zo, (size_t)MO * n * sizeof(double));
+    A((void**)&w->Ro, (size_t)(MO + 1) * MO * sizeof(double));
+    A((void**)&w->go, (size_t)2 * (MO + 1) * sizeof(double));
+    A((void**)&w->cso, (size_t)MO * sizeof(double));
+    A((void**)&w->sno, (size_t)MO * sizeof(double));
+    A((void**)&w->yo, (size_t)MO * sizeof(double));
+    A((void**)&w->ctlo, sizeof(GmCtl));
+    ok = ok && hipHostMalloc((void**)&w->ctlo_host, sizeof(GmCtl)) == hipSuccess;
+    if (!ok) return hip_fail(ctx, hipErrorOutOfMemory, "mixed-precision outer workspace");
+    std::memset(w->ctlo_host, 0, sizeof(GmCtl));
+    EC_HIP_TRY(ctx, hipMemcpy(w->ctlo, w->ctlo_host, sizeof(GmCtl), hipMemcpyHostToDevice));
+  }
+  w->exec_outer.assign(MO + 1, nullptr);
+  for (int j = 0; j <= MO; ++j) {
+    const ec_status st = cycle_graph(ctx, w, &w->exec_outer[j], [&](hipStream_t gs) {
+      if (j < MO) {
+        enqueue_outer_step(w, gs, j);
+      } else {
+        enqueue_outer_close(w, gs);
+      }
+    });
+    if (st != EC_OK) return st;
+  }
+  return EC_OK;
+}
+
+// Mixed form (FGMRES-IR): outer FGMRES(MO) cycles in FP64 from the current
+// x, each outer step preconditioned by an FP32 GMRES(m) solve, until the
+// reference's criterion holds.  Iterations are the FP32 inner iterations,
+// refinements the outer steps.
+static ec_status solve_mixed(ec_ctx* ctx, ImexWork* w, double target, double rhs_norm,
+                             const ec_imex_opts& o, double rinf, double r2, SolveResult* out)
+{
+  ec_status st = outer_setup(ctx, w);
+  if (st != EC_OK) return st;
+  const long long n = (long long)w->nx * w->ny;
+  GmCtl& h = *w->ctlo_host;
+  int total = 0;
+  while (rinf > target && total < o.max_iter) {
+    const double beta = std::sqrt(r2);
+    if (!(beta > 0.0)) break;
+    h.done = 0;
+    h.kk = 0;
+    h.bad = 0;
+    h.breakdown = 0;
+    h.mlim = MO;
+    h.inner_its = 0;
+    h.beta = beta;
+    h.thr = target * (beta / rinf);  // as in the FP64 form
+    h.inner_thr = o.inner_reduction;
+    EC_HIP_TRY(ctx, hipMemcpyAsync(w->ctlo, w->ctlo_host, offsetof(GmCtl, dots1),
+                                   hipMemcpyHostToDevice, w->gs));
+    hipLaunchKernelGGL((cycle_start_kernel<double>), dim3(grid_blocks(n)), dim3(RB), 0, w->gs, w->r,
+                       w->Vo, n, w->ctlo, w->go);
+    EC_HIP_TRY(ctx, hipGetLastError());
+    for (int j = 0; j <= MO; ++j) EC_HIP_TRY(ctx, hipGraphLaunch(w->exec_outer[j], w->gs));
+    EC_HIP_TRY(ctx, hipStreamSynchronize(w->gs));
+    const int kk = h.kk;
+    total += h.inner_its;
+    out->refinements += kk;
+    out->breakdown = h.breakdown != 0;
+    rinf = h.rinf;
+    r2 = h.r2;
+    out->rinf = rinf;
+    out->rel = rinf / rhs_norm;
+    if (h.bad || !std::isfinite(rinf)) {
+      out->bad = true;
+      break;
+    }
+    if (out->breakdown && rinf > target) break;
+    if (kk == 0) break;
+  }
+  out->iterations = total;
+  return EC_OK;
+}
+
 // Solves A x = b (x holds the initial guess) to the reference's criterion
 // |r|_inf <= tol or |r|_inf / max(|b|_inf,1) <= tol
 // (imex/imex_pressure_helpers.hpp:105-125).
@@ -1189,10 +1748,12 @@ static ec_status solve_pressure(ec_ctx* ctx, ImexWork* w, const OpArgs<double>& 
     out->bad = true;
     return EC_OK;
   }
-  const bool fp64 = o.precision == EC_IMEX_PRECISION_FP64;
-  hipGraphExec_t* exec = fp64 ? &w->exec64 : &w->exec32;
-  st = fp64 ? cycle_graph<double>(ctx, w, w->dA64, w->V, w->w, 1, exec)
-            : cycle_graph<float>(ctx, w, w->dA32, w->V32, w->w32, 0, exec);
+  if (o.precision == EC_IMEX_PRECISION_MIXED) {
+    return solve_mixed(ctx, w, target, rhs_norm, o, rinf, r2, out);
+  }
+  st = cycle_graph(ctx, w, &w->exec64, [&](hipStream_t gs) {
+    enqueue_cycle<double>(w, gs, w->dA64, w->V, w->w, 1);
+  });
   if (st != EC_OK) return st;
   GmCtl& h = *w->ctl_host;
   while (rinf > target && total < max_it) {
@@ -1205,18 +1766,10 @@ static ec_status solve_pressure(ec_ctx* ctx, ImexWork* w, const OpArgs<double>& 
     h.mlim = std::min(m, max_it - total);
     h.beta = beta;
     h.counter = 0;
-    if (fp64) {
-      // 2-norm estimate threshold scaled by the |r|_2 / |r|_inf ratio of the
-      // cycle's start residual; the true residual decides convergence.
-      h.thr = target * (beta / rinf);
-    } else {
-      // Iterative refinement: FP32 GMRES on A d = r/beta from d = 0, x += d
-      // in FP64; the inner solve reduces the 2-norm by o.inner_reduction,
-      // never below the FP64 goal.
-      h.thr = smax(o.inner_reduction * beta, target * (beta / rinf));
-      ++out->refinements;
-    }
-    EC_HIP_TRY(ctx, hipGraphLaunch(*exec, w->gs));
+    // 2-norm estimate threshold scaled by the |r|_2 / |r|_inf ratio of the
+    // cycle's start residual; the true residual decides convergence.
+    h.thr = target * (beta / rinf);
+    EC_HIP_TRY(ctx, hipGraphLaunch(w->exec64, w->gs));
     EC_HIP_TRY(ctx, hipStreamSynchronize(w->gs));
     const int kk = h.kk;
     const bool bad = h.bad != 0, brk = h.breakdown != 0;
