@@ -732,13 +732,16 @@ __device__ GivensOut givens_step(double v, double* col, const double* c, const d
   // breakdown tolerance 100 eps max(beta, 1) (imex_pressure_helpers.hpp:167-169)
   const double breakdown_tol = 2.220446049250313e-16 * 100.0 * smax(ctl->beta, 1.0);
   const double hn = sqrt(v);
-  col[j + 1] = hn;
+  // Rotation i mixes entries i and i+1; the rotated entry i+1 is the next
+  // rotation's first operand, so it is carried in a register and the loads
+  // of the column stay off the dependency chain.
+  double carry = col[0];
   for (int i = 0; i < j; ++i) {
-    const double a = col[i], b = col[i + 1];
+    const double a = carry, b = col[i + 1];
     col[i] = c[i] * a + sv[i] * b;
-    col[i + 1] = -sv[i] * a + c[i] * b;
+    carry = -sv[i] * a + c[i] * b;
   }
-  const double a = col[j], b = col[j + 1];
+  const double a = carry, b = hn;
   const double rr = hypot(a, b);
   const double cj = rr == 0.0 ? 1.0 : a / rr;
   const double sj = rr == 0.0 ? 0.0 : b / rr;
@@ -819,7 +822,7 @@ __global__ void __launch_bounds__(RB) norm_givens_kernel(const double* npart, in
 //   3. reduces u_q . z_k (q <= k) and u_k . u_k per tile.
 // The mathematics is that of CGS1 (the FP32 inner solver's
 // orthogonalisation), in one kernel per Arnoldi step instead of three.
-constexpr int OTX = 32, OTY = 32, ONT = OTX * OTY;  // one cell per thread
+constexpr int OTX = 64, OTY = 16, ONT = OTX * OTY;  // one cell per thread
 constexpr int OHX = OTX + 4, OHY = OTY + 4, ONR = OHX * OHY;
 constexpr int KP = MAXK + 2;  // partial entries per tile: MAXK + 1 dots and one norm
 
@@ -843,20 +846,22 @@ __device__ __forceinline__ void sum_tile_entries(const double* partials, int npa
   }
 }
 
-// Tile-local (li, lj) of halo slot h < 256 of the 32 x 32 tile: the two
-// rows above, the two below, the two columns left and the two right.
+// Tile-local (li, lj) of halo slot h < NHALO: the two rows above the tile,
+// the two below, then the two columns left and the two right.
+constexpr int NHALO = 4 * OTX + 4 * OTY;
 __device__ __forceinline__ void halo_slot(int h, int& li, int& lj)
 {
-  const int seg = h >> 6, r = h & 63;
-  if (seg < 2) {
-    li = 2 + (r & (OTX - 1));
-    lj = (seg == 0 ? 0 : OTY + 2) + (r >> 5);
+  if (h < 4 * OTX) {
+    const int r = h % (2 * OTX), below = h / (2 * OTX);
+    li = 2 + r % OTX;
+    lj = (below ? OTY + 2 : 0) + r / OTX;
   } else {
-    li = (seg == 2 ? 0 : OTX + 2) + (r & 1);
+    const int s = h - 4 * OTX, r = s % (2 * OTY), right = s / (2 * OTY);
+    li = (right ? OTX + 2 : 0) + (r & 1);
     lj = 2 + (r >> 1);
   }
 }
-static_assert(OTX == 32 && OTY == 32, "halo_slot enumerates the 256 halo cells of a 32 x 32 tile");
+static_assert(NHALO <= ONT, "one halo slot per thread at most");
 
 // u_k at column c: z_{k-1} / nu_{k-1} - sum_q c_q u_q (FP64 accumulation),
 // with the first QB basis values already loaded (uv) and the rest read here;
@@ -896,6 +901,8 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
   __shared__ int s_done;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  constexpr int GIVENS_TID = ONT - 64;  // a lane of the last wave, which has no halo slot
+  static_assert(GIVENS_TID >= NHALO, "the Givens lane's wave forms no halo slot");
   // workgroup 0 may set ctl->done during this kernel: read it once per
   // workgroup so every wave takes the same exit
   if (tid == 0) s_done = ctl->done;
@@ -930,7 +937,7 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
   const int oslot = (cj + 2) * OHX + (ci + 2);
   long long co, ch = 0;
   const bool vo = slot_col(ci + 2, cj + 2, co);
-  const bool has_h = tid < 256;
+  const bool has_h = tid < NHALO;
   int hslot = 0;
   bool vh = false;
   if (has_h) {
@@ -942,9 +949,25 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
 
   // Every global read that does not depend on the reductions, issued up
   // front and unconditionally (indices clamped into the arrays, unused values
-  // selected away), so that none waits for another: the own row's face
-  // coefficients, z_{k-1} and the first QB basis vectors at the own and halo
-  // slots, and the previous kernel's partials.
+  // selected away), so that none waits for another.  The previous kernel's
+  // partials go first: loads complete in order, so stage 0 can run while the
+  // basis loads behind them are still in flight.
+  double pv[2][4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int e = wave + 16 * r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int b = lane + 64 * i;
+      pv[r][i] = partials[(long long)(b < nparts ? b : 0) * KP + e];
+    }
+  }
+  // Givens inputs of column k-2 and the basis scales, ahead of the basis
+  // loads too (thread i holds entry i)
+  const int gi_ = tid < m ? tid : 0;  // clamped: every load is in bounds
+  const double gcol = hraw[(long long)(k >= 2 ? k - 2 : 0) * (m + 1) + gi_];
+  const double gcs = cs[gi_], gsn = sn[gi_];
+  const double ginv = inv_nu[gi_];
   const int hi = gi < nx ? gi : nx - 1, hj = gj < ny ? gj : ny - 1;
   const T he = A.hx[(long long)hj * (nx + 1) + hi + 1];
   const T hw = A.hx[(long long)hj * (nx + 1) + hi];
@@ -959,70 +982,52 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
     uo[q] = U[qn + co];
     uh[q] = U[qn + ch];
   }
-  double pv[2][4];
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int e = wave + 16 * r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int b = lane + 64 * i;
-      pv[r][i] = partials[(long long)(b < nparts ? b : 0) * KP + (e < KP ? e : 0)];
-    }
-  }
 
-  // 0. the previous kernel's reductions and column k-2's Givens step
+  // 0. the previous kernel's reductions; column k-2's Givens step runs in
+  // one lane while the others form u_k
   double cz = 1.0;
   if (k > 0) {
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int e = wave + 16 * r;
-      if (e <= k) {
-        double v = 0.0;
+      double v = 0.0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v += lane + 64 * i < nparts ? pv[r][i] : 0.0;
+      for (int i = 0; i < 4; ++i) v += lane + 64 * i < nparts ? pv[r][i] : 0.0;
+      if (nparts > 256) {  // more tiles than the held partials cover
         for (int b = lane + 256; b < nparts; b += 64) v += partials[(long long)b * KP + e];
-        v = wave_sum_lane63(v);
-        if (lane == 63) tot[e] = v;
       }
+      v = wave_sum_lane63(v);
+      if (lane == 63 && e <= k) tot[e] = v;
     }
-    sum_tile_entries(partials, nparts, 32, k + 1, tot);
-    if (k >= 2) {
-      for (int i = tid; i <= k - 2; i += ONT) {
-        col[i] = hraw[(long long)(k - 2) * (m + 1) + i];
-        if (i < k - 2) {
-          gc[i] = cs[i];
-          gsv[i] = sn[i];
-        }
+    if (k >= 32) sum_tile_entries(partials, nparts, 32, k + 1, tot);
+    if (k >= 2 && tid <= k - 2) {
+      col[tid] = gcol;
+      if (tid < k - 2) {
+        gc[tid] = gcs;
+        gsv[tid] = gsn;
       }
     }
     __syncthreads();
-    const double nrm2 = tot[k];  // |u_{k-1}|^2
-    const double inv_prev = k == 1 ? 1.0 : 1.0 / sqrt(nrm2);
-    for (int q = tid; q < k; q += ONT) {
-      const double iq = q == k - 1 ? inv_prev : inv_nu[q];
-      const double h = tot[q] * iq * inv_prev;
-      coef[q] = h * iq;
-      if (blockIdx.x == 0) hraw[(long long)(k - 1) * (m + 1) + q] = h;
+    const double inv_prev = k == 1 ? 1.0 : 1.0 / sqrt(tot[k]);  // 1 / |u_{k-1}|
+    if (tid < k) {  // k <= MAXK < ONT: one entry per thread
+      const double iq = tid == k - 1 ? inv_prev : ginv;
+      const double h = tot[tid] * iq * inv_prev;
+      coef[tid] = h * iq;
+      if (blockIdx.x == 0) hraw[(long long)(k - 1) * (m + 1) + tid] = h;
     }
-    if (tid == 0) {
-      if (blockIdx.x == 0) inv_nu[k - 1] = inv_prev;
-      int done = 0;
-      if (k >= 2) {
-        done = givens_step(nrm2, col, gc, gsv, R, g, cs, sn, k - 2, m, ctl, blockIdx.x == 0).done;
-      }
-      s_done = done;
-    }
+    if (tid == 0 && blockIdx.x == 0) inv_nu[k - 1] = inv_prev;
     __syncthreads();
-    if (s_done) return;
+    if (tid == GIVENS_TID && k >= 2) {
+      s_done = givens_step(tot[k], col, gc, gsv, R, g, cs, sn, k - 2, m, ctl, blockIdx.x == 0).done;
+    }
     cz = inv_prev;
-  } else if (blockIdx.x == 0 && tid == 0) {
-    inv_nu[0] = 1.0;
   }
 
   // 1. u_k at the own and halo slots (FP64 accumulation, stored in T)
   if (has_h) tile[hslot] = vh ? form_value<T, QB>(uh, zh, U, n, ch, coef, cz, k) : T(0);
   tile[oslot] = vo ? form_value<T, QB>(uo, zo, U, n, co, coef, cz, k) : T(0);
   __syncthreads();
+  if (s_done) return;
 
   // 2. z_k = A u_k (the operator row of apply_row_g, from LDS)
   double ud = 0.0, zd = 0.0;
@@ -1044,54 +1049,44 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
     ud = (double)pc;
   }
 
-  // 3. tile partials: u_q . z_k (q < k from the held values), u_k . z_k,
-  // u_k . u_k
-  const bool dots = own && apply;
+  // 3. tile partials: u_q . z_k (q < k from the held values), u_k . z_k and
+  // u_k . u_k, eight entries per wavefront butterfly
+  const int ne = k + 2;
+  auto tail = [&](int e) -> double { return e == k ? ud * zd : (e == k + 1 ? ud * ud : 0.0); };
+  auto store8 = [&](int e0, const double (&v)[8]) {
+    const double t8 = wave_sum8(v);
+    const int e = e0 + (lane >> 3);
+    if ((lane & 7) == 0 && e < ne) sh[wave][e] = t8;
+  };
 #pragma unroll
   for (int e0 = 0; e0 < QB; e0 += 8) {
-    if (e0 < k) {
+    if (e0 < ne) {
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = (dots && e0 + u < k) ? (double)uo[e0 + u] * zd : 0.0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = wave_sum_lane63(v[u]);
-      if (lane == 63) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (e0 + u < k) sh[wave][e0 + u] = v[u];
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u;
+        v[u] = !own ? 0.0 : e < k ? (apply ? (double)uo[e] * zd : 0.0) : tail(e);
       }
+      store8(e0, v);
     }
   }
-  for (int e0 = QB; e0 < k; e0 += 8) {
+  for (int e0 = QB; e0 < ne; e0 += 8) {  // basis vectors beyond the held ones
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const double x = (double)U[(long long)(e0 + u < k ? e0 + u : 0) * n + co];
-      v[u] = (dots && e0 + u < k) ? x * zd : 0.0;
+      const int e = e0 + u;
+      const double uq = (double)U[(long long)(e < k ? e : 0) * n + co];
+      v[u] = !own ? 0.0 : e < k ? (apply ? uq * zd : 0.0) : tail(e);
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = wave_sum_lane63(v[u]);
-    if (lane == 63) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (e0 + u < k) sh[wave][e0 + u] = v[u];
-    }
-  }
-  {
-    const double a = wave_sum_lane63(own ? ud * zd : 0.0);
-    const double b2 = wave_sum_lane63(own ? ud * ud : 0.0);
-    if (lane == 63) {
-      sh[wave][k] = a;
-      sh[wave][k + 1] = b2;
-    }
+    store8(e0, v);
   }
   __syncthreads();
-  const int ne = k + 2;
   if (tid < ne) {
     double v = sh[0][tid];
     for (int w = 1; w < ONT / 64; ++w) v += sh[w][tid];
     partials[(long long)blockIdx.x * KP + tid] = v;
   }
+  if (k == 0 && blockIdx.x == 0 && tid == 0) inv_nu[0] = 1.0;  // u_0 = r / beta
 }
 
 // Closes a one-reduction cycle: |u_m| from the last kernel's partials and
@@ -1544,8 +1539,10 @@ static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
                      w->g);
   for (int k = 0; k <= m; ++k) {
     // the smallest instantiation that holds the k basis values in registers
-    auto kern = k <= 8 ? onesync_kernel<float, 8> : k <= 16 ? onesync_kernel<float, 16>
-                                                            : onesync_kernel<float, 32>;
+    auto kern = k <= 8    ? onesync_kernel<float, 8>
+                : k <= 16 ? onesync_kernel<float, 16>
+                : k <= 24 ? onesync_kernel<float, 24>
+                          : onesync_kernel<float, 32>;
     hipLaunchKernelGGL(kern, dim3(w->otiles), dim3(ONT), 0, st, w->dA32, w->V32, w->w32, k, m, w->opart,
                        w->otiles, w->ctl, w->hraw, w->inv_nu, w->R, w->g, w->cs, w->sn);
   }

@@ -52,4 +52,43 @@ __device__ __forceinline__ double wave_sum_lane63(double v)
   return v;
 }
 
+// Swizzle of a double within 32-lane halves (bit-mask mode, xor XM).
+template <int XM>
+__device__ __forceinline__ double swizzle_xor_f64(double v)
+{
+  constexpr int pattern = 0x1F | (XM << 10);  // and 0x1F, or 0, xor XM
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_swizzle((int)b, pattern);
+  const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), pattern);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sums each of v[0..7] over the wavefront with a reduce-scatter butterfly:
+// three exchanges halve the values each lane holds (partners across 32
+// lanes, 16 lanes and the mirrored half of each 16-lane row), three DPP
+// steps finish the sum over each 8-lane group.  Returns, in every lane, the
+// total of value (lane >> 3) & 7.  The summation order is fixed.
+__device__ __forceinline__ double wave_sum8(const double (&v)[8])
+{
+  const int lane = threadIdx.x & 63;
+  double a[4], b[2];
+  const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0, h8 = (lane & 8) != 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = h32 ? v[4 + i] : v[i];
+    const double send = h32 ? v[i] : v[4 + i];
+    a[i] = keep + __shfl_xor(send, 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = h16 ? a[2 + i] : a[i];
+    const double send = h16 ? a[i] : a[2 + i];
+    b[i] = keep + swizzle_xor_f64<16>(send);
+  }
+  double c = (h8 ? b[1] : b[0]) + dpp_f64<0x140, 0xF>(h8 ? b[0] : b[1]);  // row_mirror
+  c += dpp_f64<0xB1, 0xF>(c);   // quad_perm [1,0,3,2]
+  c += dpp_f64<0x4E, 0xF>(c);   // quad_perm [2,3,0,1]
+  c += dpp_f64<0x141, 0xF>(c);  // row_half_mirror
+  return c;
+}
 }  // namespace ec
