@@ -725,12 +725,24 @@ struct GivensOut {
   int done;
 };
 
-__device__ GivensOut givens_step(double v, double* col, const double* c, const double* sv, double* R,
-                                 double* g, double* cs, double* sn, int j, int m, GmCtl* ctl,
-                                 bool write)
+// The control values column j's Givens step reads, loaded together (one
+// memory round trip) and ahead of the chain that needs them.
+struct GivensIn {
+  double beta, thr, gj;
+  int bad, breakdown, mlim;
+};
+
+__device__ __forceinline__ GivensIn givens_in(const GmCtl* ctl, const double* g, int j)
+{
+  return GivensIn{ctl->beta, ctl->thr, g[j], ctl->bad, ctl->breakdown, ctl->mlim};
+}
+
+__device__ GivensOut givens_step(double v, double* col, const double* c, const double* sv,
+                                 const GivensIn& in, double* R, double* g, double* cs, double* sn,
+                                 int j, int m, GmCtl* ctl, bool write)
 {
   // breakdown tolerance 100 eps max(beta, 1) (imex_pressure_helpers.hpp:167-169)
-  const double breakdown_tol = 2.220446049250313e-16 * 100.0 * smax(ctl->beta, 1.0);
+  const double breakdown_tol = 2.220446049250313e-16 * 100.0 * smax(in.beta, 1.0);
   const double hn = sqrt(v);
   // Rotation i mixes entries i and i+1; the rotated entry i+1 is the next
   // rotation's first operand, so it is carried in a register and the loads
@@ -747,12 +759,12 @@ __device__ GivensOut givens_step(double v, double* col, const double* c, const d
   const double sj = rr == 0.0 ? 0.0 : b / rr;
   col[j] = rr;
   col[j + 1] = 0.0;
-  const double gj = g[j];
+  const double gj = in.gj;
   const double gnext = -sj * gj;
   const double est = fabs(gnext);
-  const int bad = (!isfinite(hn) || !isfinite(est) || !isfinite(rr)) ? 1 : ctl->bad;
-  const int brk = hn <= breakdown_tol ? 1 : ctl->breakdown;
-  const int done = (est <= ctl->thr || j + 1 >= ctl->mlim || bad || brk) ? 1 : 0;
+  const int bad = (!isfinite(hn) || !isfinite(est) || !isfinite(rr)) ? 1 : in.bad;
+  const int brk = hn <= breakdown_tol ? 1 : in.breakdown;
+  const int done = (est <= in.thr || j + 1 >= in.mlim || bad || brk) ? 1 : 0;
   if (write) {
     cs[j] = cj;
     sn[j] = sj;
@@ -769,6 +781,13 @@ __device__ GivensOut givens_step(double v, double* col, const double* c, const d
     for (int i = 0; i <= j + 1; ++i) rc[i] = col[i];
   }
   return GivensOut{1.0 / hn, done};
+}
+
+__device__ GivensOut givens_step(double v, double* col, const double* c, const double* sv, double* R,
+                                 double* g, double* cs, double* sn, int j, int m, GmCtl* ctl,
+                                 bool write)
+{
+  return givens_step(v, col, c, sv, givens_in(ctl, g, j), R, g, cs, sn, j, m, ctl, write);
 }
 
 // Stages column j (first- plus, with two passes, second-pass dots) and the
@@ -968,6 +987,7 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
   const double gcol = hraw[(long long)(k >= 2 ? k - 2 : 0) * (m + 1) + gi_];
   const double gcs = cs[gi_], gsn = sn[gi_];
   const double ginv = inv_nu[gi_];
+  const GivensIn gin = givens_in(ctl, g, k >= 2 ? k - 2 : 0);
   const int hi = gi < nx ? gi : nx - 1, hj = gj < ny ? gj : ny - 1;
   const T he = A.hx[(long long)hj * (nx + 1) + hi + 1];
   const T hw = A.hx[(long long)hj * (nx + 1) + hi];
@@ -1018,7 +1038,7 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
     if (tid == 0 && blockIdx.x == 0) inv_nu[k - 1] = inv_prev;
     __syncthreads();
     if (tid == GIVENS_TID && k >= 2) {
-      s_done = givens_step(tot[k], col, gc, gsv, R, g, cs, sn, k - 2, m, ctl, blockIdx.x == 0).done;
+      s_done = givens_step(tot[k], col, gc, gsv, gin, R, g, cs, sn, k - 2, m, ctl, blockIdx.x == 0).done;
     }
     cz = inv_prev;
   }
