@@ -72,6 +72,47 @@ def test_gresho_imex_step_parity(ecor, ecamd, gpu, precision):
     assert d.coupled_residual_linf < 1e-3 * d.rhs_norm
 
 
+def test_mixed_long_inner_cycles_match_fp64(ecor, ecamd, gpu):
+    """The mixed form with GMRES(64) inner cycles run to their full length
+    (inner tolerance 1e-14): more than 32 Arnoldi steps per inner cycle, so
+    the one-reduction kernel's paths beyond its register-held basis (the
+    second sum of the tile partials, the basis vectors re-read for the
+    formation and the dots) all run; a 72 x 40 grid leaves partial 64 x 16
+    tiles on two sides (a Gresho vortex at M = 0.01 on non-square cells).
+    The pressure must satisfy the oracle's assembled rows to the solver
+    tolerance and agree with the FP64 form's state."""
+    nx, ny = 72, 40
+    U = ecor.init_gresho(nx, ny, mach=0.01)
+    g = ecor.make_grid(nx, ny, 1 / nx, 1 / ny)
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    S = ecor.imex_star(g, U, dt)
+    p0 = 0.4 * (U[3] - 0.5 * (U[1] ** 2 + U[2] ** 2) / U[0])
+    rp, cols, vals, rhs = ecor.imex_assemble(g, S, p0, np.ascontiguousarray(S[1:3]), dt)
+    s = ecamd.Solver(ecamd.Grid(nx, ny, 1 / nx, 1 / ny, 1.4, "hllc", "minmod", 2), gpu)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        Ud = torch.from_numpy(U).to(gpu)
+        p = torch.empty(nx * ny, dtype=torch.float64, device=gpu)
+        d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-10, max_iter=4000, precision=prec,
+                                            picard_iterations=1, restart=64,
+                                            inner_reduction=1e-14), p)
+        print(prec, ecamd.IMEX_STATUS[d.status], d.iterations, d.refinements,
+              d.relative_residual_linf)
+        assert ecamd.IMEX_STATUS[d.status] in ("converged_absolute", "converged_relative"), d.status
+        pd = p.cpu().numpy()
+        Ax = np.array([np.dot(vals[rp[r]:rp[r + 1]], pd[cols[rp[r]:rp[r + 1]]])
+                       for r in range(nx * ny)])
+        rel = np.abs(Ax - rhs).max() / max(np.abs(rhs).max(), 1.0)
+        print(prec, d.iterations, d.refinements, rel)
+        assert rel <= 2e-10, rel
+        if prec == "mixed":
+            assert d.refinements >= 1 and d.iterations > 32 * d.refinements, (d.iterations,
+                                                                              d.refinements)
+        out[prec] = Ud.cpu().numpy()
+    err = rel_l1(out["mixed"], out["fp64"])
+    assert err[0] == 0.0 and np.all(err <= 1e-7), err
+
+
 def test_uniform_flow_is_exact_on_device(ecor, ecamd, gpu):
     n = 32
     q = np.array([1.3, 0.2, -0.1, 5.0])
