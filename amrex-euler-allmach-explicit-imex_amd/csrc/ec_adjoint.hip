@@ -76,6 +76,9 @@ struct AdjWork {
   int* sel = nullptr;      // HLLC-P: sensor argmin per flux face
   double* mbar = nullptr;  // HLLC-P: bar of the selected ratio per flux face
   double* q = nullptr;     // 12 x GX*GY: primitives, x-slopes, y-slopes of grown cells
+  // checkpointed sweep: segment recompute on its own stream, double-buffered
+  hipStream_t rstream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_use[2] = {nullptr, nullptr};
 };
 
 void adjoint_release(ec_ctx* ctx)
@@ -90,6 +93,10 @@ void adjoint_release(ec_ctx* ctx)
   if (w->states) (void)hipFree(w->states);
   if (w->lam) (void)hipFree(w->lam);
   if (w->dtlog) (void)hipFree(w->dtlog);
+  for (hipEvent_t e : {w->ev_start, w->ev_rec[0], w->ev_rec[1], w->ev_use[0], w->ev_use[1]}) {
+    if (e) (void)hipEventDestroy(e);
+  }
+  if (w->rstream) (void)hipStreamDestroy(w->rstream);
   delete w;
   ctx->adjoint = nullptr;
 }
@@ -813,6 +820,8 @@ extern "C" ec_status ec_adjoint_step(ec_ctx* ctx, const double* U_n, const doubl
 namespace ec {
 ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, long long rs,
                                long long cs, double dt, int device_dt);
+ec_status launch_explicit_step_on(ec_ctx* ctx, hipStream_t stream, const double* uin, double* uout,
+                                  long long rs, long long cs, double dt, int device_dt);
 ec_status launch_state_reduce(ec_ctx* ctx, const double* u, long long rs, long long cs, int mode);
 
 // Copy a state between layouts (4 planes of nx x ny).
@@ -929,30 +938,57 @@ extern "C" ec_status ec_adjoint_checkpointed(ec_ctx* ctx, const double* checkpoi
   ec_status st = adjoint_workspace(ctx, &w);
   if (st != EC_OK) return st;
   const int K = checkpoint_every;
-  if ((st = ensure_states(ctx, w, K < 2 ? 2 : K)) != EC_OK) return st;
+  // Two segment buffers: segment seg - 1 is recomputed from its checkpoint on
+  // a second stream while the adjoint steps of segment seg run on the
+  // context's stream (the grids of one step fill only part of the chip).
+  if ((st = ensure_states(ctx, w, 2 * (K < 2 ? 2 : K))) != EC_OK) return st;
   if ((st = ensure_lam(w)) != EC_OK) return st;
+  if (!w->rstream) {
+    EC_HIP_TRY(ctx, hipStreamCreateWithFlags(&w->rstream, hipStreamNonBlocking));
+    EC_HIP_TRY(ctx, hipEventCreateWithFlags(&w->ev_start, hipEventDisableTiming));
+    for (int b = 0; b < 2; ++b) {
+      EC_HIP_TRY(ctx, hipEventCreateWithFlags(&w->ev_rec[b], hipEventDisableTiming));
+      EC_HIP_TRY(ctx, hipEventCreateWithFlags(&w->ev_use[b], hipEventDisableTiming));
+    }
+  }
   const long long nx = ctx->grid.nx, n = nx * ctx->grid.ny;
+  const int slots = K < 2 ? 2 : K;
   double* la = w->lam;
   double* lb = w->lam + 4 * n;
   if ((st = copy_state(ctx, la, nx, n, lambda_N, lay->row_stride, lay->comp_stride)) != EC_OK) return st;
+  // the recompute stream starts after the work already queued on the context's stream
+  EC_HIP_TRY(ctx, hipEventRecord(w->ev_start, ctx->stream));
+  EC_HIP_TRY(ctx, hipStreamWaitEvent(w->rstream, w->ev_start, 0));
   const int nseg = (nsteps + K - 1) / K;
+  bool used[2] = {false, false};  // ev_use[b] recorded (buffer b read by an adjoint segment)
+  auto seg_states = [&](int seg) { return w->states + (size_t)(seg & 1) * slots * 4 * n; };
+  auto recompute = [&](int seg) -> ec_status {
+    const int b = seg & 1, start = seg * K;
+    const int len = nsteps - start < K ? nsteps - start : K;
+    double* S = seg_states(seg);
+    if (used[b]) EC_HIP_TRY(ctx, hipStreamWaitEvent(w->rstream, w->ev_use[b], 0));
+    EC_HIP_TRY(ctx, hipMemcpyAsync(S, checkpoints + (size_t)seg * 4 * n, 4 * n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, w->rstream));
+    for (int s = 1; s < len; ++s) {
+      const ec_status e = launch_explicit_step_on(ctx, w->rstream, S + (size_t)(s - 1) * 4 * n,
+                                                  S + (size_t)s * 4 * n, nx, n, dts[start + s - 1], 2);
+      if (e != EC_OK) return e;
+    }
+    EC_HIP_TRY(ctx, hipEventRecord(w->ev_rec[b], w->rstream));
+    return EC_OK;
+  };
   AdjArgs prev{};
   bool pending = false;  // a step whose fold has not been launched yet
+  if ((st = recompute(nseg - 1)) != EC_OK) return st;
   for (int seg = nseg - 1; seg >= 0; --seg) {
-    const int start = seg * K;
+    const int b = seg & 1, start = seg * K;
     const int len = nsteps - start < K ? nsteps - start : K;
-    // recompute the forward states of the segment from its checkpoint
-    EC_HIP_TRY(ctx, hipMemcpyAsync(w->states, checkpoints + (size_t)seg * 4 * n, 4 * n * sizeof(double),
-                                   hipMemcpyDeviceToDevice, ctx->stream));
-    for (int s = 1; s < len; ++s) {
-      st = launch_explicit_step(ctx, w->states + (size_t)(s - 1) * 4 * n, w->states + (size_t)s * 4 * n,
-                                nx, n, dts[start + s - 1], 2);
-      if (st != EC_OK) return st;
-    }
+    if (seg > 0 && (st = recompute(seg - 1)) != EC_OK) return st;
+    EC_HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, w->ev_rec[b], 0));
+    double* S = seg_states(seg);
     for (int s = len - 1; s >= 0; --s) {
       AdjArgs A;
-      if ((st = adjoint_args(ctx, w->states + (size_t)s * 4 * n, la, lb, nx, n, dts[start + s], &A)) !=
-          EC_OK) {
+      if ((st = adjoint_args(ctx, S + (size_t)s * 4 * n, la, lb, nx, n, dts[start + s], &A)) != EC_OK) {
         return st;
       }
       const long long G = grown_count(A.g);
@@ -971,6 +1007,10 @@ extern "C" ec_status ec_adjoint_checkpointed(ec_ctx* ctx, const double* checkpoi
       la = lb;
       lb = t;
     }
+    // the buffer is free once this segment's last reads (its step 0 prep,
+    // already enqueued) are done; the pending fold reads lambda only
+    EC_HIP_TRY(ctx, hipEventRecord(w->ev_use[b], ctx->stream));
+    used[b] = true;
   }
   if (pending) {
     hipLaunchKernelGGL(adj_fold_kernel, dim3(adj_blocks(ring_count(prev.g))), dim3(256), 0, ctx->stream,

@@ -155,8 +155,8 @@ int step_partials_needed(const ec_ctx* ctx, int tile_id)
 
 // device_dt: 0 host dt, 1 device-loop dt, 2 host dt without the diagnostics
 // epilogue (state only; the run control is left untouched).
-ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, long long rs,
-                               long long cs, double dt, int device_dt)
+ec_status launch_explicit_step_on(ec_ctx* ctx, hipStream_t stream, const double* uin, double* uout,
+                                  long long rs, long long cs, double dt, int device_dt)
 {
   const TileShape s = kShapes[ctx->tile_id];
   StepArgs a;
@@ -178,20 +178,26 @@ ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, lon
     ctx->last_error = "partials buffer too small for the tile shape";
     return EC_ERR_INVALID_ARGUMENT;
   }
-  const bool timed = ctx->timing && ctx->tused + 2 <= (int)ctx->tev.size();
-  if (timed) EC_HIP_TRY(ctx, hipEventRecord(ctx->tev[ctx->tused], ctx->stream));
-  step_fn(ctx->tile_id, ctx->grid.riemann, ctx->grid.order)(a, dim3(a.nblocks), dim3(s.nt),
-                                                             ctx->stream);
+  const bool timed =
+      ctx->timing && stream == ctx->stream && ctx->tused + 2 <= (int)ctx->tev.size();
+  if (timed) EC_HIP_TRY(ctx, hipEventRecord(ctx->tev[ctx->tused], stream));
+  step_fn(ctx->tile_id, ctx->grid.riemann, ctx->grid.order)(a, dim3(a.nblocks), dim3(s.nt), stream);
   if (timed) {
-    EC_HIP_TRY(ctx, hipEventRecord(ctx->tev[ctx->tused + 1], ctx->stream));
+    EC_HIP_TRY(ctx, hipEventRecord(ctx->tev[ctx->tused + 1], stream));
     ctx->tused += 2;
   }
   if (a.diag) {
-    hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0,
-                       ctx->stream, ctx->partials, a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
+    hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0, stream,
+                       ctx->partials, a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
   }
   EC_HIP_TRY(ctx, hipGetLastError());
   return EC_OK;
+}
+
+ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, long long rs,
+                               long long cs, double dt, int device_dt)
+{
+  return launch_explicit_step_on(ctx, ctx->stream, uin, uout, rs, cs, dt, device_dt);
 }
 
 int reduce_blocks(const ec_ctx* ctx)
