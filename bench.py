@@ -178,7 +178,7 @@ def bench_imex(dev, n: int = 512, mach: float = 0.001, reps: int = 3) -> dict:
 
 
 def bench_adjoint(dev, nx: int = 640, ny: int = 160, nsteps: int = 768, K: int = 32,
-                  ndirs: int = 16) -> dict:
+                  ndirs: int = 16, fd_eps: float = 1e-7) -> dict:
     """Checkpointed adjoint of the explicit O2 HLLC run: single-gamma shock
     (Mach 1.22) into a light bubble on [0,4]x[0,1]; gradient of the bubble
     kinetic energy after nsteps w.r.t. the initial density; cost ratio of the
@@ -267,7 +267,7 @@ def bench_adjoint(dev, nx: int = 640, ny: int = 160, nsteps: int = 768, K: int =
     (grad,) = torch.autograd.grad(J, rho_req)
     rng = np.random.default_rng(20261203)
     errs = []
-    eps = 1e-6
+    eps = fd_eps
     for _ in range(ndirs):
         ph = rng.uniform(0, 2 * np.pi, 2)
         kx, ky = rng.integers(2, 7, 2)
