@@ -15,10 +15,12 @@
 //              which is the reference row (:533-543) re-associated so that a
 //              constant pressure cancels exactly (it removes the FP64
 //              cancellation floor of the assembled row at low Mach)
-//   GMRES(m)   restarted, CGS2 orthogonalisation with fused multi-dots,
-//              Givens least squares and early exit on device (one host
-//              synchronisation per restart cycle), FP64 or FP32 inner
-//              iterations with FP64 residual refinement
+//   GMRES(m)   FP64 form: restarted, CGS2 orthogonalisation with fused
+//              multi-dots, Givens least squares and early exit on device (one
+//              host synchronisation per restart cycle).  Mixed form: FP64
+//              flexible outer GMRES whose preconditioner is an FP32 inner
+//              GMRES(m) with one reduction per Arnoldi step (FGMRES-based
+//              iterative refinement, one synchronisation per outer cycle)
 //   refresh    m = m* - dt grad_c p (imex_source_map_helpers.hpp:249-268)
 //   closeout   E = E* - dt div(h m) and the new state
 //              (imex_source_map_step.hpp:75-276)
@@ -822,6 +824,46 @@ __global__ void __launch_bounds__(RB) norm_givens_kernel(const double* npart, in
   if (threadIdx.x == 0) (void)givens_step(v, col, c, sv, R, g, cs, sn, j, m, ctl, true);
 }
 
+// y = R^{-1} g for the kk x kk upper triangle, by the 64 lanes of one
+// wavefront: column-oriented back substitution, one lane per row, the
+// columns staged in LDS (Rs, MAXK x MAXK) and each solved entry broadcast
+// from its lane; y_q times scale[q] when a scale is given (the one-reduction
+// cycle's unnormalised basis).  g holds the rotated right-hand side at
+// g[m + 1 + q].
+__device__ void backsolve_wave(const double* R, const double* g, double* y, int m, int kk, GmCtl* ctl,
+                               const double* scale, double* Rs)
+{
+  const int r = threadIdx.x & 63;
+  const int rc = r < m ? r : m;  // clamped row: every load is in bounds
+  for (int q0 = 0; q0 < kk; q0 += 8) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = q0 + u < kk ? q0 + u : kk - 1;
+      t[u] = R[(long long)q * (m + 1) + rc];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (q0 + u < kk) Rs[(q0 + u) * MAXK + r] = r <= q0 + u ? t[u] : 0.0;
+  }
+  double s = g[m + 1 + rc];
+  s = r < kk ? s : 0.0;
+  __builtin_amdgcn_wave_barrier();
+  double yr = 0.0;
+  for (int i = kk - 1; i >= 0; --i) {
+    const long long sb = __double_as_longlong(s);
+    const int lo = __builtin_amdgcn_readlane((int)sb, i);
+    const int hi = __builtin_amdgcn_readlane((int)(sb >> 32), i);
+    const double yi = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo) / Rs[i * MAXK + i];
+    if (r == i) yr = yi;
+    if (r < i) s -= Rs[i * MAXK + r] * yi;
+  }
+  if (r < kk) {
+    if (!isfinite(yr)) ctl->bad = 1;
+    y[r] = scale ? yr * scale[r] : yr;
+  }
+}
+
 // ------------------------------------------- one-reduction GMRES cycle (FP32)
 // Classical Gram-Schmidt with lagged normalisation: one global reduction per
 // Arnoldi step (the one-synchronisation CGS of Swirydowicz, Langou,
@@ -841,7 +883,7 @@ __global__ void __launch_bounds__(RB) norm_givens_kernel(const double* npart, in
 //   3. reduces u_q . z_k (q <= k) and u_k . u_k per tile.
 // The mathematics is that of CGS1 (the FP32 inner solver's
 // orthogonalisation), in one kernel per Arnoldi step instead of three.
-constexpr int OTX = 64, OTY = 16, ONT = OTX * OTY;  // one cell per thread
+constexpr int OTX = 128, OTY = 8, ONT = OTX * OTY;  // one cell per thread
 constexpr int OHX = OTX + 4, OHY = OTY + 4, ONR = OHX * OHY;
 constexpr int KP = MAXK + 2;  // partial entries per tile: MAXK + 1 dots and one norm
 
@@ -1110,25 +1152,40 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
 }
 
 // Closes a one-reduction cycle: |u_m| from the last kernel's partials and
-// column m-1's Givens step.
+// column m-1's Givens step (unless the cycle ended earlier), then wavefront 0
+// solves for the cycle's coefficients, y_q / |u_q|.
 __global__ void __launch_bounds__(ONT) onesync_close_kernel(const double* partials, int nparts, int m,
                                                             GmCtl* ctl, const double* hraw, double* R,
-                                                            double* g, double* cs, double* sn)
+                                                            double* g, double* cs, double* sn,
+                                                            double* y, const double* inv_nu)
 {
   __shared__ double tot[KP];
   __shared__ double col[MAXK + 1], gc[MAXK], gsv[MAXK];
-  if (ctl->done) return;  // no other workgroup writes ctl during this kernel
+  __shared__ double Rs[MAXK * MAXK];
+  __shared__ int s_done, s_kk;
   const int tid = threadIdx.x;
-  sum_tile_entries(partials, nparts, m + 1, m + 2, tot);
-  for (int i = tid; i <= m - 1; i += ONT) {
-    col[i] = hraw[(long long)(m - 1) * (m + 1) + i];
-    if (i < m - 1) {
-      gc[i] = cs[i];
-      gsv[i] = sn[i];
-    }
+  if (tid == 0) {
+    s_done = ctl->done;
+    s_kk = ctl->kk;
   }
   __syncthreads();
-  if (tid == 0) (void)givens_step(tot[m + 1], col, gc, gsv, R, g, cs, sn, m - 1, m, ctl, true);
+  if (!s_done) {
+    sum_tile_entries(partials, nparts, m + 1, m + 2, tot);
+    for (int i = tid; i <= m - 1; i += ONT) {
+      col[i] = hraw[(long long)(m - 1) * (m + 1) + i];
+      if (i < m - 1) {
+        gc[i] = cs[i];
+        gsv[i] = sn[i];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      (void)givens_step(tot[m + 1], col, gc, gsv, R, g, cs, sn, m - 1, m, ctl, true);
+      s_kk = m;
+    }
+  }
+  __syncthreads();  // also orders thread 0's R and g writes before wavefront 0's reads
+  if (tid < 64) backsolve_wave(R, g, y, m, s_kk, ctl, inv_nu, Rs);
 }
 
 // ------------------------------------------------ mixed form: FGMRES-IR
@@ -1142,18 +1199,26 @@ __global__ void __launch_bounds__(ONT) onesync_close_kernel(const double* partia
 // accuracy is FP64's while the Krylov work is FP32.
 constexpr int MO = 20;  // outer restart length
 
-// A fresh inner cycle on a unit right-hand side, or none once the outer
-// cycle is done.
-__global__ void inner_reset_kernel(GmCtl* ci, const GmCtl* co, int mlim)
+// A fresh inner cycle on the unit right-hand side v_j: u_0 = v_j in FP32,
+// g_0 = 1 and the inner control block reset (done at once when the outer
+// cycle is done).
+__global__ void __launch_bounds__(RB) inner_start_kernel(const double* __restrict__ v,
+                                                         float* __restrict__ U, long long n, GmCtl* ci,
+                                                         const GmCtl* co, double* g, int mlim)
 {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  ci->done = co->done;
-  ci->kk = 0;
-  ci->bad = 0;
-  ci->breakdown = 0;
-  ci->mlim = mlim;
-  ci->beta = 1.0;
-  ci->thr = co->inner_thr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ci->done = co->done;
+    ci->kk = 0;
+    ci->bad = 0;
+    ci->breakdown = 0;
+    ci->mlim = mlim;
+    ci->beta = 1.0;
+    ci->thr = co->inner_thr;
+    g[0] = 1.0;
+  }
+  for (long long t = (long long)blockIdx.x * RB + threadIdx.x; t < n; t += (long long)gridDim.x * RB) {
+    U[t] = (float)v[t];
+  }
 }
 
 // Outer step j: y = A z_j (FP64) into w and the first Gram-Schmidt pass's
@@ -1266,29 +1331,12 @@ __global__ void __launch_bounds__(256) cycle_start_kernel(const double* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) g[0] = beta;
 }
 
-// y = R^{-1} g (kk x kk upper triangular; column-oriented back substitution,
-// one lane per row with the columns staged in LDS), each y_q times scale[q]
-// when a scale is given (the one-reduction cycle's unnormalised basis); then
-// x += V y (x in FP64; V in T).
+// y = R^{-1} g (backsolve_wave), then x += V y (x in FP64; V in T).
 __global__ void __launch_bounds__(64) backsolve_kernel(const double* R, const double* g, double* y,
                                                        int m, GmCtl* ctl, const double* scale)
 {
   __shared__ double Rs[MAXK * MAXK];
-  const int kk = ctl->kk;
-  const int r = threadIdx.x;
-  for (int q = 0; q < kk; ++q) Rs[q * MAXK + r] = r <= q ? R[(long long)q * (m + 1) + r] : 0.0;
-  double s = r < kk ? g[m + 1 + r] : 0.0;  // rotated right-hand side
-  __syncthreads();
-  double yr = 0.0;
-  for (int i = kk - 1; i >= 0; --i) {
-    const double yi = __shfl(s, i) / Rs[i * MAXK + i];
-    if (r == i) yr = yi;
-    if (r < i) s -= Rs[i * MAXK + r] * yi;
-  }
-  if (r < kk) {
-    if (!isfinite(yr)) ctl->bad = 1;
-    y[r] = scale ? yr * scale[r] : yr;
-  }
+  backsolve_wave(R, g, y, m, ctl->kk, ctl, scale, Rs);
 }
 
 template <typename T, typename X>
@@ -1554,9 +1602,8 @@ static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
   const int m = w->m;
   double* vj = w->Vo + (long long)j * n;
   double* zj = w->Zo + (long long)j * n;
-  hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(64), 0, st, w->ctl, w->ctlo, m);
-  hipLaunchKernelGGL((cycle_start_kernel<float>), dim3(nb), dim3(RB), 0, st, vj, w->V32, n, w->ctl,
-                     w->g);
+  hipLaunchKernelGGL(inner_start_kernel, dim3(nb), dim3(RB), 0, st, vj, w->V32, n, w->ctl, w->ctlo,
+                     w->g, m);
   for (int k = 0; k <= m; ++k) {
     // the smallest instantiation that holds the k basis values in registers
     auto kern = k <= 8    ? onesync_kernel<float, 8>
@@ -1567,9 +1614,7 @@ static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
                        w->otiles, w->ctl, w->hraw, w->inv_nu, w->R, w->g, w->cs, w->sn);
   }
   hipLaunchKernelGGL(onesync_close_kernel, dim3(1), dim3(ONT), 0, st, w->opart, w->otiles, m, w->ctl,
-                     w->hraw, w->R, w->g, w->cs, w->sn);
-  hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, st, w->R, w->g, w->y, m, w->ctl,
-                     (const double*)w->inv_nu);
+                     w->hraw, w->R, w->g, w->cs, w->sn, w->y, (const double*)w->inv_nu);
   hipLaunchKernelGGL((xupdate_kernel<float, double>), dim3(nb), dim3(RB), 0, st, w->V32, n, w->y, zj,
                      1.0, w->ctl, 1);
   hipLaunchKernelGGL(outer_arnoldi_kernel, dim3(nb), dim3(RB), 0, st, w->dA64, w->Vo, zj, j, w->w,
