@@ -96,7 +96,8 @@ struct ImexWork {
   GmCtl* ctlo_host = nullptr;
   std::vector<hipGraphExec_t> exec_outer;  // MO step graphs, then the closing graph
   // one-reduction cycle (FP32 inner solver)
-  double* opart = nullptr;   // tile partials, otiles x KP
+  double* opart = nullptr;   // tile partials, 2 x otiles x KP (ping-pong by Arnoldi step)
+  double* otot = nullptr;    // their totals (KP) on grids of many tiles
   double* hraw = nullptr;    // unrotated Hessenberg columns, (m+1) x m
   double* inv_nu = nullptr;  // 1 / |u_q|, m+1
   int otiles = 0;
@@ -120,7 +121,7 @@ void imex_release(ec_ctx* ctx)
   void* ptrs[] = {w->star, w->p,   w->x,    w->mom, w->hx,       w->hy,      w->fx, w->fy,
                   w->hx32, w->hy32, w->b,   w->r,   w->V,        w->V32,     w->w,  w->w32,
                   w->R,    w->g,    w->cs,   w->sn,  w->y,        w->partials, w->ctl,
-                  w->npart, w->opart,  w->hraw,  w->inv_nu,
+                  w->npart, w->opart,  w->otot,  w->hraw,   w->inv_nu,
                   w->Vo,   w->Zo,   w->Ro,   w->go,  w->cso,      w->sno,     w->yo, w->ctlo,
                   w->dA64, w->dA32};
   for (void* p : ptrs) {
@@ -886,6 +887,7 @@ __device__ void backsolve_wave(const double* R, const double* g, double* y, int 
 constexpr int OTX = 128, OTY = 8, ONT = OTX * OTY;  // one cell per thread
 constexpr int OHX = OTX + 4, OHY = OTY + 4, ONR = OHX * OHY;
 constexpr int KP = MAXK + 2;  // partial entries per tile: MAXK + 1 dots and one norm
+constexpr int ONE_SUM_TILES = 256;  // tiles up to which each workgroup sums the partials itself
 
 static inline int onesync_tiles(int nx, int ny)
 {
@@ -947,10 +949,12 @@ __device__ __forceinline__ T form_value(const T (&uv)[QB], T z, const T* __restr
   return (T)sum;
 }
 
-template <typename T, int QB>
+template <typename T, int QB, bool SUMMED>
 __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restrict__ Ap, T* __restrict__ U,
-                                                      T* __restrict__ Z, int k, int m, double* partials,
-                                                      int nparts, GmCtl* ctl, double* hraw,
+                                                      T* __restrict__ Z, int k, int m,
+                                                      const double* __restrict__ partials,
+                                                      double* __restrict__ pout, int nparts,
+                                                      GmCtl* ctl, double* hraw,
                                                       double* inv_nu, double* R, double* g, double* cs,
                                                       double* sn)
 {
@@ -1013,14 +1017,20 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
   // selected away), so that none waits for another.  The previous kernel's
   // partials go first: loads complete in order, so stage 0 can run while the
   // basis loads behind them are still in flight.
-  double pv[2][4];
+  // SUMMED: partials holds the previous kernel's k + 1 totals (a separate
+  // fixed-order sum, for grids of many tiles); otherwise one partial per tile.
+  double pv[2][4], tv = 0.0;
+  if constexpr (SUMMED) {
+    tv = partials[tid < KP ? tid : 0];
+  } else {
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int e = wave + 16 * r;
+    for (int r = 0; r < 2; ++r) {
+      const int e = wave + 16 * r;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int b = lane + 64 * i;
-      pv[r][i] = partials[(long long)(b < nparts ? b : 0) * KP + e];
+      for (int i = 0; i < 4; ++i) {
+        const int b = lane + 64 * i;
+        pv[r][i] = partials[(long long)(b < nparts ? b : 0) * KP + e];
+      }
     }
   }
   // Givens inputs of column k-2 and the basis scales, ahead of the basis
@@ -1049,19 +1059,23 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
   // one lane while the others form u_k
   double cz = 1.0;
   if (k > 0) {
+    if constexpr (SUMMED) {
+      if (tid <= k) tot[tid] = tv;
+    } else {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int e = wave + 16 * r;
-      double v = 0.0;
+      for (int r = 0; r < 2; ++r) {
+        const int e = wave + 16 * r;
+        double v = 0.0;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) v += lane + 64 * i < nparts ? pv[r][i] : 0.0;
-      if (nparts > 256) {  // more tiles than the held partials cover
-        for (int b = lane + 256; b < nparts; b += 64) v += partials[(long long)b * KP + e];
+        for (int i = 0; i < 4; ++i) v += lane + 64 * i < nparts ? pv[r][i] : 0.0;
+        if (nparts > 256) {  // more tiles than the held partials cover
+          for (int b = lane + 256; b < nparts; b += 64) v += partials[(long long)b * KP + e];
+        }
+        v = wave_sum_lane63(v);
+        if (lane == 63 && e <= k) tot[e] = v;
       }
-      v = wave_sum_lane63(v);
-      if (lane == 63 && e <= k) tot[e] = v;
+      if (k >= 32) sum_tile_entries(partials, nparts, 32, k + 1, tot);
     }
-    if (k >= 32) sum_tile_entries(partials, nparts, 32, k + 1, tot);
     if (k >= 2 && tid <= k - 2) {
       col[tid] = gcol;
       if (tid < k - 2) {
@@ -1146,7 +1160,7 @@ __global__ void __launch_bounds__(ONT) onesync_kernel(const OpArgs<T>* __restric
   if (tid < ne) {
     double v = sh[0][tid];
     for (int w = 1; w < ONT / 64; ++w) v += sh[w][tid];
-    partials[(long long)blockIdx.x * KP + tid] = v;
+    pout[(long long)blockIdx.x * KP + tid] = v;
   }
   if (k == 0 && blockIdx.x == 0 && tid == 0) inv_nu[0] = 1.0;  // u_0 = r / beta
 }
@@ -1504,7 +1518,8 @@ static ec_status imex_workspace(ec_ctx* ctx, int m)
   A((void**)&w->partials, (size_t)w->nparts * MAXK * sizeof(double));
   A((void**)&w->npart, (size_t)w->nparts * sizeof(double));
   w->otiles = onesync_tiles(w->nx, w->ny);
-  A((void**)&w->opart, (size_t)w->otiles * KP * sizeof(double));
+  A((void**)&w->opart, (size_t)2 * w->otiles * KP * sizeof(double));  // two halves
+  A((void**)&w->otot, (size_t)KP * sizeof(double));
   A((void**)&w->hraw, (size_t)(m + 1) * m * sizeof(double));
   A((void**)&w->inv_nu, (size_t)(m + 1) * sizeof(double));
   A((void**)&w->ctl, sizeof(GmCtl));
@@ -1605,15 +1620,33 @@ static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
   hipLaunchKernelGGL(inner_start_kernel, dim3(nb), dim3(RB), 0, st, vj, w->V32, n, w->ctl, w->ctlo,
                      w->g, m);
   for (int k = 0; k <= m; ++k) {
-    // the smallest instantiation that holds the k basis values in registers
-    auto kern = k <= 8    ? onesync_kernel<float, 8>
-                : k <= 16 ? onesync_kernel<float, 16>
-                : k <= 24 ? onesync_kernel<float, 24>
-                          : onesync_kernel<float, 32>;
-    hipLaunchKernelGGL(kern, dim3(w->otiles), dim3(ONT), 0, st, w->dA32, w->V32, w->w32, k, m, w->opart,
+    // The smallest instantiation that holds the k basis values in registers.
+    // Up to ONE_SUM_TILES tiles every workgroup sums the previous kernel's
+    // tile partials itself; beyond, that redundant sum grows as tiles^2, and
+    // a fixed-order finalize after each kernel sums them once instead.
+    const bool summed = w->otiles > ONE_SUM_TILES;
+    auto kern = summed ? (k <= 8    ? onesync_kernel<float, 8, true>
+                          : k <= 16 ? onesync_kernel<float, 16, true>
+                          : k <= 24 ? onesync_kernel<float, 24, true>
+                                    : onesync_kernel<float, 32, true>)
+                       : (k <= 8    ? onesync_kernel<float, 8, false>
+                          : k <= 16 ? onesync_kernel<float, 16, false>
+                          : k <= 24 ? onesync_kernel<float, 24, false>
+                                    : onesync_kernel<float, 32, false>);
+    // the partials ping-pong like z: a workgroup that finishes early must not
+    // overwrite what a later workgroup of the same launch still has to read
+    const size_t ps = (size_t)w->otiles * KP;
+    double* pout = w->opart + (k & 1) * ps;
+    const double* pin = summed ? w->otot : w->opart + ((k + 1) & 1) * ps;
+    hipLaunchKernelGGL(kern, dim3(w->otiles), dim3(ONT), 0, st, w->dA32, w->V32, w->w32, k, m, pin, pout,
                        w->otiles, w->ctl, w->hraw, w->inv_nu, w->R, w->g, w->cs, w->sn);
+    if (summed && k < m) {
+      hipLaunchKernelGGL(finalize_dots_kernel, dim3(k + 2), dim3(RB), 0, st, (const double*)pout,
+                         w->otiles, KP, w->otot, (const GmCtl*)w->ctl);
+    }
   }
-  hipLaunchKernelGGL(onesync_close_kernel, dim3(1), dim3(ONT), 0, st, w->opart, w->otiles, m, w->ctl,
+  hipLaunchKernelGGL(onesync_close_kernel, dim3(1), dim3(ONT), 0, st,
+                     w->opart + (m & 1) * (size_t)w->otiles * KP, w->otiles, m, w->ctl,
                      w->hraw, w->R, w->g, w->cs, w->sn, w->y, (const double*)w->inv_nu);
   hipLaunchKernelGGL((xupdate_kernel<float, double>), dim3(nb), dim3(RB), 0, st, w->V32, n, w->y, zj,
                      1.0, w->ctl, 1);

@@ -113,6 +113,30 @@ def test_mixed_long_inner_cycles_match_fp64(ecor, ecamd, gpu):
     assert err[0] == 0.0 and np.all(err <= 1e-7), err
 
 
+def test_mixed_many_tiles_matches_fp64(ecor, ecamd, gpu):
+    """A grid of 1024 x 512 cells is 512 tiles of the one-reduction kernel,
+    more workgroups than the chip holds at once, so workgroups of one launch
+    run one after another: each launch must read the previous launch's tile
+    partials, never its own (the partials ping-pong between launches).  Gresho
+    vortex at M = 0.1, one IMEX step: the mixed form must converge and agree
+    with the FP64 form."""
+    nx, ny = 1024, 512
+    U = ecor.init_gresho(nx, ny, mach=0.1)
+    g = ecor.make_grid(nx, ny, 1 / nx, 1 / ny)
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    s = ecamd.Solver(ecamd.Grid(nx, ny, 1 / nx, 1 / ny, 1.4, "hllc", "minmod", 2), gpu)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        Ud = torch.from_numpy(U).to(gpu)
+        d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-10, max_iter=3000, precision=prec))
+        print(prec, ecamd.IMEX_STATUS[d.status], d.iterations, d.refinements,
+              d.relative_residual_linf)
+        assert ecamd.IMEX_STATUS[d.status] in ("converged_absolute", "converged_relative"), d.status
+        out[prec] = Ud.cpu().numpy()
+    err = rel_l1(out["mixed"], out["fp64"])
+    assert err[0] == 0.0 and np.all(err <= 1e-8), err
+
+
 def test_uniform_flow_is_exact_on_device(ecor, ecamd, gpu):
     n = 32
     q = np.array([1.3, 0.2, -0.1, 5.0])
