@@ -136,7 +136,7 @@ def _events():
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
-def bench_imex(dev, n: int = 512, mach: float = 0.001, reps: int = 3) -> dict:
+def bench_imex(dev, n: int = 512, mach: float = 0.001, reps: int = 3, max_iter: int = 2000) -> dict:
     """IMEX T1/S2 BDLTV20 source-map Picard step, Gresho n x n periodic at
     Mach `mach`: device GMRES(30) to 1e-10 relative residual in FP64 and in
     mixed form (FP32 inner Krylov, FP64 residual refinement), same dt, same
@@ -152,7 +152,7 @@ def bench_imex(dev, n: int = 512, mach: float = 0.001, reps: int = 3) -> dict:
            "dt": dt, "tol": 1e-10, "restart": 30}
     results = {}
     for prec in ("fp64", "mixed"):
-        opts = s.imex_opts(precision=prec, tol=1e-10, max_iter=2000, restart=30)
+        opts = s.imex_opts(precision=prec, tol=1e-10, max_iter=max_iter, restart=30)
         U = U0.clone()
         s.imex_step(U, dt, opts)            # warm-up
         times = []

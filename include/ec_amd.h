@@ -186,19 +186,21 @@ typedef enum { EC_IMEX_PRECISION_FP64 = 0, EC_IMEX_PRECISION_MIXED = 1 } ec_imex
 typedef struct {
   int picard_iterations;     /* euler.imex_picard_iterations */
   double tol;                /* euler.imex_solver_tol (absolute or relative-to-max(|b|inf,1)) */
-  int max_iter;              /* euler.imex_solver_max_iter: Krylov iterations per solve */
-  int restart;               /* GMRES restart length, 1..64 */
-  int precision;             /* ec_imex_precision */
+  int max_iter;              /* euler.imex_solver_max_iter: Krylov iterations per solve (mixed:
+                                FP32 inner iterations, checked at outer restarts) */
+  int restart;               /* GMRES restart length, 1..64 (mixed: of the FP32 inner cycles) */
+  int precision;             /* ec_imex_precision: FP64 GMRES, or MIXED: FP64 flexible outer
+                                GMRES(20) preconditioned by FP32 GMRES(restart) */
   int predictor_dissipation; /* euler.imex_predictor_dissipation: 0 material, 1 acoustic */
-  double inner_reduction;    /* mixed: 2-norm reduction asked of each FP32 inner cycle */
+  double inner_reduction;    /* mixed: relative 2-norm reduction asked of each FP32 inner solve */
 } ec_imex_opts;
 
 typedef struct {
   int status;                /* ec_imex_status */
-  int iterations;            /* Krylov iterations summed over the Picard solves */
+  int iterations;            /* Krylov iterations summed over the Picard solves (mixed: FP32) */
   int last_iterations;
   int picard_solves;
-  int refinements;           /* mixed: FP64 refinement steps of the last solve */
+  int refinements;           /* mixed: FP64 outer steps (one FP32 inner solve each), last solve */
   int density_floor_count;
   int guard_count;           /* face-enthalpy denominator guards in the closeout */
   int nonfinite_count;
