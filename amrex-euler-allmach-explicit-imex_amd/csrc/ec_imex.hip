@@ -46,6 +46,7 @@ struct GmCtl {
   int done, kk, breakdown, bad;
   int mlim;                    // iterations allowed in this cycle
   int inner_its;               // flexible outer solver: FP32 iterations of this cycle
+  int inner_mlim, pad1;        // flexible outer solver: the inner restart length
   double thr, est, beta, inv_norm;
   double rinf, r2;             // true residual norms
   double inner_thr;            // flexible outer solver: inner relative tolerance
@@ -1218,14 +1219,14 @@ constexpr int MO = 20;  // outer restart length
 // cycle is done).
 __global__ void __launch_bounds__(RB) inner_start_kernel(const double* __restrict__ v,
                                                          float* __restrict__ U, long long n, GmCtl* ci,
-                                                         const GmCtl* co, double* g, int mlim)
+                                                         const GmCtl* co, double* g)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ci->done = co->done;
     ci->kk = 0;
     ci->bad = 0;
     ci->breakdown = 0;
-    ci->mlim = mlim;
+    ci->mlim = co->inner_mlim;  // the requested restart (the graph holds the workspace's m)
     ci->beta = 1.0;
     ci->thr = co->inner_thr;
     g[0] = 1.0;
@@ -1618,7 +1619,7 @@ static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
   double* vj = w->Vo + (long long)j * n;
   double* zj = w->Zo + (long long)j * n;
   hipLaunchKernelGGL(inner_start_kernel, dim3(nb), dim3(RB), 0, st, vj, w->V32, n, w->ctl, w->ctlo,
-                     w->g, m);
+                     w->g);
   for (int k = 0; k <= m; ++k) {
     // The smallest instantiation that holds the k basis values in registers.
     // Up to ONE_SUM_TILES tiles every workgroup sums the previous kernel's
@@ -1788,6 +1789,7 @@ static ec_status solve_mixed(ec_ctx* ctx, ImexWork* w, double target, double rhs
     h.beta = beta;
     h.thr = target * (beta / rinf);  // as in the FP64 form
     h.inner_thr = o.inner_reduction;
+    h.inner_mlim = std::min(o.restart, w->m);
     EC_HIP_TRY(ctx, hipMemcpyAsync(w->ctlo, w->ctlo_host, offsetof(GmCtl, dots1),
                                    hipMemcpyHostToDevice, w->gs));
     hipLaunchKernelGGL((cycle_start_kernel<double>), dim3(grid_blocks(n)), dim3(RB), 0, w->gs, w->r,

@@ -137,6 +137,34 @@ def test_mixed_many_tiles_matches_fp64(ecor, ecamd, gpu):
     assert err[0] == 0.0 and np.all(err <= 1e-8), err
 
 
+@pytest.mark.parametrize("restart", [1, 2, 5])
+def test_mixed_short_inner_cycles(ecor, ecamd, gpu, restart):
+    """The shortest inner cycles (GMRES(1), GMRES(2), GMRES(5)), on a context
+    whose workspace was sized by an earlier GMRES(30) solve: the inner cycles
+    must honour the requested restart.  In GMRES(1) the one-reduction kernels'
+    last launch is also the first to see a finished column.  Gresho 48^2 at
+    M = 0.1, one Picard solve: the mixed form converges and agrees with the
+    FP64 form."""
+    n = 48
+    U = ecor.init_gresho(n, n, mach=0.1)
+    g = ecor.make_grid(n, n, 1 / n, 1 / n)
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    s = solver(ecamd, n, gpu)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        Ud = torch.from_numpy(U).to(gpu)
+        d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-10, max_iter=20000, precision=prec,
+                                            picard_iterations=1,
+                                            restart=restart if prec == "mixed" else 30))
+        print(prec, restart, ecamd.IMEX_STATUS[d.status], d.iterations, d.refinements)
+        assert ecamd.IMEX_STATUS[d.status] in ("converged_absolute", "converged_relative"), d.status
+        if prec == "mixed":  # each outer step runs at most `restart` inner iterations
+            assert 1 <= d.iterations <= restart * d.refinements, (d.iterations, d.refinements)
+        out[prec] = Ud.cpu().numpy()
+    err = rel_l1(out["mixed"], out["fp64"])
+    assert err[0] == 0.0 and np.all(err <= 1e-8), err
+
+
 def test_uniform_flow_is_exact_on_device(ecor, ecamd, gpu):
     n = 32
     q = np.array([1.3, 0.2, -0.1, 5.0])
