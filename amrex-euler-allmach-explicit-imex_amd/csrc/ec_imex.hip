@@ -874,10 +874,12 @@ __device__ void backsolve_wave(const double* R, const double* g, double* y, int 
 // kept unnormalised, u_q = nu_q v_q with nu_q = |u_q| (nu_0 = 1: u_0 = r /
 // beta), next to z = A u of the newest direction.  Kernel k:
 //   0. (k > 0) sums the previous kernel's tile partials in a fixed order, in
-//      every workgroup: nu_{k-1} = |u_{k-1}|, h_{q,k-1} = (u_q . z_{k-1}) /
-//      (nu_q nu_{k-1}), and column k-2's Givens step with its subdiagonal
-//      entry nu_{k-1} (workgroup 0 commits H, the scales and the rotation);
-//      every workgroup exits on the same early-exit decision;
+//      every workgroup (beyond ONE_SUM_TILES tiles it reads the totals of a
+//      fixed-order finalize kernel instead): nu_{k-1} = |u_{k-1}|,
+//      h_{q,k-1} = (u_q . z_{k-1}) / (nu_q nu_{k-1}), and column k-2's
+//      Givens step with its subdiagonal entry nu_{k-1} (workgroup 0 commits
+//      H, the scales and the rotation); every workgroup exits on the same
+//      early-exit decision;
 //   1. forms u_k = z_{k-1} / nu_{k-1} - sum_q h_{q,k-1} u_q / nu_q on its
 //      tile and the operator's cross-shaped 2-cell halo, in LDS (FP64
 //      accumulation, stored in T), and stores the tile's cells;
@@ -885,6 +887,9 @@ __device__ void backsolve_wave(const double* R, const double* g, double* y, int 
 //   3. reduces u_q . z_k (q <= k) and u_k . u_k per tile.
 // The mathematics is that of CGS1 (the FP32 inner solver's
 // orthogonalisation), in one kernel per Arnoldi step instead of three.
+// z and the tile partials ping-pong between consecutive launches: a launch
+// can have more workgroups than the chip holds, so workgroups of one launch
+// run one after another, and none may overwrite what a later one reads.
 constexpr int OTX = 128, OTY = 8, ONT = OTX * OTY;  // one cell per thread
 constexpr int OHX = OTX + 4, OHY = OTY + 4, ONR = OHX * OHY;
 constexpr int KP = MAXK + 2;  // partial entries per tile: MAXK + 1 dots and one norm
@@ -1566,8 +1571,9 @@ struct HostCtl {
 // Every value that changes between cycles or solves is read from device
 // memory (ctl, the operator block), so the sequence is captured once into a
 // hipGraph and replayed.  two_pass: classical Gram-Schmidt with
-// re-orthogonalisation (CGS2); otherwise one pass (CGS1, the FP32 inner
-// solver, whose loss of orthogonality the FP64 refinement absorbs).
+// re-orthogonalisation (CGS2, what the FP64 form uses); otherwise one pass
+// (CGS1).  The mixed form's FP32 inner cycles use the one-reduction kernels
+// instead (enqueue_outer_step).
 template <typename T>
 static void enqueue_cycle(ImexWork* w, hipStream_t st, const OpArgs<T>* dA, T* V, T* wv, int two_pass)
 {
