@@ -64,6 +64,8 @@ extern "C" ec_status ec_flux_vjp_host(const double* qL, const double* qR, int ax
 // Each pass writes every output exactly once: no atomics, deterministic.
 namespace ec {
 
+bool has_exact_side(const GridParams& g);
+
 struct AdjWork {
   int nx = 0, ny = 0;
   double* states = nullptr;  // segment of forward states (compact), seg_cap of them
@@ -812,6 +814,10 @@ extern "C" ec_status ec_adjoint_step(ec_ctx* ctx, const double* U_n, const doubl
       !(dt >= 0.0)) {
     return EC_ERR_INVALID_ARGUMENT;
   }
+  if (has_exact_side(ctx->grid)) {
+    ctx->last_error = "the adjoint does not take exact-Dirichlet sides";
+    return EC_ERR_UNSUPPORTED;
+  }
   EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
   return launch_adjoint_step(ctx, U_n, lambda_next, lambda_n, lay->row_stride, lay->comp_stride, dt);
 }
@@ -873,6 +879,10 @@ extern "C" ec_status ec_forward_checkpointed(ec_ctx* ctx, const double* U0, doub
       !checkpoints || !dts || !steps_done) {
     return EC_ERR_INVALID_ARGUMENT;
   }
+  if (has_exact_side(ctx->grid)) {
+    ctx->last_error = "the adjoint does not take exact-Dirichlet sides";
+    return EC_ERR_UNSUPPORTED;
+  }
   EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
   AdjWork* w = nullptr;
   ec_status st = adjoint_workspace(ctx, &w);
@@ -932,6 +942,10 @@ extern "C" ec_status ec_adjoint_checkpointed(ec_ctx* ctx, const double* checkpoi
   if (!ctx || !checkpoints || !lay_ok(ctx, lay) || nsteps < 1 || checkpoint_every < 1 || !dts ||
       !lambda_N || !lambda_0) {
     return EC_ERR_INVALID_ARGUMENT;
+  }
+  if (has_exact_side(ctx->grid)) {
+    ctx->last_error = "the adjoint does not take exact-Dirichlet sides";
+    return EC_ERR_UNSUPPORTED;
   }
   EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
   AdjWork* w = nullptr;

@@ -11,6 +11,7 @@ int step_partials_needed(const ec_ctx* ctx, int tile_id);
 int reduce_blocks(const ec_ctx* ctx);
 void imex_release(ec_ctx* ctx);
 void adjoint_release(ec_ctx* ctx);
+void exact_release(ec_ctx* ctx);
 
 GridParams make_grid_params(const ec_grid_desc& d)
 {
@@ -118,6 +119,7 @@ extern "C" ec_status ec_destroy(ec_ctx* ctx)
   else (void)hipDeviceSynchronize();
   imex_release(ctx);
   adjoint_release(ctx);
+  exact_release(ctx);
   for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
   if (ctx->ctrl) (void)hipFree(ctx->ctrl);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);

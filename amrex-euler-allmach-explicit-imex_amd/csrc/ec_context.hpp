@@ -11,7 +11,31 @@
 
 namespace ec {
 
-enum BcKind { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECT = 2, BC_FIXED = 3 };
+enum BcKind { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECT = 2, BC_FIXED = 3, BC_EXACT = 4 };
+
+// Exact-Dirichlet ghost states of one test problem
+// (euler.field_boundary=exact_dirichlet, problems/boundary_conditions.hpp:
+// 133-205).  RING: time-independent states (Gresho, Riemann quadrant)
+// evaluated once on the host into a table of the 2-cell ghost ring; TORO and
+// BLOB: time-dependent states evaluated on read from the constants below.
+enum ExactKind { EXACT_NONE = 0, EXACT_RING = 1, EXACT_TORO = 2, EXACT_BLOB = 3 };
+
+struct ExactBc {
+  int kind;
+  int nx, ny;
+  int periodic_x, periodic_y;
+  double xlo, ylo, dx, dy, gamma;  // cell centres xlo + (i + 1/2) dx
+  const double* ring;              // RING: 4 conserved values per ghost cell
+  // TORO: left/right primitive states and the host-evaluated constants of
+  // the exact solution (problems/exact_riemann.hpp:114-205).
+  double ql[4], qr[4];
+  double x0, p_star, u_star, a_l, a_r;
+  int shock_l, shock_r;
+  double speed_l, head_l, tail_l, rho_star_l;
+  double speed_r, head_r, tail_r, rho_star_r;
+  // BLOB: problems/problem_initialization.hpp:209-263.
+  double blob_cx, blob_cy, blob_u, blob_v, blob_p, blob_r, rho_in, rho_out;
+};
 
 // Device-resident state of the explicit driver loop.  The last workgroup of
 // every step kernel updates it; the next step kernel reads it at start.
@@ -39,6 +63,9 @@ struct GridParams {
   int dx_pow2, dy_pow2;  // dx (dy) is a power of two: x / dx == x * inv_dx exactly
   int bc[4];
   double fixed[4][4];
+  const ExactBc* exact;     // device copy; null unless a side is BC_EXACT
+  const double* time_ptr;   // device-loop steps: the run control's time
+  double time;              // otherwise: time of the exact ghost states
 };
 
 }  // namespace ec
@@ -64,6 +91,10 @@ struct ec_ctx {
   bool timing = false;
   std::vector<hipEvent_t> tev;
   int tused = 0;
+  // Exact-Dirichlet boundary: device descriptor and ghost-ring table.
+  ec::ExactBc* exact_dev = nullptr;
+  double* exact_ring = nullptr;
+  int exact_kind = 0;
 };
 
 namespace ec {

@@ -165,6 +165,9 @@ ec_status launch_explicit_step_on(ec_ctx* ctx, hipStream_t stream, const double*
   a.rs = rs;
   a.cs = cs;
   a.g = ctx->grid;
+  // Exact ghost states at the step's start time: the run control's time in
+  // the device loop (read by the kernel), the context's boundary time else.
+  a.g.time_ptr = device_dt == 1 ? &ctx->ctrl->time : nullptr;
   a.dt = dt;
   a.device_dt = device_dt == 1 ? 1 : 0;
   a.diag = device_dt == 2 ? 0 : 1;

@@ -38,6 +38,8 @@
 
 namespace ec {
 
+bool has_exact_side(const GridParams& g);
+
 constexpr int RB = 256;        // threads per reduction workgroup
 constexpr int MAXK = 64;       // largest GMRES restart length
 
@@ -1939,6 +1941,12 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
   if (!ctx || !U || !lay || !opts || !diag) return EC_ERR_INVALID_ARGUMENT;
   if (lay->row_stride < ctx->grid.nx || lay->comp_stride < lay->row_stride * ctx->grid.ny) {
     return EC_ERR_INVALID_ARGUMENT;
+  }
+  if (has_exact_side(ctx->grid)) {
+    // The source-map step fills exact ghosts at time 0 (imex/
+    // imex_source_map_step.hpp:275,406); that boundary is not built here.
+    ctx->last_error = "the IMEX step does not take exact-Dirichlet sides";
+    return EC_ERR_UNSUPPORTED;
   }
   if (opts->picard_iterations < 1 || opts->restart < 1 || opts->restart > MAXK ||
       !(opts->tol > 0.0) || opts->max_iter < 0 ||

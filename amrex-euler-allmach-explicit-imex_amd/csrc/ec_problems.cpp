@@ -5,11 +5,9 @@
 // amrex/apps/euler_compare/src/euler_compare/.
 #include <cmath>
 
-#include "../../include/ec_amd.h"
+#include "ec_problems.hpp"
 
-namespace {
-
-struct P4 { double r, u, v, p; };
+namespace ec::prob {
 
 void to_cons(const P4& q, double gamma, double* u)
 {
@@ -66,6 +64,67 @@ P4 gresho_state(double x, double y, const ec_problem_desc& d)
   return P4{rho, -vt * dy / r, vt * dx / r, p};
 }
 
+// problems/exact_riemann.hpp:37-112: Newton iteration on the pressure
+// function f_L(p) + f_R(p) + u_R - u_L = 0 (Toro, ch. 4), from the
+// two-rarefaction-style guess, halving on a non-positive or non-finite step.
+static void pressure_function(double p, const P4& q, double gamma, double& value, double& derivative)
+{
+  const double a = std::sqrt(gamma * q.p / q.r);
+  if (p > q.p) {
+    const double acoef = 2.0 / ((gamma + 1.0) * q.r);
+    const double bcoef = (gamma - 1.0) / (gamma + 1.0) * q.p;
+    const double root = std::sqrt(acoef / (p + bcoef));
+    value = (p - q.p) * root;
+    derivative = root * (1.0 - 0.5 * (p - q.p) / (p + bcoef));
+    return;
+  }
+  const double ratio = p / q.p;
+  value = 2.0 * a / (gamma - 1.0) * (std::pow(ratio, (gamma - 1.0) / (2.0 * gamma)) - 1.0);
+  derivative = 1.0 / (q.r * a) * std::pow(ratio, -(gamma + 1.0) / (2.0 * gamma));
+}
+
+void exact_riemann_star(const P4& L, const P4& R, double gamma, double& p_star, double& u_star)
+{
+  const double a_l = std::sqrt(gamma * L.p / L.r);
+  const double a_r = std::sqrt(gamma * R.p / R.r);
+  const double guess = 0.5 * (L.p + R.p) - 0.125 * (R.u - L.u) * (L.r + R.r) * (a_l + a_r);
+  double p = 1.0e-12 < guess ? guess : 1.0e-12;
+  for (int iter = 0; iter < 80; ++iter) {
+    double fl, dl, fr, dr;
+    pressure_function(p, L, gamma, fl, dl);
+    pressure_function(p, R, gamma, fr, dr);
+    double next = p - (fl + fr + R.u - L.u) / (dl + dr);
+    if (!(next > 0.0) || !std::isfinite(next)) next = 0.5 * p;
+    const bool done = std::fabs(next - p) <= 1.0e-12 * (1.0 < next ? next : 1.0);
+    p = next;
+    if (done) break;
+  }
+  double fl, dl, fr, dr;
+  pressure_function(p, L, gamma, fl, dl);
+  pressure_function(p, R, gamma, fr, dr);
+  p_star = p;
+  u_star = 0.5 * (L.u + R.u + fr - fl);
+}
+
+// problems/problem_state_functions.hpp:73-98 (Liska-Wendroff quadrant)
+P4 quadrant_state(double x, double y, const ec_problem_desc& d)
+{
+  const bool right = x >= d.riemann_interface_x;
+  const bool top = y >= d.riemann_interface_y;
+  if (right && top) return P4{1.5, 0.0, 0.0, 1.5};
+  if (!right && top) return P4{0.532258064516129, 1.206045378311055, 0.0, 0.3};
+  if (!right && !top) {
+    return P4{0.137992831541219, 1.206045378311055, 1.206045378311055, 0.029032258064516};
+  }
+  return P4{0.532258064516129, 0.0, 1.206045378311055, 0.3};
+}
+
+}  // namespace ec::prob
+
+namespace {
+
+using namespace ec::prob;
+
 // problems/problem_state_functions.hpp:32-71
 double periodic_nearest_delta(double x, double c, double lo, double hi)
 {
@@ -92,18 +151,6 @@ P4 isentropic_vortex_state(double x, double y, const ec_problem_desc& d)
   return P4{rho, d.velocity[0] - vf * dy, d.velocity[1] + vf * dx, p};
 }
 
-// problems/problem_state_functions.hpp:73-98 (Liska-Wendroff quadrant)
-P4 quadrant_state(double x, double y, const ec_problem_desc& d)
-{
-  const bool right = x >= d.riemann_interface_x;
-  const bool top = y >= d.riemann_interface_y;
-  if (right && top) return P4{1.5, 0.0, 0.0, 1.5};
-  if (!right && top) return P4{0.532258064516129, 1.206045378311055, 0.0, 0.3};
-  if (!right && !top) {
-    return P4{0.137992831541219, 1.206045378311055, 1.206045378311055, 0.029032258064516};
-  }
-  return P4{0.532258064516129, 0.0, 1.206045378311055, 0.3};
-}
 
 // problems/problem_initialization.hpp:2-102 (fixed geometry of the
 // reference's shock-density bubble: shock x=0.2, bubble (0.5,0) r=0.2).
@@ -199,8 +246,11 @@ extern "C" ec_status ec_init_state(const ec_problem_desc* d, double* U, const ec
           }
           break;
         case EC_PROBLEM_ADVECTION_BLOB: {  // problem_initialization.hpp:127-155
+          // The reference places the blob's cell centres in y with the x
+          // cell size (:146); on square cells this is the usual centre.
+          const double yb = d->prob_lo[1] + ((double)j + 0.5) * dx;
           const double cx = d->blob_center[0], cy = d->blob_center[1];
-          const double r2 = (x - cx) * (x - cx) + (y - cy) * (y - cy);
+          const double r2 = (x - cx) * (x - cx) + (yb - cy) * (yb - cy);
           const double rho = r2 <= d->blob_radius * d->blob_radius ? d->density_inner : d->density_outer;
           const double u = d->velocity[0], v = d->velocity[1];
           const long long k = (long long)j * lay->row_stride + i;

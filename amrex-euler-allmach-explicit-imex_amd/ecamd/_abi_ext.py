@@ -56,6 +56,10 @@ def declare(L) -> None:
     L.ec_init_state.argtypes = [C.POINTER(ProblemDesc), dp, C.POINTER(Layout)]
     L.ec_init_state.restype = st
     L.ec_post_shock_state.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
+    L.ec_set_exact_boundary.argtypes = [vp, C.POINTER(ProblemDesc), C.POINTER(C.c_int)]
+    L.ec_set_exact_boundary.restype = st
+    L.ec_set_boundary_time.argtypes = [vp, C.c_double]
+    L.ec_set_boundary_time.restype = st
     L.ec_post_shock_state.restype = st
     from ._lib import GridDesc
     L.ec_adjoint_step_host.argtypes = [C.POINTER(GridDesc), dp, dp, dp, C.c_double]

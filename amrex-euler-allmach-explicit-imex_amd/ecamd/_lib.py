@@ -20,7 +20,7 @@ RIEMANN = {"rusanov": 0, "local_lax_friedrichs": 0, "llf": 0, "hllc": 1, "lowmac
            "lm_hllc": 2, "low_mach_hllc": 2, "xie_am_hllc_p": 3, "am_hllc_p": 3,
            "xie_hllc_p": 3}
 LIMITER = {"minmod": 0, "mc": 1, "monotonized_central": 1, "van_leer": 2, "vanleer": 2}
-BC = {"periodic": 0, "outflow": 1, "reflect": 2, "fixed": 3}
+BC = {"periodic": 0, "outflow": 1, "reflect": 2, "fixed": 3, "exact": 4}
 
 RUN_STATUS = {0: "ok", 1: "initial_nonphysical_state", 2: "invalid_time_step",
               3: "nonfinite_state", 4: "nonpositive_density", 5: "nonpositive_pressure"}

@@ -118,6 +118,21 @@ class ExplicitSolver:
             out.append((a.value, b.value, c.value))
         return out
 
+    def set_exact_boundary(self, problem, periodic) -> None:
+        """euler.field_boundary=exact_dirichlet: the exact branches of
+        fill_problem_ghosts (problems/boundary_conditions.hpp:133-219).
+        problem: a ProblemDesc (see problems.problem_desc) of this grid;
+        periodic: geometry.is_periodic as two ints."""
+        per = (C.c_int * 2)(int(periodic[0]), int(periodic[1]))
+        check(self._ctx, lib().ec_set_exact_boundary(self._ctx, C.byref(problem), per),
+              "ec_set_exact_boundary")
+
+    def set_boundary_time(self, time: float) -> None:
+        """Time of the exact ghost states for the next host-dt steps (the time
+        argument of euler_fv_one_step, numerics/explicit_update.hpp:58-65)."""
+        check(self._ctx, lib().ec_set_boundary_time(self._ctx, float(time)),
+              "ec_set_boundary_time")
+
     def step(self, U_in: torch.Tensor, U_out: torch.Tensor, dt: float,
              diag: bool = True) -> StepResult | None:
         """One explicit step U_out = U_in - dt div F(U_in)."""

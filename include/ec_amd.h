@@ -49,8 +49,16 @@ typedef enum { EC_LIMITER_MINMOD = 0, EC_LIMITER_MC = 1, EC_LIMITER_VAN_LEER = 2
 /* Per-side ghost rule.  PERIODIC and OUTFLOW reproduce
  * problems/boundary_conditions.hpp:2-25; FIXED (inflow state, priority over
  * the other axis) and REFLECT (mirror, normal momentum negated) reproduce the
- * shock-bubble fill at problems/boundary_conditions.hpp:27-73. */
-typedef enum { EC_BC_PERIODIC = 0, EC_BC_OUTFLOW = 1, EC_BC_REFLECT = 2, EC_BC_FIXED = 3 } ec_bc;
+ * shock-bubble fill at problems/boundary_conditions.hpp:27-73.  EXACT (the
+ * problem's exact state, priority over the other axis) is set only by
+ * ec_set_exact_boundary. */
+typedef enum {
+  EC_BC_PERIODIC = 0,
+  EC_BC_OUTFLOW = 1,
+  EC_BC_REFLECT = 2,
+  EC_BC_FIXED = 3,
+  EC_BC_EXACT = 4
+} ec_bc;
 
 typedef struct {
   int nx, ny;               /* valid cells */
@@ -373,6 +381,30 @@ ec_status ec_init_state(const ec_problem_desc* desc, double* host_U, const ec_la
 /* Post-shock primitive state (rho, u, v, p) behind a shock of the given Mach
  * number moving into gas at rest (Rankine-Hugoniot). */
 ec_status ec_post_shock_state(double gamma, double mach, double rho1, double p1, double* prim_out);
+
+/* ------------------------------------------- exact-Dirichlet ghost cells */
+/* euler.field_boundary=exact_dirichlet for the context's grid.  Replaces the
+ * exact branches of void fill_problem_ghosts(MultiFab&, const Geometry&,
+ * const RunConfig&, Real time) (problems/boundary_conditions.hpp:202-219):
+ * fill_gresho_exact_dirichlet_ghosts (:133-163),
+ * fill_toro_x_exact_dirichlet_ghosts, fill_riemann_quadrant_exact_dirichlet_ghosts
+ * and fill_advection_blob_exact_dirichlet_ghosts
+ * (problems/problem_initialization.hpp:308-342, :344-375, :209-263).
+ * problem: kind GRESHO, TORO, RIEMANN_QUADRANT or ADVECTION_BLOB with the
+ * context's nx, ny and gamma; periodic: geometry.is_periodic (2 ints).  With
+ * an axis not periodic, the sides of every non-periodic axis (and the x sides
+ * of a Toro run) become EC_BC_EXACT and the others EC_BC_PERIODIC; with both
+ * axes periodic every side becomes EC_BC_PERIODIC, as in the reference.
+ * Ghost states are evaluated on read: Gresho and the quadrant from a table
+ * built here, the Toro exact solution and the advected blob at the step's
+ * time -- the time set by ec_set_boundary_time for ec_explicit_step and
+ * ec_explicit_step_inplace, the run's own time in ec_explicit_advance.
+ * ec_imex_step and the adjoint entry points return EC_ERR_UNSUPPORTED on
+ * such a context.  Synchronises the context's stream. */
+ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* problem, const int* periodic);
+/* The time argument of euler_fv_one_step (numerics/explicit_update.hpp:58-65)
+ * for the exact ghost states of the next host-dt steps (default 0). */
+ec_status ec_set_boundary_time(ec_ctx* ctx, double time);
 
 #ifdef __cplusplus
 }

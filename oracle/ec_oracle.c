@@ -345,9 +345,69 @@ static int map_index(int i, int n, int lo_bc, int hi_bc, int* reflected)
   return i;
 }
 
+static void gresho_prim(double x, double y, double gamma, double cx, double cy, double rho,
+                        double mach, double p0, double* q);
+
+/* The exact branches of fill_problem_ghosts (boundary_conditions.hpp:165-219):
+ * active when an axis is not periodic; the physical ghosts are those of a
+ * non-periodic axis (corners included), and every x ghost for Toro
+ * (problem_initialization.hpp:329-330). */
+int ecor_exact_ghost(const ecor_grid* g, int i, int j)
+{
+  const ecor_exact* e = g->exact;
+  if (!e || e->kind == ECOR_EXACT_NONE || (e->periodic[0] && e->periodic[1])) return 0;
+  const int x_out = i < 0 || i >= g->nx, y_out = j < 0 || j >= g->ny;
+  if (e->kind == ECOR_EXACT_TORO) return x_out || (!e->periodic[1] && y_out);
+  return (!e->periodic[0] && x_out) || (!e->periodic[1] && y_out);
+}
+
+/* Gresho (boundary_conditions.hpp:133-163) and quadrant
+ * (problem_initialization.hpp:344-375) at the ghost's own centre; Toro at the
+ * ghost's x and time (:308-342); the blob at the periodic image of a periodic
+ * axis, its centre advected to the time (:209-263). */
+void ecor_exact_state(const ecor_exact* e, int nx, int ny, double time, int i, int j, double* u)
+{
+  const double dx = (e->prob_hi[0] - e->prob_lo[0]) / nx;
+  const double dy = (e->prob_hi[1] - e->prob_lo[1]) / ny;
+  double q[4];
+  if (e->kind == ECOR_EXACT_BLOB) {
+    int ii = i, jj = j;
+    if (e->periodic[0]) { const int o = i % nx; ii = o < 0 ? o + nx : o; }
+    if (e->periodic[1]) { const int o = j % ny; jj = o < 0 ? o + ny : o; }
+    const double x = e->prob_lo[0] + ((double)ii + 0.5) * dx;
+    const double y = e->prob_lo[1] + ((double)jj + 0.5) * dy;
+    const double cx = e->blob_center[0] + e->velocity[0] * time;
+    const double cy = e->blob_center[1] + e->velocity[1] * time;
+    const double r2 = (x - cx) * (x - cx) + (y - cy) * (y - cy);
+    const double rho = r2 <= e->blob_radius * e->blob_radius ? e->density_inner : e->density_outer;
+    q[0] = rho; q[1] = e->velocity[0]; q[2] = e->velocity[1]; q[3] = e->pressure;
+    ecor_to_conserved(q, e->gamma, u);
+    return;
+  }
+  const double x = e->prob_lo[0] + ((double)i + 0.5) * dx;
+  const double y = e->prob_lo[1] + ((double)j + 0.5) * dy;
+  if (e->kind == ECOR_EXACT_GRESHO) {
+    gresho_prim(x, y, e->gamma, e->vortex_center[0], e->vortex_center[1], e->density_outer,
+                e->mach, e->pressure, q);
+  } else if (e->kind == ECOR_EXACT_TORO) {
+    ecor_exact_riemann_sample(x, time, e->ql, e->qr, e->gamma, e->interface_x, q);
+  } else {
+    /* problem_state_functions.hpp:73-98 */
+    const int right = x >= e->interface_x, top = y >= e->interface_y;
+    if (right && top) { q[0] = 1.5; q[1] = 0.0; q[2] = 0.0; q[3] = 1.5; }
+    else if (!right && top) { q[0] = 0.532258064516129; q[1] = 1.206045378311055; q[2] = 0.0; q[3] = 0.3; }
+    else if (!right && !top) {
+      q[0] = 0.137992831541219; q[1] = 1.206045378311055; q[2] = 1.206045378311055;
+      q[3] = 0.029032258064516;
+    } else { q[0] = 0.532258064516129; q[1] = 0.0; q[2] = 1.206045378311055; q[3] = 0.3; }
+  }
+  ecor_to_conserved(q, e->gamma, u);
+}
+
 void ecor_load(const ecor_grid* g, const double* U, int i, int j, double* u)
 {
   const int nx = g->nx, ny = g->ny;
+  if (ecor_exact_ghost(g, i, j)) { ecor_exact_state(g->exact, nx, ny, g->time, i, j, u); return; }
   if (i < 0 && g->bc[0] == ECOR_BC_FIXED) { memcpy(u, g->fixed[0], 4 * sizeof(double)); return; }
   if (i >= nx && g->bc[1] == ECOR_BC_FIXED) { memcpy(u, g->fixed[1], 4 * sizeof(double)); return; }
   if (j < 0 && g->bc[2] == ECOR_BC_FIXED) { memcpy(u, g->fixed[2], 4 * sizeof(double)); return; }
@@ -559,10 +619,12 @@ void ecor_run_explicit(const ecor_grid* g, double* U, double cfl, double stop_ti
   int step = 0;
   ecor_diagnostics(g, U, &out->rho_min, &out->p_min, &out->nonfinite);
   if (out->nonfinite > 0 || out->rho_min <= 0.0 || out->p_min <= 0.0) out->status = 1;
+  ecor_grid gt = *g; /* the step's time for exact ghost states (explicit_update.hpp:65) */
   while (out->status == 0 && step < max_step && time < stop_time - 1.0e-14) {
     const double dt = ecor_euler_dt(ecor_max_rate(g, U), cfl, time, stop_time);
     if (!isfinite(dt) || dt <= 0.0) { out->status = 2; break; }
-    out->degenerate_total += ecor_fv_step(g, U, next, dt);
+    gt.time = time;
+    out->degenerate_total += ecor_fv_step(&gt, U, next, dt);
     memcpy(U, next, (size_t)n * sizeof(double));
     if (dts) dts[step] = dt;
     out->dt_last = dt;
@@ -581,34 +643,41 @@ void ecor_run_explicit(const ecor_grid* g, double* U, double cfl, double stop_ti
 /* ------------------------------------------------------------------------ */
 /* problem_state_functions.hpp:2-30 with velocity_scale = mach * sqrt(gamma p0 / rho)
  * (state_primitives.hpp:207-215), and problem_initialization.hpp:377-392. */
+static void gresho_prim(double x, double y, double gamma, double cx, double cy, double rho,
+                        double mach, double p0, double* q)
+{
+  const double vscale = mach * sqrt(gamma * p0 / rho);
+  const double ddx = x - cx, ddy = y - cy;
+  const double r = sqrt(ddx * ddx + ddy * ddy);
+  double tf = 0.0;
+  double pf = -2.0 + 4.0 * log(2.0);
+  if (r < 0.2) {
+    tf = 5.0 * r;
+    pf = 12.5 * r * r;
+  } else if (r < 0.4) {
+    tf = 2.0 - 5.0 * r;
+    pf = 12.5 * r * r + 4.0 * (1.0 - 5.0 * r - log(0.2) + log(r));
+  }
+  const double vt = vscale * tf;
+  const double p = p0 + rho * vscale * vscale * pf;
+  q[0] = rho; q[1] = 0.0; q[2] = 0.0; q[3] = p;
+  if (!(r < 1.0e-14)) {
+    q[1] = -vt * ddy / r;
+    q[2] = vt * ddx / r;
+  }
+}
+
 void ecor_init_gresho(int nx, int ny, double xlo, double ylo, double dx, double dy,
                       double gamma, double cx, double cy, double rho, double mach, double p0,
                       double* U)
 {
   const long plane = (long)nx * ny;
-  const double vscale = mach * sqrt(gamma * p0 / rho);
   for (int j = 0; j < ny; ++j) {
     for (int i = 0; i < nx; ++i) {
       const double x = xlo + ((double)i + 0.5) * dx;
       const double y = ylo + ((double)j + 0.5) * dy;
-      const double ddx = x - cx, ddy = y - cy;
-      const double r = sqrt(ddx * ddx + ddy * ddy);
-      double tf = 0.0;
-      double pf = -2.0 + 4.0 * log(2.0);
-      if (r < 0.2) {
-        tf = 5.0 * r;
-        pf = 12.5 * r * r;
-      } else if (r < 0.4) {
-        tf = 2.0 - 5.0 * r;
-        pf = 12.5 * r * r + 4.0 * (1.0 - 5.0 * r - log(0.2) + log(r));
-      }
-      const double vt = vscale * tf;
-      const double p = p0 + rho * vscale * vscale * pf;
-      double q[4] = {rho, 0.0, 0.0, p};
-      if (!(r < 1.0e-14)) {
-        q[1] = -vt * ddy / r;
-        q[2] = vt * ddx / r;
-      }
+      double q[4];
+      gresho_prim(x, y, gamma, cx, cy, rho, mach, p0, q);
       double u[4];
       ecor_to_conserved(q, gamma, u);
       const long k = (long)j * nx + i;

@@ -22,6 +22,23 @@ enum { ECOR_RUSANOV = 0, ECOR_HLLC = 1, ECOR_LOWMACH_HLLC = 2, ECOR_XIE_HLLCP = 
 enum { ECOR_MINMOD = 0, ECOR_MC = 1, ECOR_VANLEER = 2 };
 enum { ECOR_BC_PERIODIC = 0, ECOR_BC_OUTFLOW = 1, ECOR_BC_REFLECT = 2, ECOR_BC_FIXED = 3 };
 
+/* Exact-Dirichlet ghost states, euler.field_boundary=exact_dirichlet
+ * (problems/boundary_conditions.hpp:133-219; problems/
+ * problem_initialization.hpp:209-263, 308-375). */
+enum { ECOR_EXACT_NONE = 0, ECOR_EXACT_GRESHO = 1, ECOR_EXACT_TORO = 2,
+       ECOR_EXACT_QUADRANT = 3, ECOR_EXACT_BLOB = 4 };
+
+typedef struct {
+  int kind;                      /* ECOR_EXACT_* */
+  int periodic[2];               /* geometry.is_periodic */
+  double prob_lo[2], prob_hi[2]; /* cell size (hi - lo) / n */
+  double gamma;
+  double vortex_center[2], density_outer, pressure, mach; /* Gresho; blob outer/pressure */
+  double interface_x, interface_y;                        /* quadrant; Toro x0 */
+  double ql[4], qr[4];                                    /* Toro primitive states */
+  double blob_center[2], blob_radius, density_inner, velocity[2];
+} ecor_exact;
+
 typedef struct {
   int nx, ny;
   double dx, dy;
@@ -31,6 +48,8 @@ typedef struct {
   int order;     /* 1 or 2 */
   int bc[4];     /* xlo, xhi, ylo, yhi */
   double fixed[4][4]; /* conserved state used by an ECOR_BC_FIXED side */
+  const ecor_exact* exact; /* NULL, or the exact ghost states (priority over bc) */
+  double time;             /* time of the exact ghost states */
 } ecor_grid;
 
 /* ---- pointwise algebra (state_primitives.hpp / reconstruction_fluxes.hpp) */
@@ -49,6 +68,9 @@ void ecor_exact_riemann_sample(double x, double t, const double* qL, const doubl
 
 /* ---- grid operations (explicit_update.hpp, diagnostic_metrics_timestep.hpp) */
 void ecor_load(const ecor_grid* g, const double* U, int i, int j, double* u);
+/* Whether ghost (i, j) holds an exact-Dirichlet state, and that state. */
+int ecor_exact_ghost(const ecor_grid* g, int i, int j);
+void ecor_exact_state(const ecor_exact* e, int nx, int ny, double time, int i, int j, double* u);
 void ecor_reconstruct(const ecor_grid* g, const double* U, int i, int j, int axis,
                       int side, double* q);
 void ecor_face_flux(const ecor_grid* g, const double* U, int face_i, int face_j,

@@ -28,10 +28,49 @@ RIEMANN_NAMES = {"rusanov": RUSANOV, "hllc": HLLC, "low_mach_hllc": LOWMACH_HLLC
 _dp = C.POINTER(C.c_double)
 
 
+EXACT_NONE, EXACT_GRESHO, EXACT_TORO, EXACT_QUADRANT, EXACT_BLOB = 0, 1, 2, 3, 4
+
+
+class Exact(C.Structure):
+    _fields_ = [("kind", C.c_int), ("periodic", C.c_int * 2), ("prob_lo", C.c_double * 2),
+                ("prob_hi", C.c_double * 2), ("gamma", C.c_double),
+                ("vortex_center", C.c_double * 2), ("density_outer", C.c_double),
+                ("pressure", C.c_double), ("mach", C.c_double), ("interface_x", C.c_double),
+                ("interface_y", C.c_double), ("ql", C.c_double * 4), ("qr", C.c_double * 4),
+                ("blob_center", C.c_double * 2), ("blob_radius", C.c_double),
+                ("density_inner", C.c_double), ("velocity", C.c_double * 2)]
+
+
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
                 ("gamma", C.c_double), ("riemann", C.c_int), ("limiter", C.c_int),
-                ("order", C.c_int), ("bc", C.c_int * 4), ("fixed", (C.c_double * 4) * 4)]
+                ("order", C.c_int), ("bc", C.c_int * 4), ("fixed", (C.c_double * 4) * 4),
+                ("exact", C.POINTER(Exact)), ("time", C.c_double)]
+
+
+def make_exact(kind, periodic, prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), gamma=1.4, **kw) -> Exact:
+    """Exact-Dirichlet ghost description; keyword names are the Exact fields."""
+    e = Exact()
+    e.kind = kind
+    e.periodic[0], e.periodic[1] = int(periodic[0]), int(periodic[1])
+    e.prob_lo[0], e.prob_lo[1] = prob_lo
+    e.prob_hi[0], e.prob_hi[1] = prob_hi
+    e.gamma = gamma
+    for k, v in kw.items():
+        f = getattr(e, k)
+        if isinstance(f, float) or isinstance(f, int):
+            setattr(e, k, v)
+        else:
+            for c, x in enumerate(v):
+                f[c] = float(x)
+    return e
+
+
+def set_exact(g: Grid, e: Exact | None, time: float = 0.0) -> Grid:
+    """Attach (or detach) exact ghost states and their time to a grid."""
+    g.exact = C.pointer(e) if e is not None else C.POINTER(Exact)()
+    g.time = float(time)
+    return g
 
 
 class RunResult(C.Structure):
@@ -93,6 +132,10 @@ def lib():
         L.ecor_face_flux.argtypes = [G, _dp, C.c_int, C.c_int, C.c_int, _dp]
         L.ecor_reconstruct.argtypes = [G, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.ecor_load.argtypes = [G, _dp, C.c_int, C.c_int, _dp]
+        L.ecor_exact_ghost.argtypes = [G, C.c_int, C.c_int]
+        L.ecor_exact_ghost.restype = C.c_int
+        L.ecor_exact_state.argtypes = [C.POINTER(Exact), C.c_int, C.c_int, C.c_double, C.c_int,
+                                       C.c_int, _dp]
         L.ecor_max_rate.argtypes = [G, _dp]
         L.ecor_max_rate.restype = C.c_double
         L.ecor_euler_dt.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
@@ -205,6 +248,16 @@ def face_flux(g: Grid, U, fi, fj, axis):
 def load(g: Grid, U, i, j):
     U = np.ascontiguousarray(U, dtype=np.float64); u = np.zeros(4)
     lib().ecor_load(C.byref(g), _ptr(U), i, j, _ptr(u)); return u
+
+
+def exact_ghost(g: Grid, i, j) -> bool:
+    return bool(lib().ecor_exact_ghost(C.byref(g), i, j))
+
+
+def exact_state(e: Exact, nx, ny, time, i, j) -> np.ndarray:
+    u = np.zeros(4)
+    lib().ecor_exact_state(C.byref(e), nx, ny, time, i, j, _ptr(u))
+    return u
 
 
 def max_rate(g: Grid, U):

@@ -79,3 +79,33 @@ def test_imex_row(tmp_path):
     assert kv["imex_step_count"] == "3"
     assert kv["imex_solver_status"] in ("converged_absolute", "converged_relative")
     assert float(kv["imex_solver_relative_residual_max"]) <= 1e-10
+
+
+def test_toro_exact_dirichlet_row(tmp_path, ecor):
+    """A Riemann-strip row of the convergence study: geometry.is_periodic=0 0,
+    euler.field_boundary=exact_dirichlet, 5 cells across (the device loop with
+    the exact Toro ghost states at each step's time)."""
+    out_csv = tmp_path / "toro.csv"
+    nx, ny = 200, 5
+    rc, kv, err = run(tmp_path, f"amr.n_cell={nx} {ny}", "geometry.prob_hi=1.0 0.1",
+                      "geometry.is_periodic=0 0", "euler.problem=toro1", "prob.test=1",
+                      "euler.field_boundary=exact_dirichlet", "euler.cfl=0.4", "stop_time=0.25",
+                      "max_step=100000", "euler.riemann=xie_am_hllc_p", "euler.spatial_order=2",
+                      f"euler.final_csv={out_csv}")
+    assert rc == 0, err
+    assert kv["project_amrex_euler_compare_status"] == "ok"
+    assert kv["field_boundary"] == "exact_dirichlet"
+    L, R = [1.0, 0.0, 0.0, 1.0], [0.125, 0.0, 0.0, 0.1]
+    e = ecor.make_exact(ecor.EXACT_TORO, (0, 0), prob_hi=(1.0, 0.1), ql=L, qr=R, interface_x=0.5)
+    g = ecor.set_exact(ecor.make_grid(nx, ny, 1 / nx, 0.1 / ny, riemann=ecor.XIE_HLLCP,
+                                      bc=(1, 1, 1, 1)), e, 0.0)
+    ref, res, _ = ecor.run_explicit(g, ecor.init_toro(nx, ny, L, R), 0.4, 0.25, 100000)
+    assert int(kv["completed_steps"]) == res.steps
+    rows = list(csv.DictReader(open(out_csv)))
+    rho = np.array([float(r["rho"]) for r in rows]).reshape(ny, nx)
+    # the fan's ghost states take the device pow(): relative 1e-12 in L1
+    assert np.abs(rho - ref[0]).sum() <= 1e-12 * np.abs(ref[0]).sum()
+    # the exact boundary differs from outflow (the y ghost rows)
+    g0 = ecor.make_grid(nx, ny, 1 / nx, 0.1 / ny, riemann=ecor.XIE_HLLCP, bc=(1, 1, 1, 1))
+    out0, _, _ = ecor.run_explicit(g0, ecor.init_toro(nx, ny, L, R), 0.4, 0.25, 100000)
+    assert not np.array_equal(out0[0], ref[0])
