@@ -386,6 +386,12 @@ void ecor_exact_state(const ecor_exact* e, int nx, int ny, double time, int i, i
   }
   const double x = e->prob_lo[0] + ((double)i + 0.5) * dx;
   const double y = e->prob_lo[1] + ((double)j + 0.5) * dy;
+  ecor_exact_prim_at(e, x, y, time, q);
+  ecor_to_conserved(q, e->gamma, u);
+}
+
+void ecor_exact_prim_at(const ecor_exact* e, double x, double y, double time, double* q)
+{
   if (e->kind == ECOR_EXACT_GRESHO) {
     gresho_prim(x, y, e->gamma, e->vortex_center[0], e->vortex_center[1], e->density_outer,
                 e->mach, e->pressure, q);
@@ -401,7 +407,6 @@ void ecor_exact_state(const ecor_exact* e, int nx, int ny, double time, int i, i
       q[3] = 0.029032258064516;
     } else { q[0] = 0.532258064516129; q[1] = 0.0; q[2] = 1.206045378311055; q[3] = 0.3; }
   }
-  ecor_to_conserved(q, e->gamma, u);
 }
 
 void ecor_load(const ecor_grid* g, const double* U, int i, int j, double* u)

@@ -71,6 +71,8 @@ void ecor_load(const ecor_grid* g, const double* U, int i, int j, double* u);
 /* Whether ghost (i, j) holds an exact-Dirichlet state, and that state. */
 int ecor_exact_ghost(const ecor_grid* g, int i, int j);
 void ecor_exact_state(const ecor_exact* e, int nx, int ny, double time, int i, int j, double* u);
+/* The problem's exact primitive state at a point (no periodic image). */
+void ecor_exact_prim_at(const ecor_exact* e, double x, double y, double time, double* q);
 void ecor_reconstruct(const ecor_grid* g, const double* U, int i, int j, int axis,
                       int side, double* q);
 void ecor_face_flux(const ecor_grid* g, const double* U, int face_i, int face_j,
@@ -148,6 +150,29 @@ void ecor_imex_step(const ecor_grid* g, double* U, double dt, int picard, double
 double ecor_imex_dt(const ecor_grid* g, const double* U, double cfl, double time, double stop,
                     int acoustic_startup, double acoustic_cap, double* material_max,
                     double* acoustic_max);
+
+/* ---- direct BDLTV20 T1/S2 step (ec_oracle_direct.c) */
+typedef struct {
+  ecor_exact exact; /* problem and periodicity; kind TORO or GRESHO gives the
+                       exact-Dirichlet boundary, NONE a periodic case */
+  double epsilon;   /* euler.bdltv20_paper_epsilon */
+  int x_only_row;   /* Riemann strips: pressure row and correction in x only */
+} ecor_direct;
+
+typedef struct {
+  int solver_failures, iterations_max, iterations, guard_count, nonfinite_count, nonpositive_count;
+  double residual_linf_max, relative_residual_linf_max, last_residual_linf;
+} ecor_direct_result;
+
+void ecor_eps_to_primitive(const double* u, double gamma, double eps, double* q);
+void ecor_eps_to_conserved(const double* q, double gamma, double eps, double* u);
+void ecor_direct_boundary_state(const ecor_grid* g, const ecor_direct* d, int i, int j, double t,
+                                double* u);
+/* One step of U (epsilon-scaled conserved, in place) from time to time+dt. */
+void ecor_direct_step(const ecor_grid* g, const ecor_direct* d, double* U, double time, double dt,
+                      int picard, double tol, int max_iter, double* p_out, ecor_direct_result* out);
+double ecor_direct_dt(const ecor_grid* g, const ecor_direct* d, const double* U, double imex_cfl,
+                      double fixed_dt, double time, double stop_time);
 
 #ifdef __cplusplus
 }

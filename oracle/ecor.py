@@ -48,6 +48,17 @@ class Grid(C.Structure):
                 ("exact", C.POINTER(Exact)), ("time", C.c_double)]
 
 
+class Direct(C.Structure):
+    _fields_ = [("exact", Exact), ("epsilon", C.c_double), ("x_only_row", C.c_int)]
+
+
+class DirectResult(C.Structure):
+    _fields_ = [("solver_failures", C.c_int), ("iterations_max", C.c_int), ("iterations", C.c_int),
+                ("guard_count", C.c_int), ("nonfinite_count", C.c_int),
+                ("nonpositive_count", C.c_int), ("residual_linf_max", C.c_double),
+                ("relative_residual_linf_max", C.c_double), ("last_residual_linf", C.c_double)]
+
+
 def make_exact(kind, periodic, prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), gamma=1.4, **kw) -> Exact:
     """Exact-Dirichlet ghost description; keyword names are the Exact fields."""
     e = Exact()
@@ -132,6 +143,14 @@ def lib():
         L.ecor_face_flux.argtypes = [G, _dp, C.c_int, C.c_int, C.c_int, _dp]
         L.ecor_reconstruct.argtypes = [G, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.ecor_load.argtypes = [G, _dp, C.c_int, C.c_int, _dp]
+        D = C.POINTER(Direct)
+        L.ecor_eps_to_primitive.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.ecor_eps_to_conserved.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.ecor_direct_boundary_state.argtypes = [G, D, C.c_int, C.c_int, C.c_double, _dp]
+        L.ecor_direct_step.argtypes = [G, D, _dp, C.c_double, C.c_double, C.c_int, C.c_double,
+                                       C.c_int, _dp, C.POINTER(DirectResult)]
+        L.ecor_direct_dt.argtypes = [G, D, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.ecor_direct_dt.restype = C.c_double
         L.ecor_exact_ghost.argtypes = [G, C.c_int, C.c_int]
         L.ecor_exact_ghost.restype = C.c_int
         L.ecor_exact_state.argtypes = [C.POINTER(Exact), C.c_int, C.c_int, C.c_double, C.c_int,
@@ -257,6 +276,49 @@ def exact_ghost(g: Grid, i, j) -> bool:
 def exact_state(e: Exact, nx, ny, time, i, j) -> np.ndarray:
     u = np.zeros(4)
     lib().ecor_exact_state(C.byref(e), nx, ny, time, i, j, _ptr(u))
+    return u
+
+
+def make_direct(exact: Exact, epsilon=1.0, x_only_row=False) -> Direct:
+    d = Direct()
+    d.exact = exact
+    d.epsilon = float(epsilon)
+    d.x_only_row = int(bool(x_only_row))
+    return d
+
+
+def eps_to_conserved(q, gamma=1.4, eps=1.0):
+    q = _vec(q); u = np.zeros(4)
+    lib().ecor_eps_to_conserved(_ptr(q), gamma, eps, _ptr(u))
+    return u
+
+
+def eps_to_primitive(u, gamma=1.4, eps=1.0):
+    u = _vec(u); q = np.zeros(4)
+    lib().ecor_eps_to_primitive(_ptr(u), gamma, eps, _ptr(q))
+    return q
+
+
+def direct_step(g: Grid, d: Direct, U, time, dt, picard=4, tol=1e-10, max_iter=1000):
+    """One direct BDLTV20 step on a copy of U (epsilon-scaled conserved);
+    returns (U_new, pressure, result)."""
+    U = np.array(U, dtype=np.float64, order="C", copy=True)
+    p = np.zeros(g.nx * g.ny)
+    res = DirectResult()
+    lib().ecor_direct_step(C.byref(g), C.byref(d), _ptr(U), time, dt, picard, tol, max_iter,
+                           _ptr(p), C.byref(res))
+    return U, p.reshape(g.ny, g.nx), res
+
+
+def direct_dt(g: Grid, d: Direct, U, imex_cfl, fixed_dt, time, stop_time):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    return lib().ecor_direct_dt(C.byref(g), C.byref(d), _ptr(U), imex_cfl, fixed_dt, time,
+                                stop_time)
+
+
+def direct_boundary_state(g: Grid, d: Direct, i, j, t):
+    u = np.zeros(4)
+    lib().ecor_direct_boundary_state(C.byref(g), C.byref(d), i, j, t, _ptr(u))
     return u
 
 
