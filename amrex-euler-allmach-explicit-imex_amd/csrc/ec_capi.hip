@@ -12,6 +12,7 @@ int reduce_blocks(const ec_ctx* ctx);
 void imex_release(ec_ctx* ctx);
 void adjoint_release(ec_ctx* ctx);
 void exact_release(ec_ctx* ctx);
+void direct_release(ec_ctx* ctx);
 
 GridParams make_grid_params(const ec_grid_desc& d)
 {
@@ -120,6 +121,7 @@ extern "C" ec_status ec_destroy(ec_ctx* ctx)
   imex_release(ctx);
   adjoint_release(ctx);
   exact_release(ctx);
+  direct_release(ctx);
   for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
   if (ctx->ctrl) (void)hipFree(ctx->ctrl);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);

@@ -86,6 +86,7 @@ struct ec_ctx {
   // IMEX, adjoint and batch workspaces are attached lazily by their modules.
   void* imex = nullptr;
   void* adjoint = nullptr;
+  void* direct = nullptr;
   // Optional step-kernel timing: event pairs recorded around each explicit
   // step kernel launch on the context's stream.
   bool timing = false;

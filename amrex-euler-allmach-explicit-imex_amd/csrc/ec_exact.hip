@@ -21,8 +21,6 @@
 
 namespace ec {
 
-namespace {
-
 // Toro constants of exact_riemann_sample (problems/exact_riemann.hpp:114-205),
 // evaluated once in the reference's operation order.
 void toro_constants(ExactBc& e, const prob::P4& L, const prob::P4& R, double gamma)
@@ -57,8 +55,6 @@ void toro_constants(ExactBc& e, const prob::P4& L, const prob::P4& R, double gam
     e.rho_star_r = rarefaction_density(R);
   }
 }
-
-}  // namespace
 
 void exact_release(ec_ctx* ctx)
 {

@@ -35,6 +35,7 @@
 #include "ec_ghost.hpp"
 #include "ec_state.hpp"
 #include "ec_tile.hpp"
+#include "ec_imex_api.hpp"
 
 namespace ec {
 
@@ -1891,6 +1892,60 @@ static ec_status solve_pressure(ec_ctx* ctx, ImexWork* w, const OpArgs<double>& 
     if (kk == 0) break;
   }
   out->iterations = total;
+  return EC_OK;
+}
+
+ec_status imex_buffers(ec_ctx* ctx, int restart, ImexBuffers* out)
+{
+  const ec_status st = imex_workspace(ctx, restart);
+  if (st != EC_OK) return st;
+  ImexWork* w = static_cast<ImexWork*>(ctx->imex);
+  *out = ImexBuffers{w->nx, w->ny, w->star, w->p, w->x, w->mom, w->hx, w->hy, w->hx32, w->hy32, w->b};
+  return EC_OK;
+}
+
+ec_status imex_solve_pressure(ec_ctx* ctx, const int bc[4], double cx, double cy, double diag,
+                              const ec_imex_opts& o, PressureSolve* out)
+{
+  ec_status st = imex_workspace(ctx, o.restart);
+  if (st != EC_OK) return st;
+  ImexWork* w = static_cast<ImexWork*>(ctx->imex);
+  const long long n = (long long)w->nx * w->ny;
+  ImexGrid G{};
+  G.nx = w->nx;
+  G.ny = w->ny;
+  for (int s = 0; s < 4; ++s) G.bc[s] = bc[s];
+  G.gamma = ctx->grid.gamma;
+  G.dx = ctx->grid.dx;
+  G.dy = ctx->grid.dy;
+  G.inv_dx = ctx->grid.inv_dx;
+  G.inv_dy = ctx->grid.inv_dy;
+  const OpArgs<double> A64{G, w->hx, w->hy, cx, cy, diag};
+  const OpArgs<float> A32{G, w->hx32, w->hy32, (float)cx, (float)cy, (float)diag};
+  *w->hA64 = A64;
+  *w->hA32 = A32;
+  EC_HIP_TRY(ctx, hipMemcpyAsync(w->dA64, w->hA64, sizeof(A64), hipMemcpyHostToDevice, ctx->stream));
+  EC_HIP_TRY(ctx, hipMemcpyAsync(w->dA32, w->hA32, sizeof(A32), hipMemcpyHostToDevice, ctx->stream));
+  // |b|_inf as the residual of x = 0, then the initial guess
+  EC_HIP_TRY(ctx, hipMemsetAsync(w->x, 0, n * sizeof(double), ctx->stream));
+  double binf, b2;
+  if ((st = true_residual(ctx, w, ctx->stream, &binf, &b2)) != EC_OK) return st;
+  *out = PressureSolve{};
+  out->rhs_norm = smax(binf, 1.0);
+  if (!std::isfinite(binf)) {
+    out->bad = 1;
+    out->rinf = out->rel = INFINITY;
+    return EC_OK;
+  }
+  EC_HIP_TRY(ctx, hipMemcpyAsync(w->x, w->p, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  SolveResult sr;
+  if ((st = solve_pressure(ctx, w, A64, A32, out->rhs_norm, o, &sr)) != EC_OK) return st;
+  out->iterations = sr.iterations;
+  out->refinements = sr.refinements;
+  out->bad = sr.bad ? 1 : 0;
+  out->breakdown = sr.breakdown ? 1 : 0;
+  out->rinf = sr.rinf;
+  out->rel = sr.rel;
   return EC_OK;
 }
 

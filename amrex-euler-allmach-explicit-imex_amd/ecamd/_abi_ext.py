@@ -31,6 +31,18 @@ class ImexDiag(C.Structure):
                 ("coupled_residual_linf", C.c_double)]
 
 
+class DirectDiag(C.Structure):
+    _fields_ = [("solver_failures", C.c_int), ("iterations_max", C.c_int), ("iterations", C.c_int),
+                ("guard_count", C.c_int), ("nonfinite_count", C.c_int),
+                ("nonpositive_count", C.c_int), ("residual_linf_max", C.c_double),
+                ("relative_residual_linf_max", C.c_double)]
+
+
+DIRECT_CASES = {"toro_x_exact_dirichlet_y_periodic_2d": 0, "toro_xy_exact_dirichlet_2d": 1,
+                "isentropic_vortex_periodic_2d": 2, "gresho_exact_dirichlet_2d": 3,
+                "advection_blob_periodic_2d": 4}
+
+
 class MemberDesc(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
                 ("bc", C.c_int * 4), ("fixed_state", (C.c_double * 4) * 4), ("cfl", C.c_double),
@@ -83,6 +95,14 @@ def declare(L) -> None:
     L.ec_imex_dt.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.c_double, C.c_double, C.c_int,
                              C.c_double, dp, dp, dp]
     L.ec_imex_dt.restype = st
+    L.ec_direct_setup.argtypes = [vp, C.c_int, C.POINTER(ProblemDesc), C.c_double]
+    L.ec_direct_setup.restype = st
+    L.ec_direct_step.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.c_double,
+                                 C.POINTER(ImexOpts), C.POINTER(DirectDiag), vp]
+    L.ec_direct_step.restype = st
+    L.ec_direct_dt.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.c_double, C.c_double,
+                               C.c_double, dp]
+    L.ec_direct_dt.restype = st
     from ._lib import RunState
     ip = C.POINTER(C.c_int)
     L.ec_batch_create.argtypes = [C.POINTER(MemberDesc), C.c_int, C.c_double, C.c_int, C.c_int,

@@ -233,6 +233,42 @@ class ExplicitSolver:
                                              C.byref(o), C.byref(d), pp), "ec_imex_step")
         return d
 
+    # ---- direct BDLTV20 T1/S2 driver (imex/imex_bdltv20_direct_driver.hpp) --
+    def direct_setup(self, case, problem, epsilon: float = 1.0) -> None:
+        """case: a euler.bdltv20_paper_t1_s2 name or ec_direct_case value;
+        problem: a ProblemDesc of this grid (see problems.problem_desc)."""
+        from ._abi_ext import DIRECT_CASES
+        k = DIRECT_CASES[case] if isinstance(case, str) else int(case)
+        check(self._ctx, lib().ec_direct_setup(self._ctx, k, C.byref(problem), float(epsilon)),
+              "ec_direct_setup")
+
+    def direct_step(self, U: torch.Tensor, time: float, dt: float, opts=None,
+                    p_out: torch.Tensor | None = None):
+        """One direct step of U (epsilon-scaled conserved) in place; returns the counters."""
+        from ._abi_ext import DirectDiag
+        self._check_state(U)
+        o = opts if opts is not None else self.imex_opts()
+        d = DirectDiag()
+        pp = None
+        if p_out is not None:
+            if (p_out.dtype != torch.float64 or p_out.numel() != self.grid.nx * self.grid.ny
+                    or p_out.device != self.device or not p_out.is_contiguous()):
+                raise ValueError("p_out must be a contiguous float64 nx*ny device tensor")
+            pp = _ptr(p_out)
+        check(self._ctx, lib().ec_direct_step(self._ctx, _ptr(U), C.byref(self.layout), float(time),
+                                               float(dt), C.byref(o), C.byref(d), pp),
+              "ec_direct_step")
+        return d
+
+    def direct_dt(self, U: torch.Tensor, imex_cfl: float, fixed_dt: float, time: float,
+                  stop_time: float) -> float:
+        self._check_state(U)
+        dt = C.c_double()
+        check(self._ctx, lib().ec_direct_dt(self._ctx, _ptr(U), C.byref(self.layout), imex_cfl,
+                                             fixed_dt, time, stop_time, C.byref(dt)),
+              "ec_direct_dt")
+        return dt.value
+
     def imex_dt(self, U: torch.Tensor, cfl: float, time: float, stop_time: float,
                 acoustic_startup: int = 0, acoustic_cfl_cap: float = 0.0):
         self._check_state(U)

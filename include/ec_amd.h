@@ -406,6 +406,54 @@ ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* problem, con
  * for the exact ghost states of the next host-dt steps (default 0). */
 ec_status ec_set_boundary_time(ec_ctx* ctx, double time);
 
+/* ------------------------------------------ direct BDLTV20 T1/S2 driver */
+/* The cases of euler.bdltv20_paper_t1_s2 (core/config_reader.hpp:136-142),
+ * each with the periodicity the reference requires of it
+ * (core/config_reader.hpp:228-250). */
+typedef enum {
+  EC_DIRECT_TORO_X_EXACT_Y_PERIODIC = 0, /* toro_x_exact_dirichlet_y_periodic_2d: is_periodic 0 1 */
+  EC_DIRECT_TORO_XY_EXACT = 1,           /* toro_xy_exact_dirichlet_2d: 0 0 */
+  EC_DIRECT_ISENTROPIC_VORTEX_PERIODIC = 2, /* isentropic_vortex_periodic_2d: 1 1 */
+  EC_DIRECT_GRESHO_EXACT = 3,            /* gresho_exact_dirichlet_2d: 0 0 */
+  EC_DIRECT_ADVECTION_BLOB_PERIODIC = 4  /* advection_blob_periodic_2d: 1 1 */
+} ec_direct_case;
+
+/* Per-step counters of the direct driver (imex/imex_bdltv20_direct_driver.hpp:
+ * 140-147); a run passes when every count is zero. */
+typedef struct {
+  int solver_failures;   /* Picard solves not converged; the step continues, as the reference does */
+  int iterations_max;    /* largest Krylov iteration count of one solve (mixed: FP32) */
+  int iterations;        /* summed over the Picard solves */
+  int guard_count;       /* face-enthalpy zero-denominator guards */
+  int nonfinite_count;
+  int nonpositive_count;
+  double residual_linf_max;
+  double relative_residual_linf_max;
+} ec_direct_diag;
+
+/* Prepares the context for the direct step of one case: problem (kind TORO,
+ * ISENTROPIC_VORTEX, GRESHO or ADVECTION_BLOB to match the case, with the
+ * context's nx, ny and gamma) gives the exact solution sampled at the domain
+ * edge for the exact-Dirichlet cases (:101-131); epsilon is
+ * euler.bdltv20_paper_epsilon.  The context's ghost-rule sides are not used
+ * by the direct step.  No reference counterpart: it replaces the set-up part
+ * of int run_bdltv20_paper_t1_s2(const RunConfig&) (:44-138). */
+ec_status ec_direct_setup(ec_ctx* ctx, int direct_case, const ec_problem_desc* problem,
+                          double epsilon);
+/* One step of U (epsilon-scaled conserved state, E = p/(g-1) + eps rho |u|^2/2)
+ * from time to time + dt: the loop body of run_bdltv20_paper_t1_s2
+ * (imex/imex_bdltv20_direct_driver.hpp:148-601) -- star predictor, Picard
+ * iterations of face enthalpies, rows with the exact-Dirichlet pressures
+ * folded into the right-hand side and the pressure solve (opts: Picard count,
+ * tol, max_iter, restart, precision), the momentum correction and the
+ * pressure closeout.  p_out (optional, device, nx*ny) receives the pressure. */
+ec_status ec_direct_step(ec_ctx* ctx, double* U, const ec_layout* layout, double time, double dt,
+                         const ec_imex_opts* opts, ec_direct_diag* diag, double* p_out);
+/* compute_bdltv20_paper_t1_s2_dt (:8-41): fixed_dt > 0 (euler.bdltv20_paper_t1_s2_dt)
+ * clipped to the stop time, else imex_cfl / max(2 |u| / min(dx, dy)). */
+ec_status ec_direct_dt(ec_ctx* ctx, const double* U, const ec_layout* layout, double imex_cfl,
+                       double fixed_dt, double time, double stop_time, double* dt);
+
 #ifdef __cplusplus
 }
 #endif
