@@ -119,6 +119,46 @@ P4 quadrant_state(double x, double y, const ec_problem_desc& d)
   return P4{0.532258064516129, 0.0, 1.206045378311055, 0.3};
 }
 
+// exact_riemann_sample (problems/exact_riemann.hpp:114-205).
+P4 exact_riemann_sample(double x, double t, const P4& L, const P4& R, double gamma, double x0)
+{
+  if (t <= 0.0) return x < x0 ? L : R;
+  double ps, us;
+  exact_riemann_star(L, R, gamma, ps, us);
+  const double xi = (x - x0) / t;
+  const double a_l = std::sqrt(gamma * L.p / L.r), a_r = std::sqrt(gamma * R.p / R.r);
+  const double gm1 = gamma - 1.0, gp1 = gamma + 1.0;
+  auto shock_rho = [&](const P4& q) {
+    const double ratio = ps / q.p;
+    return q.r * ((ratio + gm1 / gp1) / ((gm1 / gp1) * ratio + 1.0));
+  };
+  auto fan_rho = [&](const P4& q) { return q.r * std::pow(ps / q.p, 1.0 / gamma); };
+  if (xi <= us) {
+    if (ps > L.p) {
+      const double s = L.u - a_l * std::sqrt(gp1 / (2.0 * gamma) * ps / L.p + gm1 / (2.0 * gamma));
+      return xi <= s ? L : P4{shock_rho(L), us, L.v, ps};
+    }
+    const double head = L.u - a_l;
+    const double tail = us - a_l * std::pow(ps / L.p, gm1 / (2.0 * gamma));
+    if (xi <= head) return L;
+    if (xi >= tail) return P4{fan_rho(L), us, L.v, ps};
+    const double u = 2.0 / gp1 * (a_l + 0.5 * gm1 * L.u + xi);
+    const double a = 2.0 / gp1 * (a_l + 0.5 * gm1 * (L.u - xi));
+    return P4{L.r * std::pow(a / a_l, 2.0 / gm1), u, L.v, L.p * std::pow(a / a_l, 2.0 * gamma / gm1)};
+  }
+  if (ps > R.p) {
+    const double s = R.u + a_r * std::sqrt(gp1 / (2.0 * gamma) * ps / R.p + gm1 / (2.0 * gamma));
+    return xi >= s ? R : P4{shock_rho(R), us, R.v, ps};
+  }
+  const double head = R.u + a_r;
+  const double tail = us + a_r * std::pow(ps / R.p, gm1 / (2.0 * gamma));
+  if (xi >= head) return R;
+  if (xi <= tail) return P4{fan_rho(R), us, R.v, ps};
+  const double u = 2.0 / gp1 * (-a_r + 0.5 * gm1 * R.u + xi);
+  const double a = 2.0 / gp1 * (a_r - 0.5 * gm1 * (R.u - xi));
+  return P4{R.r * std::pow(a / a_r, 2.0 / gm1), u, R.v, R.p * std::pow(a / a_r, 2.0 * gamma / gm1)};
+}
+
 }  // namespace ec::prob
 
 namespace {
@@ -134,13 +174,16 @@ double periodic_nearest_delta(double x, double c, double lo, double hi)
   return delta;
 }
 
-P4 isentropic_vortex_state(double x, double y, const ec_problem_desc& d)
+P4 isentropic_vortex_state(double x, double y, const ec_problem_desc& d, double time = 0.0)
 {
   const double pi = 3.141592653589793238462643383279502884;
   const double g = d.gamma;
   const double s = d.isentropic_vortex_strength;
-  const double dx = periodic_nearest_delta(x, d.vortex_center[0], d.prob_lo[0], d.prob_hi[0]);
-  const double dy = periodic_nearest_delta(y, d.vortex_center[1], d.prob_lo[1], d.prob_hi[1]);
+  // the vortex advects with the mean flow (problem_state_functions.hpp:52-53)
+  const double cx = d.vortex_center[0] + d.velocity[0] * time;
+  const double cy = d.vortex_center[1] + d.velocity[1] * time;
+  const double dx = periodic_nearest_delta(x, cx, d.prob_lo[0], d.prob_hi[0]);
+  const double dy = periodic_nearest_delta(y, cy, d.prob_lo[1], d.prob_hi[1]);
   const double r2 = dx * dx + dy * dy;
   const double exp_half = std::exp(0.5 * (1.0 - r2));
   const double exp_full = std::exp(1.0 - r2);
@@ -269,6 +312,69 @@ extern "C" ec_status ec_init_state(const ec_problem_desc* d, double* U, const ec
       for (int c = 0; c < 4; ++c) U[c * lay->comp_stride + k] = u[c];
     }
   }
+  return EC_OK;
+}
+
+// problems/problem_initialization.hpp:157-184
+static double periodic_signed_distance(double x, double c, double lo, double hi, bool periodic)
+{
+  double d = x - c;
+  if (periodic) {
+    const double length = hi - lo;
+    if (length > 0.0) d -= std::round(d / length) * length;
+  }
+  return d;
+}
+
+static double wrap_periodic_coordinate(double x, double lo, double hi)
+{
+  const double length = hi - lo;
+  if (length <= 0.0) return x;
+  double w = lo + std::fmod(x - lo, length);
+  if (w < lo) w += length;
+  return w;
+}
+
+extern "C" ec_status ec_exact_state_at(const ec_problem_desc* d, const int* periodic, double x,
+                                       double y, double t, double* prim)
+{
+  if (!d || !periodic || !prim) return EC_ERR_INVALID_ARGUMENT;
+  P4 q{};
+  switch (d->kind) {
+    case EC_PROBLEM_TORO: {
+      P4 l, r;
+      if (!toro_state(d->toro_test, l, r)) return EC_ERR_INVALID_ARGUMENT;
+      q = exact_riemann_sample(x, t, l, r, d->gamma, d->riemann_interface_x);
+      break;
+    }
+    case EC_PROBLEM_GRESHO:
+      q = gresho_state(x, y, *d);
+      break;
+    case EC_PROBLEM_ISENTROPIC_VORTEX:
+      q = isentropic_vortex_state(x, y, *d, t);
+      break;
+    case EC_PROBLEM_RIEMANN_QUADRANT:
+      q = quadrant_state(x, y, *d);
+      break;
+    case EC_PROBLEM_ADVECTION_BLOB: {  // problem_initialization.hpp:186-207
+      double cx = d->blob_center[0] + d->velocity[0] * t;
+      double cy = d->blob_center[1] + d->velocity[1] * t;
+      if (periodic[0]) cx = wrap_periodic_coordinate(cx, d->prob_lo[0], d->prob_hi[0]);
+      if (periodic[1]) cy = wrap_periodic_coordinate(cy, d->prob_lo[1], d->prob_hi[1]);
+      const double ddx = periodic_signed_distance(x, cx, d->prob_lo[0], d->prob_hi[0], periodic[0] != 0);
+      const double ddy = periodic_signed_distance(y, cy, d->prob_lo[1], d->prob_hi[1], periodic[1] != 0);
+      const double r2 = ddx * ddx + ddy * ddy;
+      q = P4{r2 <= d->blob_radius * d->blob_radius ? d->density_inner : d->density_outer,
+             d->velocity[0], d->velocity[1], d->pressure};
+      break;
+    }
+    default:
+      return EC_ERR_INVALID_ARGUMENT;
+  }
+  prim[0] = q.r;
+  prim[1] = q.u;
+  prim[2] = q.v;
+  prim[3] = q.p;
   return EC_OK;
 }
 

@@ -20,5 +20,7 @@ P4 quadrant_state(double x, double y, const ec_problem_desc& d);
 // Star pressure and velocity of the exact Riemann solution
 // (problems/exact_riemann.hpp:37-112).
 void exact_riemann_star(const P4& L, const P4& R, double gamma, double& p_star, double& u_star);
+// The sampled exact Riemann solution (problems/exact_riemann.hpp:114-205).
+P4 exact_riemann_sample(double x, double t, const P4& L, const P4& R, double gamma, double x0);
 
 }  // namespace ec::prob

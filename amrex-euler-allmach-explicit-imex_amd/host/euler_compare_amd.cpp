@@ -85,6 +85,17 @@ std::string lower(std::string s)
   return s;
 }
 
+// euler.bdltv20_paper_t1_s2 -> ec_direct_case, -1 if unknown
+int direct_case(const std::string& m)
+{
+  if (m == "toro_x_exact_dirichlet_y_periodic_2d") return EC_DIRECT_TORO_X_EXACT_Y_PERIODIC;
+  if (m == "toro_xy_exact_dirichlet_2d") return EC_DIRECT_TORO_XY_EXACT;
+  if (m == "isentropic_vortex_periodic_2d") return EC_DIRECT_ISENTROPIC_VORTEX_PERIODIC;
+  if (m == "gresho_exact_dirichlet_2d") return EC_DIRECT_GRESHO_EXACT;
+  if (m == "advection_blob_periodic_2d") return EC_DIRECT_ADVECTION_BLOB_PERIODIC;
+  return -1;
+}
+
 // ---------------------------------------------------------------- config
 struct Cfg {
   int max_step = 8, max_grid_size = 32, plot_int = 8, toro_test = 1, spatial_order = 1;
@@ -107,6 +118,10 @@ struct Cfg {
   // transmissive-boundary systems that the unrestarted solve converges).
   int imex_gmres_restart = 64;
   int problem_kind = EC_PROBLEM_ADVECTION_BLOB, riemann_kind = 1, limiter_kind = 0;
+  // the direct BDLTV20 driver (core/run_config.hpp:33-35, 51-52)
+  double bdltv20_paper_t1_s2_dt = -1.0, bdltv20_paper_epsilon = 1.0;
+  int bdltv20_paper_t1_s2_max_steps = 4096;
+  std::string bdltv20_paper_pressure_solver = "gmres";
 };
 
 // core/config_parse_helpers.hpp:11-42
@@ -212,6 +227,10 @@ Cfg read_config(const Params& pp)
   pp.query("euler.field_boundary", c.field_boundary);
   pp.query("euler.imex_pressure_stabilization", c.imex_pressure_stabilization);
   pp.query("euler.bdltv20_paper_t1_s2", c.bdltv20_paper_t1_s2);
+  pp.query("euler.bdltv20_paper_pressure_solver", c.bdltv20_paper_pressure_solver);
+  pp.query("euler.bdltv20_paper_t1_s2_dt", c.bdltv20_paper_t1_s2_dt);
+  pp.query("euler.bdltv20_paper_epsilon", c.bdltv20_paper_epsilon);
+  pp.query("euler.bdltv20_paper_t1_s2_max_steps", c.bdltv20_paper_t1_s2_max_steps);
   pp.query("euler.shock_density_bubble_snapshot_times", c.snapshot_times);
   pp.query("euler.shock_density_bubble_snapshot_dir", c.snapshot_dir);
   pp.query("euler.spatial_order", c.spatial_order);
@@ -255,10 +274,58 @@ Cfg read_config(const Params& pp)
   }
   if (c.spatial_order != 1 && c.spatial_order != 2) abort_msg("euler.spatial_order must be 1 or 2.");
   if (c.imex_picard_iterations < 1) abort_msg("euler.imex_picard_iterations must be at least one.");
+  c.bdltv20_paper_t1_s2 = lower(c.bdltv20_paper_t1_s2);
+  c.bdltv20_paper_pressure_solver = lower(c.bdltv20_paper_pressure_solver);
   if (c.bdltv20_paper_t1_s2 != "off") {
-    abort_msg("euler.bdltv20_paper_t1_s2 (the direct BDLTV20 driver) is not part of this build.");
+    // core/config_reader.hpp:136-170
+    if (direct_case(c.bdltv20_paper_t1_s2) < 0) abort_msg("Unsupported euler.bdltv20_paper_t1_s2 setting.");
+    if (c.method != "imex") abort_msg("euler.bdltv20_paper_t1_s2 requires euler.method=imex.");
+    if (c.spatial_order != 2) abort_msg("euler.bdltv20_paper_t1_s2 requires euler.spatial_order=2.");
+    if (c.limiter_kind != 0) abort_msg("euler.bdltv20_paper_t1_s2 requires euler.slope_limiter=minmod.");
+    if (!(c.gamma > 1.0) || !std::isfinite(c.gamma)) {
+      abort_msg("euler.bdltv20_paper_t1_s2 requires finite gamma > 1.");
+    }
+    if (!(c.imex_solver_tol > 0.0) || !std::isfinite(c.imex_solver_tol) || c.imex_solver_max_iter < 0) {
+      abort_msg("euler.bdltv20_paper_t1_s2 requires positive finite GMRES tolerance and nonnegative max_iter.");
+    }
+    if (!(c.bdltv20_paper_t1_s2_dt < 0.0) &&
+        (!(c.bdltv20_paper_t1_s2_dt > 0.0) || !std::isfinite(c.bdltv20_paper_t1_s2_dt))) {
+      abort_msg("euler.bdltv20_paper_t1_s2_dt must be negative for CFL mode or finite positive.");
+    }
+    if (c.bdltv20_paper_pressure_solver != "gmres") abort_msg("euler.bdltv20_paper_pressure_solver must be gmres.");
+    if (!(c.bdltv20_paper_epsilon > 0.0) || !std::isfinite(c.bdltv20_paper_epsilon)) {
+      abort_msg("euler.bdltv20_paper_epsilon must be finite and positive.");
+    }
+    // core/config_reader.hpp:228-255
+    const std::string& m = c.bdltv20_paper_t1_s2;
+    const bool exact = c.field_boundary == "exact_dirichlet";
+    if (m == "toro_x_exact_dirichlet_y_periodic_2d" &&
+        (c.problem_kind != EC_PROBLEM_TORO || c.is_periodic[0] != 0 || c.is_periodic[1] != 1 || !exact ||
+         c.n_cell[0] < 16 || c.n_cell[1] < 2)) {
+      abort_msg("BDLTV20 Toro x-exact mode requires problem=toro1, periodicity=0 1, exact Dirichlet, and at least 16x2 cells.");
+    }
+    if (m == "toro_xy_exact_dirichlet_2d" &&
+        (c.problem_kind != EC_PROBLEM_TORO || c.is_periodic[0] != 0 || c.is_periodic[1] != 0 || !exact ||
+         c.n_cell[0] < 16 || c.n_cell[1] < 2)) {
+      abort_msg("BDLTV20 Toro xy-exact mode requires problem=toro1, periodicity=0 0, exact Dirichlet, and at least 16x2 cells.");
+    }
+    if (m == "gresho_exact_dirichlet_2d" &&
+        (c.problem_kind != EC_PROBLEM_GRESHO || c.is_periodic[0] != 0 || c.is_periodic[1] != 0 || !exact ||
+         c.n_cell[0] < 8 || c.n_cell[1] < 8)) {
+      abort_msg("BDLTV20 Gresho mode requires problem=gresho_vortex, periodicity=0 0, exact Dirichlet, and at least 8x8 cells.");
+    }
+    if (m == "isentropic_vortex_periodic_2d" &&
+        (c.problem_kind != EC_PROBLEM_ISENTROPIC_VORTEX || c.is_periodic[0] != 1 || c.is_periodic[1] != 1 ||
+         c.n_cell[0] < 8 || c.n_cell[1] < 8)) {
+      abort_msg("BDLTV20 isentropic-vortex mode requires problem=isentropic_vortex, periodicity=1 1, and at least 8x8 cells.");
+    }
+    if (m == "advection_blob_periodic_2d" &&
+        (c.problem_kind != EC_PROBLEM_ADVECTION_BLOB || c.is_periodic[0] != 1 || c.is_periodic[1] != 1 ||
+         c.n_cell[0] < 8 || c.n_cell[1] < 8)) {
+      abort_msg("BDLTV20 advection-blob mode requires problem=advection_blob, periodicity=1 1, and at least 8x8 cells.");
+    }
   }
-  if (c.field_boundary == "exact_dirichlet" && c.method == "imex" &&
+  if (c.field_boundary == "exact_dirichlet" && c.method == "imex" && c.bdltv20_paper_t1_s2 == "off" &&
       (!c.is_periodic[0] || !c.is_periodic[1])) {
     abort_msg("euler.field_boundary=exact_dirichlet with euler.method=imex is not part of this build.");
   }
@@ -321,6 +388,204 @@ const char* imex_status_name(int s)
   return (s >= 0 && s <= 8) ? n[s] : "unknown";
 }
 
+
+// run_bdltv20_paper_t1_s2 (imex/imex_bdltv20_direct_driver.hpp:44-779): the
+// direct driver loop on the device, the error norms against the exact
+// solution, the final CSV and the report keys.
+int run_direct(const Cfg& cfg, const ec_problem_desc& pd)
+{
+  const int nx = cfg.n_cell[0], ny = cfg.n_cell[1];
+  const size_t n = (size_t)nx * ny;
+  const double dx = (cfg.prob_hi[0] - cfg.prob_lo[0]) / nx;
+  const double dy = (cfg.prob_hi[1] - cfg.prob_lo[1]) / ny;
+  const double gamma = cfg.gamma, eps = cfg.bdltv20_paper_epsilon, gm1 = gamma - 1.0;
+  const std::string& mode = cfg.bdltv20_paper_t1_s2;
+  const bool is_toro = mode.rfind("toro", 0) == 0;
+  const bool is_vortex = mode == "isentropic_vortex_periodic_2d";
+  const bool is_blob = mode == "advection_blob_periodic_2d";
+  const bool x_only = is_toro;
+  std::vector<double> host(4 * n);
+  ec_layout lay{nx, (long long)n};
+  if (ec_init_state(&pd, host.data(), &lay) != EC_OK) abort_msg("Unsupported Toro test id.");
+  // the epsilon-scaled state (:76-84)
+  for (size_t k = 0; k < n; ++k) {
+    const double r = host[k], inv = 1.0 / r, u = host[n + k] * inv, v = host[2 * n + k] * inv;
+    const double p = gm1 * (host[3 * n + k] - 0.5 * r * (u * u + v * v));
+    host[n + k] = r * u;
+    host[2 * n + k] = r * v;
+    host[3 * n + k] = p / gm1 + eps * (0.5 * r * (u * u + v * v));
+  }
+  ec_grid_desc gd{};
+  gd.nx = nx;
+  gd.ny = ny;
+  gd.dx = dx;
+  gd.dy = dy;
+  gd.gamma = gamma;
+  gd.riemann = EC_RIEMANN_HLLC;
+  gd.limiter = EC_LIMITER_MINMOD;
+  gd.spatial_order = 2;
+  HIP_OK(hipSetDevice(0));
+  ec_ctx* ctx = nullptr;
+  if (ec_create(&gd, nullptr, &ctx) != EC_OK) abort_msg("ec_create failed");
+  EC_OK_OR_DIE(ctx, ec_direct_setup(ctx, direct_case(mode), &pd, eps));
+  double* U = nullptr;
+  HIP_OK(hipMalloc(&U, 4 * n * sizeof(double)));
+  HIP_OK(hipMemcpy(U, host.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
+  ec_imex_opts io;
+  ec_imex_default_opts(&io);
+  io.picard_iterations = cfg.imex_picard_iterations;
+  io.tol = cfg.imex_solver_tol;
+  io.max_iter = cfg.imex_solver_max_iter;
+  io.restart = cfg.imex_gmres_restart;
+  io.precision = lower(cfg.imex_gmres_precision) == "mixed" ? EC_IMEX_PRECISION_MIXED : EC_IMEX_PRECISION_FP64;
+
+  int steps = 0, solver_failures = 0, nonfinite = 0, nonpositive = 0, guards = 0, its_max = 0;
+  double res_max = 0.0, rel_max = 0.0, time = 0.0;
+  const auto t_start = std::chrono::steady_clock::now();
+  while (time < cfg.stop_time - 1.0e-14 && steps < cfg.bdltv20_paper_t1_s2_max_steps) {
+    double dt = 0.0;
+    EC_OK_OR_DIE(ctx, ec_direct_dt(ctx, U, &lay, cfg.imex_cfl, cfg.bdltv20_paper_t1_s2_dt, time,
+                                   cfg.stop_time, &dt));
+    if (!(dt > 0.0) || !std::isfinite(dt)) {
+      ++nonfinite;
+      break;
+    }
+    ec_direct_diag dd{};
+    EC_OK_OR_DIE(ctx, ec_direct_step(ctx, U, &lay, time, dt, &io, &dd, nullptr));
+    solver_failures += dd.solver_failures;
+    nonfinite += dd.nonfinite_count;
+    nonpositive += dd.nonpositive_count;
+    guards += dd.guard_count;
+    its_max = std::max(its_max, dd.iterations_max);
+    res_max = std::max(res_max, dd.residual_linf_max);
+    rel_max = std::max(rel_max, dd.relative_residual_linf_max);
+    time += dt;
+    ++steps;
+  }
+  HIP_OK(hipMemcpy(host.data(), U, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+  const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  HIP_OK(hipFree(U));
+  ec_destroy(ctx);
+
+  // error norms against the exact solution at the final time (:603-668)
+  auto prim_at = [&](size_t k, double* q) {
+    const double r = host[k], inv = 1.0 / r, u = host[n + k] * inv, v = host[2 * n + k] * inv;
+    q[0] = r;
+    q[1] = u;
+    q[2] = v;
+    q[3] = gm1 * (host[3 * n + k] - eps * (0.5 * r * (u * u + v * v)));
+  };
+  double l1r = 0, l1u = 0, l1p = 0, l2r = 0, l2u = 0, l2p = 0, lir = 0, lip = 0;
+  double rho_min = INFINITY, p_min = INFINITY;
+  std::vector<double> exact(4 * n);
+  for (int j = 0; j < ny; ++j) {
+    for (int i = 0; i < nx; ++i) {
+      const size_t k = (size_t)j * nx + i;
+      const double x = cfg.prob_lo[0] + ((double)i + 0.5) * dx;
+      const double y = cfg.prob_lo[1] + ((double)j + 0.5) * dy;
+      double q[4], e[4];
+      prim_at(k, q);
+      ec_exact_state_at(&pd, cfg.is_periodic, x, y, time, e);
+      for (int c = 0; c < 4; ++c) exact[4 * k + c] = e[c];
+      const double du = std::sqrt((q[1] - e[1]) * (q[1] - e[1]) + (q[2] - e[2]) * (q[2] - e[2]));
+      const double dr = std::abs(q[0] - e[0]), dp = std::abs(q[3] - e[3]);
+      l1r += dr;
+      l1u += du;
+      l1p += dp;
+      l2r += dr * dr;
+      l2u += du * du;
+      l2p += dp * dp;
+      lir = std::max(lir, dr);
+      lip = std::max(lip, dp);
+      rho_min = std::min(rho_min, q[0]);
+      p_min = std::min(p_min, q[3]);
+    }
+  }
+  const double denom = (double)std::max<size_t>(n, 1);
+  l1r /= denom;
+  l1u /= denom;
+  l1p /= denom;
+  bool csv_ok = false;
+  if (!cfg.final_csv.empty()) {
+    std::ofstream out(cfg.final_csv);
+    out << std::setprecision(17);
+    out << "x,y,rho,u,v,p,exact_rho,exact_u,exact_v,exact_p\n";
+    for (int j = 0; j < ny; ++j) {
+      for (int i = 0; i < nx; ++i) {
+        const size_t k = (size_t)j * nx + i;
+        double q[4];
+        prim_at(k, q);
+        out << cfg.prob_lo[0] + ((double)i + 0.5) * dx << ',' << cfg.prob_lo[1] + ((double)j + 0.5) * dy
+            << ',' << q[0] << ',' << q[1] << ',' << q[2] << ',' << q[3] << ',' << exact[4 * k] << ','
+            << exact[4 * k + 1] << ',' << exact[4 * k + 2] << ',' << exact[4 * k + 3] << '\n';
+      }
+    }
+    csv_ok = static_cast<bool>(out);
+  }
+  const bool finished = time >= cfg.stop_time - 1.0e-12;
+  const bool passed = finished && solver_failures == 0 && nonfinite == 0 && nonpositive == 0;
+  const bool periodic_y = cfg.is_periodic[1] != 0;
+  std::cout << std::setprecision(17);
+  std::cout << "bdltv20_paper_t1_s2=" << mode << "\n";
+  std::cout << "bdltv20_paper_primary_source=Boscheri_Dimarco_Loubere_Tavelli_Vignal_2020\n";
+  std::cout << "bdltv20_paper_dimension=2d_z_dropped\n";
+  std::cout << "bdltv20_paper_time_order=1\n";
+  std::cout << "bdltv20_paper_space_order=2\n";
+  std::cout << "bdltv20_paper_epsilon=" << eps << "\n";
+  std::cout << "bdltv20_paper_reconstruction=section_4_2_eq87_89_conserved_minmod\n";
+  std::cout << "bdltv20_paper_explicit_update="
+               "toro_vazquez_material_flux_lax_friedrichs_star_update_eq69_total_energy_lf_jump\n";
+  std::cout << "bdltv20_paper_face_enthalpy=eq46_momentum_weighted_with_arithmetic_zero_denominator_guard\n";
+  std::cout << "bdltv20_paper_energy_closeout="
+               "eq75_pressure_internal_energy_plus_corrected_kinetic_from_solved_pressure\n";
+  std::cout << "bdltv20_paper_pressure_solver=" << cfg.bdltv20_paper_pressure_solver << "\n";
+  std::cout << "bdltv20_paper_picard_iterations=" << cfg.imex_picard_iterations << "\n";
+  std::cout << "bdltv20_paper_picard_initialization=eq53_star_momentum_old_pressure_initial_guess\n";
+  std::cout << "bdltv20_paper_pressure_topology="
+            << (x_only ? "paper_eq49_50_1d_x_riemann_row_embedded_in_2d_strip"
+                       : "paper_collocated_source_map_eq76_77_2d_zdrop") << "\n";
+  std::cout << "bdltv20_paper_control_role=bdltv20_t1s2_report_path\n";
+  std::cout << "bdltv20_paper_boundary_policy="
+            << (is_toro ? (periodic_y ? "x_exact_dirichlet_y_periodic_paper_2d_reduction"
+                                      : "xy_exact_dirichlet_paper_2d_reduction")
+                        : (is_vortex ? "xy_periodic_isentropic_vortex_paper_section_5_1"
+                                     : (is_blob ? "xy_periodic_advection_blob_project_project"
+                                                : "xy_exact_dirichlet_gresho"))) << "\n";
+  std::cout << "bdltv20_paper_timestep_policy="
+            << (cfg.bdltv20_paper_t1_s2_dt > 0.0 ? "fixed_user_dt"
+                                                  : "eq79_min_dx_over_two_velocity_magnitude_zdrop_2d") << "\n";
+  std::cout << "bdltv20_paper_steps=" << steps << "\n";
+  std::cout << "bdltv20_paper_final_time=" << time << "\n";
+  std::cout << "bdltv20_paper_solver_failure_count=" << solver_failures << "\n";
+  std::cout << "bdltv20_paper_gmres_iterations_max=" << its_max << "\n";
+  std::cout << "bdltv20_paper_pressure_residual_linf_max=" << res_max << "\n";
+  std::cout << "bdltv20_paper_pressure_relative_residual_linf_max=" << rel_max << "\n";
+  std::cout << "bdltv20_paper_matrix_stencil="
+            << (x_only ? "wide_second_neighbor_collocated_eq49_1d_x_embedded"
+                       : "wide_second_neighbor_collocated_eq76_2d_zdrop") << "\n";
+  std::cout << "bdltv20_paper_enthalpy_guard_count=" << guards << "\n";
+  std::cout << "bdltv20_paper_nonfinite_count=" << nonfinite << "\n";
+  std::cout << "bdltv20_paper_nonpositive_count=" << nonpositive << "\n";
+  std::cout << "bdltv20_paper_rho_min=" << rho_min << "\n";
+  std::cout << "bdltv20_paper_pressure_min=" << p_min << "\n";
+  std::cout << "bdltv20_paper_density_l1_error=" << l1r << "\n";
+  std::cout << "bdltv20_paper_velocity_l1_error=" << l1u << "\n";
+  std::cout << "bdltv20_paper_pressure_l1_error=" << l1p << "\n";
+  std::cout << "bdltv20_paper_density_l2_error=" << std::sqrt(l2r / denom) << "\n";
+  std::cout << "bdltv20_paper_velocity_l2_error=" << std::sqrt(l2u / denom) << "\n";
+  std::cout << "bdltv20_paper_pressure_l2_error=" << std::sqrt(l2p / denom) << "\n";
+  std::cout << "bdltv20_paper_density_linf_error=" << lir << "\n";
+  std::cout << "bdltv20_paper_pressure_linf_error=" << lip << "\n";
+  std::cout << "bdltv20_paper_final_csv=" << (cfg.final_csv.empty() ? "none" : cfg.final_csv) << "\n";
+  std::cout << "bdltv20_paper_final_csv_written=" << (csv_ok ? 1 : 0) << "\n";
+  std::cout << "bdltv20_paper_nonclaims="
+               "no_full_3d_no_second_order_time_no_ap_proof_no_production_no_gpu_cpu_efficiency\n";
+  std::cout << "bdltv20_paper_gmres_precision=" << (io.precision ? "mixed" : "fp64") << "\n";
+  std::cout << "bdltv20_paper_wall_time_sec=" << wall << "\n";
+  std::cout << "bdltv20_paper_t1_s2_status=" << (passed ? "passed" : "failed") << "\n";
+  return passed ? 0 : 2;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -367,6 +632,8 @@ int main(int argc, char** argv)
   std::vector<double> host(4 * n);
   ec_layout lay{nx, (long long)n};
   if (ec_init_state(&pd, host.data(), &lay) != EC_OK) abort_msg("Unsupported Toro test id.");
+  // app/driver_main.hpp:12-15
+  if (cfg.bdltv20_paper_t1_s2 != "off") return run_direct(cfg, pd);
 
   ec_grid_desc gd{};
   gd.nx = nx;

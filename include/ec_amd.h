@@ -378,6 +378,14 @@ ec_status ec_problem_defaults(int kind, ec_problem_desc* desc);
  * (initialize_state, problems/problem_initialization.hpp:420-435; cell
  * centres prob_lo + (i+1/2) dx with dx = (prob_hi - prob_lo)/n). */
 ec_status ec_init_state(const ec_problem_desc* desc, double* host_U, const ec_layout* layout);
+/* The problem's exact primitive state (rho, u, v, p) at (x, y) and time t:
+ * exact_riemann_sample for Toro (problems/exact_riemann.hpp:114-205),
+ * gresho_state, the isentropic vortex advected with the flow
+ * (problems/problem_state_functions.hpp:44-71), exact_advection_blob_state
+ * (problems/problem_initialization.hpp:186-207; periodic: geometry.is_periodic)
+ * and the quadrant's initial state.  Host-only. */
+ec_status ec_exact_state_at(const ec_problem_desc* problem, const int* periodic, double x, double y,
+                            double t, double* prim_out);
 /* Post-shock primitive state (rho, u, v, p) behind a shock of the given Mach
  * number moving into gas at rest (Rankine-Hugoniot). */
 ec_status ec_post_shock_state(double gamma, double mach, double rho1, double p1, double* prim_out);

@@ -109,3 +109,45 @@ def test_toro_exact_dirichlet_row(tmp_path, ecor):
     g0 = ecor.make_grid(nx, ny, 1 / nx, 0.1 / ny, riemann=ecor.XIE_HLLCP, bc=(1, 1, 1, 1))
     out0, _, _ = ecor.run_explicit(g0, ecor.init_toro(nx, ny, L, R), 0.4, 0.25, 100000)
     assert not np.array_equal(out0[0], ref[0])
+
+
+def test_direct_bdltv20_toro_row(tmp_path, ecor):
+    """A Riemann-convergence IMEX row (euler.bdltv20_paper_t1_s2=
+    toro_xy_exact_dirichlet_2d, fixed dt, 4 Picard solves): the driver's
+    report keys, final CSV schema and states against the oracle's direct
+    step run through the same loop (solves to 1e-12: states to 1e-9)."""
+    out_csv = tmp_path / "direct.csv"
+    nx, ny, t_end, nsteps = 64, 5, 0.1, 20
+    dt = t_end / nsteps
+    rc, kv, err = run(tmp_path, f"amr.n_cell={nx} {ny}", "geometry.prob_hi=1.0 0.078125",
+                      "geometry.is_periodic=0 0", "euler.problem=toro1", "prob.test=1",
+                      "euler.field_boundary=exact_dirichlet", "euler.method=imex",
+                      "euler.spatial_order=2", "euler.imex_acoustic_startup=0",
+                      "euler.bdltv20_paper_t1_s2=toro_xy_exact_dirichlet_2d",
+                      f"euler.bdltv20_paper_t1_s2_dt={dt!r}", "euler.imex_picard_iterations=4",
+                      "euler.imex_solver_tol=1e-12", "euler.imex_solver_max_iter=4000",
+                      f"stop_time={t_end!r}", f"euler.final_csv={out_csv}")
+    assert rc == 0, err
+    assert kv["bdltv20_paper_t1_s2_status"] == "passed"
+    assert int(kv["bdltv20_paper_steps"]) == nsteps
+    assert kv["bdltv20_paper_boundary_policy"] == "xy_exact_dirichlet_paper_2d_reduction"
+    L, R = [1.0, 0.0, 0.0, 1.0], [0.125, 0.0, 0.0, 0.1]
+    e = ecor.make_exact(ecor.EXACT_TORO, (0, 0), prob_hi=(1.0, 0.078125), ql=L, qr=R, interface_x=0.5)
+    g = ecor.make_grid(nx, ny, 1 / nx, 0.078125 / ny)
+    d = ecor.make_direct(e, 1.0, True)
+    U = ecor.init_toro(nx, ny, L, R)
+    t = 0.0
+    for _ in range(nsteps):
+        h = ecor.direct_dt(g, d, U, 0.8, dt, t, t_end)
+        U, _, res = ecor.direct_step(g, d, U, t, h, picard=4, tol=1e-12, max_iter=4000)
+        assert res.solver_failures == 0
+        t += h
+    rows = list(csv.DictReader(open(out_csv)))
+    assert list(rows[0].keys()) == ["x", "y", "rho", "u", "v", "p", "exact_rho", "exact_u",
+                                    "exact_v", "exact_p"]
+    rho = np.array([float(r["rho"]) for r in rows]).reshape(ny, nx)
+    assert np.abs(rho - U[0]).max() <= 1e-9 * np.abs(U[0]).max()
+    ex = np.array([ecor.exact_riemann_sample(float(r["x"]), t, L, R)[0] for r in rows[:nx]])
+    assert np.array_equal(np.array([float(r["exact_rho"]) for r in rows[:nx]]), ex)
+    l1 = np.abs(rho - ex[None, :]).mean()
+    assert float(kv["bdltv20_paper_density_l1_error"]) == pytest.approx(l1, rel=1e-9)
