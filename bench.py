@@ -177,6 +177,69 @@ def bench_imex(dev, n: int = 512, mach: float = 0.001, reps: int = 3, max_iter: 
     return out
 
 
+def bench_direct(dev, n: int = 256, mach: float = 0.01, reps: int = 3, cpu_n: int = 48) -> dict:
+    """Direct BDLTV20 T1/S2 step (the report rows' IMEX driver), Gresho n x n
+    with exact-Dirichlet boundaries at Mach `mach`, epsilon 1, 4 Picard
+    solves to 1e-8 (the settings of the reference's Gresho matrix), eq79 dt:
+    time per step on the device in FP64 and mixed form, and one step of the
+    oracle on the host at cpu_n x cpu_n beside the device at the same size."""
+    import time as _time
+    import torch
+    import ecamd
+    from ecamd import problems
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import ecor
+
+    def case(m):
+        params = dict(vortex_center=(0.5, 0.5), density_outer=1.0, mach=mach,
+                      pressure=1.0 / (1.4 * mach * mach))
+        pd = problems.problem_desc("gresho_vortex", m, m, **params)
+        U = problems.initial_state("gresho_vortex", m, m, **params)
+        s = ecamd.ExplicitSolver(ecamd.Grid(m, m, 1.0 / m, 1.0 / m, 1.4, "hllc", "minmod", 2), dev)
+        s.direct_setup("gresho_exact_dirichlet_2d", pd, 1.0)
+        return s, U, params
+
+    def timed(s, U0, dt, prec):
+        opts = s.imex_opts(precision=prec, tol=1e-8, max_iter=1000, restart=30, picard_iterations=4)
+        U = torch.from_numpy(U0).to(dev)
+        s.direct_step(U.clone(), 0.0, dt, opts)     # warm-up
+        times = []
+        for _ in range(reps):
+            V = U.clone()
+            torch.cuda.synchronize()
+            a, b = _events()
+            a.record()
+            d = s.direct_step(V, 0.0, dt, opts)
+            b.record()
+            torch.cuda.synchronize()
+            times.append(a.elapsed_time(b))
+        return sorted(times)[len(times) // 2], d
+
+    s, U0, params = case(n)
+    dt = s.direct_dt(torch.from_numpy(U0).to(dev), 0.8, -1.0, 0.0, 1.0)
+    out = {"workload": f"gresho_{n}x{n}_exact_dirichlet_M{mach:g}_direct_bdltv20_t1s2_eps1_picard4_tol1e-8",
+           "dt": dt}
+    for prec in ("fp64", "mixed"):
+        ms, d = timed(s, U0, dt, prec)
+        out[prec] = {"ms_per_step": ms, "gmres_iterations": d.iterations,
+                     "solver_failures": d.solver_failures}
+    s.close()
+    # like for like with the oracle (one host thread) at a small size
+    s, U0, params = case(cpu_n)
+    dt_c = s.direct_dt(torch.from_numpy(U0).to(dev), 0.8, -1.0, 0.0, 1.0)
+    gpu_ms, _ = timed(s, U0, dt_c, "fp64")
+    s.close()
+    e = ecor.make_exact(ecor.EXACT_GRESHO, (0, 0), vortex_center=(0.5, 0.5),
+                        density_outer=1.0, pressure=params["pressure"], mach=mach)
+    g = ecor.make_grid(cpu_n, cpu_n, 1.0 / cpu_n, 1.0 / cpu_n)
+    t0 = _time.perf_counter()
+    ecor.direct_step(g, ecor.make_direct(e, 1.0), U0, 0.0, dt_c, picard=4, tol=1e-8, max_iter=1000)
+    cpu_ms = (_time.perf_counter() - t0) * 1e3
+    out["cpu_baseline"] = {"kind": "port", "n": cpu_n, "device_fp64_ms": gpu_ms, "oracle_ms": cpu_ms,
+                           "cores": 1}
+    return out
+
+
 def bench_adjoint(dev, nx: int = 640, ny: int = 160, nsteps: int = 768, K: int = 32,
                   ndirs: int = 16, fd_eps: float = 1e-7) -> dict:
     """Checkpointed adjoint of the explicit O2 HLLC run: single-gamma shock
@@ -343,7 +406,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the IMEX, adjoint and ensemble measurements")
-    ap.add_argument("--only", choices=("imex", "adjoint", "ensemble"),
+    ap.add_argument("--only", choices=("imex", "adjoint", "ensemble", "direct"),
                     help="print only that measurement")
     ap.add_argument("--ensemble-mode", choices=("auto", "graph", "persistent"), default="auto",
                     help="ensemble rounds: autotuned, or one mode with tile shape 0")
@@ -364,6 +427,7 @@ def main() -> int:
     n = args.n
     if args.only:
         fn = {"imex": lambda: bench_imex(dev), "adjoint": lambda: bench_adjoint(dev),
+              "direct": lambda: bench_direct(dev),
               "ensemble": lambda: {r: bench_ensemble(dev, r, mode=args.ensemble_mode)
                                    for r in ("xie_am_hllc_p", "hllc")}}
         print(json.dumps({args.only: fn[args.only]()}))
@@ -450,6 +514,7 @@ def main() -> int:
     if not args.no_extras:
         line["roofline_sweep"] = roofline_sweep(dev, args.riemann, args.order, args.mach, args.cfl)
         line["imex"] = bench_imex(dev)
+        line["direct_bdltv20"] = bench_direct(dev)
         line["adjoint"] = bench_adjoint(dev)
         line["ensemble"] = {r: bench_ensemble(dev, r) for r in ("xie_am_hllc_p", "hllc")}
     if not args.no_cpu_baseline:
