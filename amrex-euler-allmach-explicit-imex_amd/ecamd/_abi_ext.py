@@ -43,6 +43,29 @@ DIRECT_CASES = {"toro_x_exact_dirichlet_y_periodic_2d": 0, "toro_xy_exact_dirich
                 "advection_blob_periodic_2d": 4}
 
 
+class AdmissibilityDiag(C.Structure):
+    _fields_ = [("rho_min", C.c_double), ("pressure_min", C.c_double),
+                ("internal_energy_min", C.c_double), ("nonfinite_count", C.c_int),
+                ("density_failure_count", C.c_int), ("pressure_failure_count", C.c_int),
+                ("internal_energy_failure_count", C.c_int)]
+
+
+class GreshoMetricsOut(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "density_l1_error", "velocity_l1_error", "pressure_l1_error",
+        "pressure_perturbation_l1_error", "pressure_perturbation_l1_relative_error",
+        "kinetic_energy_initial", "kinetic_energy_final", "kinetic_energy_ratio",
+        "reference_sound_speed", "velocity_scale")]
+
+
+class ShockBubbleMetricsOut(C.Structure):
+    _fields_ = [("rho_min", C.c_double), ("rho_max", C.c_double), ("pressure_min", C.c_double),
+                ("pressure_max", C.c_double), ("internal_energy_min", C.c_double),
+                ("nonfinite_count", C.c_int), ("density_bubble_area_threshold", C.c_double),
+                ("density_bubble_area", C.c_double), ("density_bubble_centroid_x", C.c_double),
+                ("density_bubble_centroid_y", C.c_double), ("schlieren_max", C.c_double)]
+
+
 class MemberDesc(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
                 ("bc", C.c_int * 4), ("fixed_state", (C.c_double * 4) * 4), ("cfl", C.c_double),
@@ -95,6 +118,17 @@ def declare(L) -> None:
     L.ec_imex_dt.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.c_double, C.c_double, C.c_int,
                              C.c_double, dp, dp, dp]
     L.ec_imex_dt.restype = st
+    L.ec_admissibility.argtypes = [vp, vp, C.POINTER(Layout), C.POINTER(AdmissibilityDiag)]
+    L.ec_admissibility.restype = st
+    L.ec_gresho_metrics.argtypes = [vp, vp, C.POINTER(Layout), C.POINTER(ProblemDesc),
+                                    C.POINTER(GreshoMetricsOut)]
+    L.ec_gresho_metrics.restype = st
+    L.ec_shock_bubble_metrics.argtypes = [vp, vp, C.POINTER(Layout), dp, C.c_double,
+                                          C.POINTER(ShockBubbleMetricsOut)]
+    L.ec_shock_bubble_metrics.restype = st
+    L.ec_exact_state_at.argtypes = [C.POINTER(ProblemDesc), C.POINTER(C.c_int), C.c_double,
+                                    C.c_double, C.c_double, dp]
+    L.ec_exact_state_at.restype = st
     L.ec_direct_setup.argtypes = [vp, C.c_int, C.POINTER(ProblemDesc), C.c_double]
     L.ec_direct_setup.restype = st
     L.ec_direct_step.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.c_double,

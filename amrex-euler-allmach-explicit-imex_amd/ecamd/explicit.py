@@ -269,6 +269,36 @@ class ExplicitSolver:
               "ec_direct_dt")
         return dt.value
 
+    # ---- whole-field metrics ---------------------------------------------
+    def admissibility(self, U: torch.Tensor):
+        """compute_admissibility_diagnostics (numerics/diagnostic_metrics_timestep.hpp:49-102)."""
+        from ._abi_ext import AdmissibilityDiag
+        self._check_state(U)
+        d = AdmissibilityDiag()
+        check(self._ctx, lib().ec_admissibility(self._ctx, _ptr(U), C.byref(self.layout), C.byref(d)),
+              "ec_admissibility")
+        return d
+
+    def gresho_metrics(self, U: torch.Tensor, problem):
+        """compute_gresho_metrics (numerics/diagnostic_metrics_timestep.hpp:147-225)."""
+        from ._abi_ext import GreshoMetricsOut
+        self._check_state(U)
+        m = GreshoMetricsOut()
+        check(self._ctx, lib().ec_gresho_metrics(self._ctx, _ptr(U), C.byref(self.layout),
+                                                  C.byref(problem), C.byref(m)), "ec_gresho_metrics")
+        return m
+
+    def shock_bubble_metrics(self, U: torch.Tensor, prob_lo=(0.0, 0.0), threshold: float = 0.55):
+        """compute_shock_density_bubble_metrics (io/output_cases.hpp:329-407)."""
+        from ._abi_ext import ShockBubbleMetricsOut
+        self._check_state(U)
+        m = ShockBubbleMetricsOut()
+        lo = (C.c_double * 2)(*prob_lo)
+        check(self._ctx, lib().ec_shock_bubble_metrics(self._ctx, _ptr(U), C.byref(self.layout), lo,
+                                                        float(threshold), C.byref(m)),
+              "ec_shock_bubble_metrics")
+        return m
+
     def imex_dt(self, U: torch.Tensor, cfl: float, time: float, stop_time: float,
                 acoustic_startup: int = 0, acoustic_cfl_cap: float = 0.0):
         self._check_state(U)

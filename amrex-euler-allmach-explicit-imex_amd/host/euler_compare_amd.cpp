@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -352,6 +353,7 @@ Cfg read_config(const Params& pp)
   return c;
 }
 
+// parse_shock_density_bubble_snapshot_times (io/output_cases.hpp:276-305)
 std::vector<double> parse_times(const std::string& s, double stop)
 {
   std::vector<double> t;
@@ -360,11 +362,97 @@ std::vector<double> parse_times(const std::string& s, double stop)
   while (std::getline(in, tok, ',')) {
     tok = Params::trim(tok);
     if (tok.empty()) continue;
-    const double v = std::atof(tok.c_str());
-    if (v >= 0.0 && v <= stop + 1.0e-13) t.push_back(v);
+    const double v = std::stod(tok);
+    if (!std::isfinite(v) || v < -1.0e-14 || v > stop + 1.0e-14) {
+      abort_msg("shock_density_bubble_2d snapshot_times must lie in [0, final_time].");
+    }
+    t.push_back(std::max(0.0, std::min(v, stop)));
   }
   std::sort(t.begin(), t.end());
+  t.erase(std::unique(t.begin(), t.end(), [](double a, double b) { return std::abs(a - b) < 1.0e-13; }),
+          t.end());
   return t;
+}
+
+// Shock-bubble labels (io/output_cases.hpp:306-327, problems/problem_initialization.hpp:37-49)
+const char* kSbCaseId = "same_gamma_shock_density_bubble_clawpack_style_v1";
+const char* kSbClaim = "qualitative_project_high_speed_2d_only";
+
+std::string sb_scheme_name(const Cfg& c)
+{
+  if (c.method == "imex") return "imex_t1s2_bdltv20_ideal_gas";
+  if (c.riemann_kind == EC_RIEMANN_XIE_AM_HLLC_P) return "explicit_o2_low_mach_corrected_hllc_p";
+  if (c.riemann_kind == EC_RIEMANN_HLLC) return "explicit_o2_hllc";
+  return std::string("explicit_o") + std::to_string(c.spatial_order) + "_" + riemann_name(c.riemann_kind);
+}
+
+std::string sb_order_label(const Cfg& c)
+{
+  return c.method == "imex" ? "T1/S2" : std::string("O") + std::to_string(c.spatial_order);
+}
+
+std::string csv_escape(const std::string& v)
+{
+  if (v.find_first_of(",\"\n") == std::string::npos) return v;
+  std::string e = "\"";
+  for (const char ch : v) e += ch == '"' ? std::string("\"\"") : std::string(1, ch);
+  return e + "\"";
+}
+
+// write_shock_density_bubble_snapshot (io/output_cases.hpp:408-508) from a
+// host copy of the state; neighbour densities by the shock-bubble ghost rule
+// (fixed inflow left, outflow right and top, mirror bottom).
+bool write_sb_snapshot(const Cfg& c, const std::vector<double>& U, double dx, double dy, int step,
+                       double time, int index)
+{
+  if (c.snapshot_dir.empty()) return true;
+  std::error_code ec;
+  std::filesystem::create_directories(c.snapshot_dir, ec);
+  if (ec) return false;
+  std::ostringstream ts;
+  ts << std::fixed << std::setprecision(6) << time;
+  std::string tl = ts.str();
+  std::replace(tl.begin(), tl.end(), '.', 'p');
+  std::ostringstream name;
+  name << c.snapshot_dir << "/snapshot_" << std::setw(2) << std::setfill('0') << index << "_t" << tl << "_step"
+       << std::setw(6) << std::setfill('0') << step << ".csv";
+  std::ofstream out(name.str());
+  if (!out) return false;
+  const int nx = c.n_cell[0], ny = c.n_cell[1];
+  const size_t n = (size_t)nx * ny;
+  const double post_rho = 2.8181818181818183, threshold = 0.55, g = c.gamma;
+  auto rho_at = [&](int i, int j) {
+    if (i < 0) return post_rho;
+    if (i > nx - 1) i = nx - 1;
+    if (j < 0) j = -j - 1;
+    if (j > ny - 1) j = ny - 1;
+    return U[(size_t)j * nx + i];
+  };
+  out << std::setprecision(17);
+  out << "source_case_id,step,time,i,j,x,y,rho,u,v,pressure,internal_energy,"
+         "schlieren,density_bubble_indicator,density_bubble_area_threshold,gamma,"
+         "method,scheme_name,riemann,spatial_order,validation_claim,gfm_used,"
+         "level_set_used,material_count,cylindrical_source_used,"
+         "geometric_source_form\n";
+  const std::string scheme = sb_scheme_name(c);
+  const char* rl = c.method == "imex" ? "imex_pressure_split" : riemann_name(c.riemann_kind);
+  for (int j = 0; j < ny; ++j) {
+    for (int i = 0; i < nx; ++i) {
+      const size_t k = (size_t)j * nx + i;
+      const double r = U[k], mx = U[n + k], my = U[2 * n + k], E = U[3 * n + k];
+      const double inv = 1.0 / r, u = mx * inv, v = my * inv;
+      const double p = (g - 1.0) * (E - 0.5 * r * (u * u + v * v));
+      const double internal = E - 0.5 * (mx * mx + my * my) / r;
+      const double rx = (rho_at(i + 1, j) - rho_at(i - 1, j)) / (2.0 * dx);
+      const double ry = (rho_at(i, j + 1) - rho_at(i, j - 1)) / (2.0 * dy);
+      out << kSbCaseId << ',' << step << ',' << time << ',' << i << ',' << j << ','
+          << c.prob_lo[0] + ((double)i + 0.5) * dx << ',' << c.prob_lo[1] + ((double)j + 0.5) * dy << ','
+          << r << ',' << u << ',' << v << ',' << p << ',' << internal << ',' << std::sqrt(rx * rx + ry * ry)
+          << ',' << (r < threshold ? 1 : 0) << ',' << threshold << ',' << g << ',' << c.method << ','
+          << scheme << ',' << rl << ',' << c.spatial_order << ',' << kSbClaim << ",false,false,1,false,none\n";
+    }
+  }
+  return static_cast<bool>(out);
 }
 
 #define HIP_OK(x)                                                          \
@@ -707,9 +795,33 @@ int main(int argc, char** argv)
   }
   std::vector<double> snaps = sb ? parse_times(cfg.snapshot_times, cfg.stop_time) : std::vector<double>{};
   size_t next_snap = 0;
-  while (next_snap < snaps.size() && snaps[next_snap] <= time + 1.0e-13) ++next_snap;
+  int snaps_written = 0;
+  bool snap_failed = false;
+  double dt_min_run = INFINITY, dt_max_run = -INFINITY, dt_sum_run = 0.0;
+  int dt_count_run = 0;
+  // app/driver_main.hpp:192-211 (the initial state is still in host)
+  auto take_snapshots = [&](bool download) {
+    bool have = !download;
+    while (next_snap < snaps.size() && snaps[next_snap] <= time + 1.0e-13) {
+      if (!have) {
+        HIP_OK(hipMemcpy(host.data(), A, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+        have = true;
+      }
+      if (!write_sb_snapshot(cfg, host, dx, dy, step, download ? snaps[next_snap] : time, (int)next_snap)) {
+        snap_failed = true;
+        status = "failed";
+        failure = "snapshot_write_failed";
+        break;
+      }
+      ++snaps_written;
+      ++next_snap;
+    }
+  };
+  if (sb) take_snapshots(false);
   ec_imex_diag last_imex{};
   int imex_iter_max = 0;
+  // update_high_trial_run over the IMEX steps (numerics/diagnostic_metrics_timestep.hpp:122-131)
+  double adm_rho_min = INFINITY, adm_p_min = INFINITY, adm_e_min = INFINITY;
   double imex_res_max = 0.0, imex_rel_max = 0.0, mat_max_run = 0.0, aco_max_run = 0.0;
   ec_imex_opts io;
   ec_imex_default_opts(&io);
@@ -739,6 +851,10 @@ int main(int argc, char** argv)
     step = rs.step;
     time = rs.time;
     degenerate = rs.degenerate_total;
+    dt_count_run = rs.step;
+    dt_min_run = rs.dt_min;
+    dt_max_run = rs.dt_max;
+    dt_sum_run = rs.dt_sum;
     diag.rho_min = rs.rho_min;
     diag.pressure_min = rs.pressure_min;
     diag.nonfinite_count = rs.nonfinite_count;
@@ -783,6 +899,12 @@ int main(int argc, char** argv)
                        : std::string("imex_linear_solver_") + imex_status_name(last_imex.status);
           break;
         }
+        // app/driver_main.hpp:278-282
+        ec_admissibility_diag ad{};
+        EC_OK_OR_DIE(ctx, ec_admissibility(ctx, A, &lay, &ad));
+        adm_rho_min = std::min(adm_rho_min, ad.rho_min);
+        adm_p_min = std::min(adm_p_min, ad.pressure_min);
+        adm_e_min = std::min(adm_e_min, ad.internal_energy_min);
       } else {
         ec_step_diag sd{};
         EC_OK_OR_DIE(ctx, ec_set_boundary_time(ctx, time));
@@ -790,6 +912,10 @@ int main(int argc, char** argv)
         std::swap(A, B);
         degenerate += sd.degenerate_count;
       }
+      dt_min_run = std::min(dt_min_run, dt);
+      dt_max_run = std::max(dt_max_run, dt);
+      dt_sum_run += dt;
+      ++dt_count_run;
       time += dt;
       ++step;
       EC_OK_OR_DIE(ctx, ec_state_diagnostics(ctx, A, &lay, &diag));
@@ -803,7 +929,7 @@ int main(int argc, char** argv)
         status = "failed";
         failure = "nonpositive_pressure";
       }
-      while (next_snap < snaps.size() && snaps[next_snap] <= time + 1.0e-13) ++next_snap;
+      if (sb && status == "ok" && !snap_failed) take_snapshots(true);
     }
   }
   if (status == "ok" && step >= cfg.max_step && time < cfg.stop_time - 1.0e-14) {
@@ -816,11 +942,14 @@ int main(int argc, char** argv)
   double m1, e1;
   sums(host, m1, e1);
 
-  // final CSVs (io/output_cases.hpp:16-107)
-  if (!cfg.final_csv.empty() && (cfg.problem_kind == EC_PROBLEM_TORO || cfg.problem_kind == EC_PROBLEM_GRESHO)) {
+  // final CSVs (io/output_cases.hpp:16-211)
+  if (!cfg.final_csv.empty() &&
+      (cfg.problem_kind == EC_PROBLEM_TORO || cfg.problem_kind == EC_PROBLEM_GRESHO ||
+       cfg.problem_kind == EC_PROBLEM_RIEMANN_QUADRANT || cfg.problem_kind == EC_PROBLEM_ADVECTION_BLOB)) {
     std::ofstream out(cfg.final_csv);
     out << std::setprecision(17);
     const double g = cfg.gamma;
+    const bool imex_run = cfg.method == "imex";
     auto prim = [&](size_t k, double* q) {
       const double inv = 1.0 / host[k];
       q[0] = host[k];
@@ -829,25 +958,84 @@ int main(int argc, char** argv)
       const double kin = 0.5 * host[k] * (q[1] * q[1] + q[2] * q[2]);
       q[3] = (g - 1.0) * (host[3 * n + k] - kin);
     };
+    auto xc = [&](int i) { return cfg.prob_lo[0] + ((double)i + 0.5) * dx; };
+    auto yc = [&](int j) { return cfg.prob_lo[1] + ((double)j + 0.5) * dy; };
+    const char* riemann_label = imex_run ? "not_applicable" : riemann_name(cfg.riemann_kind);
+    const char* riemann_label_2 = imex_run ? "imex_pressure_split" : riemann_name(cfg.riemann_kind);
     if (cfg.problem_kind == EC_PROBLEM_TORO) {
       out << "i,j,x,y,rho,u,v,p,exact_rho,exact_u,exact_v,exact_p,rho_error,u_error,v_error,p_error\n";
       for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) {
           double q[4];
           prim((size_t)j * nx + i, q);
-          out << i << ',' << j << ',' << cfg.prob_lo[0] + (i + 0.5) * dx << ',' << cfg.prob_lo[1] + (j + 0.5) * dy
-              << ',' << q[0] << ',' << q[1] << ',' << q[2] << ',' << q[3]
-              << ",nan,nan,nan,nan,nan,nan,nan,nan\n";
+          out << i << ',' << j << ',' << xc(i) << ',' << yc(j) << ',' << q[0] << ',' << q[1] << ',' << q[2]
+              << ',' << q[3] << ",nan,nan,nan,nan,nan,nan,nan,nan\n";
+        }
+      }
+    } else if (cfg.problem_kind == EC_PROBLEM_GRESHO) {
+      // the run minima of the IMEX admissibility checks, NaN without IMEX steps
+      const bool have = imex_run && imex_steps > 0;
+      const double nan = std::nan("");
+      out << "x,y,rho,u,v,pressure,exact_rho,exact_u,exact_v,exact_pressure,"
+          << "density_error,velocity_error,pressure_error,pressure_perturbation_error,"
+          << "time,step,mach,method,riemann,imex_form,imex_predictor_flux,imex_predictor_dissipation,"
+          << "spatial_order,slope_limiter,"
+          << "imex_trial_density_run_min,"
+          << "imex_high_trial_pressure_run_min,imex_high_trial_internal_energy_run_min\n";
+      for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+          double q[4], e[4];
+          prim((size_t)j * nx + i, q);
+          ec_exact_state_at(&pd, cfg.is_periodic, xc(i), yc(j), time, e);
+          const double du = q[1] - e[1], dv = q[2] - e[2];
+          out << xc(i) << ',' << yc(j) << ',' << q[0] << ',' << q[1] << ',' << q[2] << ',' << q[3] << ','
+              << e[0] << ',' << e[1] << ',' << e[2] << ',' << e[3] << ',' << q[0] - e[0] << ','
+              << std::sqrt(du * du + dv * dv) << ',' << q[3] - e[3] << ','
+              << (q[3] - cfg.pressure) - (e[3] - cfg.pressure) << ',' << time << ',' << step << ','
+              << cfg.mach << ',' << cfg.method << ',' << riemann_label << ','
+              << (imex_run ? "bdltv20_t1_s2_source_map_picard" : "not_applicable") << ','
+              << (imex_run ? "bdltv20_pressure_split_lf" : "not_applicable") << ','
+              << (imex_run ? lower(cfg.imex_predictor_dissipation).c_str() : "not_applicable") << ','
+              << cfg.spatial_order << ',' << limiter_name(cfg.limiter_kind) << ','
+              << (have ? adm_rho_min : nan) << ',' << (have ? adm_p_min : nan) << ','
+              << (have ? adm_e_min : nan) << '\n';
+        }
+      }
+    } else if (cfg.problem_kind == EC_PROBLEM_RIEMANN_QUADRANT) {
+      out << "i,j,x,y,rho,u,v,p,initial_rho,initial_u,initial_v,initial_p,"
+             "time,step,method,riemann,spatial_order,slope_limiter,"
+             "field_boundary,geometry_is_periodic_x,geometry_is_periodic_y,"
+             "quadrant_case,claim_limit\n";
+      for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+          double q[4], e[4];
+          prim((size_t)j * nx + i, q);
+          ec_exact_state_at(&pd, cfg.is_periodic, xc(i), yc(j), 0.0, e);
+          out << i << ',' << j << ',' << xc(i) << ',' << yc(j) << ',' << q[0] << ',' << q[1] << ',' << q[2]
+              << ',' << q[3] << ',' << e[0] << ',' << e[1] << ',' << e[2] << ',' << e[3] << ',' << time << ','
+              << step << ',' << cfg.method << ',' << riemann_label_2 << ',' << cfg.spatial_order << ','
+              << limiter_name(cfg.limiter_kind) << ',' << cfg.field_boundary << ','
+              << (cfg.is_periodic[0] ? 1 : 0) << ',' << (cfg.is_periodic[1] ? 1 : 0) << ','
+              << "liska_wendroff_clawpack_quadrant,"
+              << "genuine_2d_visual_finite_smoke_no_exact_solution\n";
         }
       }
     } else {
-      out << "x,y,rho,u,v,pressure\n";
+      out << "x,y,rho,u,v,pressure,exact_rho,exact_u,exact_v,exact_pressure,"
+          << "density_error,velocity_error,pressure_error,time,step,method,riemann,"
+          << "spatial_order,slope_limiter,geometry_is_periodic_x,geometry_is_periodic_y\n";
       for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) {
-          double q[4];
+          double q[4], e[4];
           prim((size_t)j * nx + i, q);
-          out << cfg.prob_lo[0] + (i + 0.5) * dx << ',' << cfg.prob_lo[1] + (j + 0.5) * dy << ',' << q[0]
-              << ',' << q[1] << ',' << q[2] << ',' << q[3] << '\n';
+          ec_exact_state_at(&pd, cfg.is_periodic, xc(i), yc(j), time, e);
+          const double du = q[1] - e[1], dv = q[2] - e[2];
+          out << xc(i) << ',' << yc(j) << ',' << q[0] << ',' << q[1] << ',' << q[2] << ',' << q[3] << ','
+              << e[0] << ',' << e[1] << ',' << e[2] << ',' << e[3] << ',' << q[0] - e[0] << ','
+              << std::sqrt(du * du + dv * dv) << ',' << q[3] - e[3] << ',' << time << ',' << step << ','
+              << cfg.method << ',' << riemann_label_2 << ',' << cfg.spatial_order << ','
+              << limiter_name(cfg.limiter_kind) << ',' << (cfg.is_periodic[0] ? 1 : 0) << ','
+              << (cfg.is_periodic[1] ? 1 : 0) << '\n';
         }
       }
     }
@@ -895,6 +1083,100 @@ int main(int argc, char** argv)
     std::cout << "imex_solver_relative_residual_max=" << imex_rel_max << "\n";
     std::cout << "imex_gmres_precision=" << (io.precision ? "mixed" : "fp64") << "\n";
     std::cout << "imex_picard_pressure_solves_used=" << last_imex.picard_solves << "\n";
+  }
+  // app/driver_main.hpp:550-571 and :609-620: the problem metrics of the final state
+  if (cfg.problem_kind == EC_PROBLEM_GRESHO) {
+    ec_gresho_metrics_out gm{};
+    EC_OK_OR_DIE(ctx, ec_gresho_metrics(ctx, A, &lay, &pd, &gm));
+    std::cout << "density_l1_error=" << gm.density_l1_error << "\n";
+    std::cout << "velocity_l1_error=" << gm.velocity_l1_error << "\n";
+    std::cout << "pressure_l1_error=" << gm.pressure_l1_error << "\n";
+    std::cout << "pressure_perturbation_l1_error=" << gm.pressure_perturbation_l1_error << "\n";
+    std::cout << "pressure_perturbation_l1_relative_error=" << gm.pressure_perturbation_l1_relative_error
+              << "\n";
+    std::cout << "kinetic_energy_initial=" << gm.kinetic_energy_initial << "\n";
+    std::cout << "kinetic_energy_final=" << gm.kinetic_energy_final << "\n";
+    std::cout << "kinetic_energy_ratio=" << gm.kinetic_energy_ratio << "\n";
+    std::cout << "reference_sound_speed=" << gm.reference_sound_speed << "\n";
+    std::cout << "velocity_scale=" << gm.velocity_scale << "\n";
+  }
+  if (sb) {
+    ec_shock_bubble_metrics_out sm{};
+    EC_OK_OR_DIE(ctx, ec_shock_bubble_metrics(ctx, A, &lay, cfg.prob_lo, 0.55, &sm));
+    // app/driver_main.hpp:375-398
+    std::string snap_status;
+    if (snap_failed) snap_status = "snapshot_write_failed";
+    else if (cfg.snapshot_dir.empty()) snap_status = "not_requested";
+    else if (snaps_written == (int)snaps.size()) snap_status = "ok";
+    else snap_status = "missing_requested_snapshot";
+    const double nan = std::nan("");
+    const double dt_mean = dt_count_run > 0 ? dt_sum_run / dt_count_run : nan;
+    const double dt_lo = dt_count_run > 0 ? dt_min_run : nan, dt_hi = dt_count_run > 0 ? dt_max_run : nan;
+    const std::string scheme = sb_scheme_name(cfg), order = sb_order_label(cfg);
+    const char* rl = cfg.method == "imex" ? "imex_pressure_split" : riemann_name(cfg.riemann_kind);
+    if (!cfg.final_csv.empty()) {
+      // write_shock_density_bubble_summary_csv (io/output_cases.hpp:510-592)
+      std::ofstream out(cfg.final_csv);
+      out << std::setprecision(17);
+      out << "source_case_id,run_id,backend,scheme_family,scheme_name,order,riemann,"
+             "gamma,grid_nx,grid_ny,x_lower,x_upper,y_lower,y_upper,shock_initial_x,"
+             "post_shock_density,post_shock_velocity,post_shock_pressure,bubble_density,"
+             "bubble_center_x,bubble_center_y,bubble_radius,boundary_left,boundary_right,"
+             "boundary_bottom,boundary_top,final_time,snapshot_times,status,"
+             "failure_category,steps_accepted,steps_rejected,wall_time_sec,dt_min,dt_max,"
+             "dt_mean,rho_min,rho_max,pressure_min,pressure_max,internal_energy_min,"
+             "nonfinite_count,density_bubble_centroid_x,density_bubble_centroid_y,"
+             "density_bubble_area_threshold,schlieren_max,snapshot_status,snapshot_dir,"
+             "validation_claim,gfm_used,level_set_used,material_count,"
+             "cylindrical_source_used,geometric_source_form\n";
+      const std::string run_id = scheme + "_" + std::to_string(nx) + "x" + std::to_string(ny);
+      out << kSbCaseId << ',' << csv_escape(run_id) << ",amrex_single_level," << cfg.method << ','
+          << csv_escape(scheme) << ',' << csv_escape(order) << ',' << rl << ',' << cfg.gamma << ',' << nx << ','
+          << ny << ',' << cfg.prob_lo[0] << ',' << cfg.prob_hi[0] << ',' << cfg.prob_lo[1] << ','
+          << cfg.prob_hi[1] << ',' << 0.2 << ',' << 2.8181818181818183 << ',' << 1.6064386578049976 << ','
+          << 5.0 << ',' << 0.1 << ',' << 0.5 << ',' << 0.0 << ',' << 0.2
+          << ",fixed_post_shock_inflow,outflow_zero_gradient,reflective_symmetry,"
+          << "outflow_zero_gradient," << cfg.stop_time << ',' << csv_escape(cfg.snapshot_times) << ','
+          << status << ',' << failure << ',' << step << ',' << 0 << ',' << wall << ',' << dt_lo << ','
+          << dt_hi << ',' << dt_mean << ',' << sm.rho_min << ',' << sm.rho_max << ',' << sm.pressure_min
+          << ',' << sm.pressure_max << ',' << sm.internal_energy_min << ',' << sm.nonfinite_count << ','
+          << sm.density_bubble_centroid_x << ',' << sm.density_bubble_centroid_y << ','
+          << sm.density_bubble_area_threshold << ',' << sm.schlieren_max << ',' << snap_status << ','
+          << csv_escape(cfg.snapshot_dir) << ',' << kSbClaim << ",false,false,1,false,none\n";
+    }
+    std::cout << "source_case_id=" << kSbCaseId << "\n";
+    std::cout << "scheme_name=" << scheme << "\n";
+    std::cout << "order=" << order << "\n";
+    std::cout << "gamma=" << cfg.gamma << "\n";
+    std::cout << "grid_nx=" << nx << "\n";
+    std::cout << "grid_ny=" << ny << "\n";
+    std::cout << "shock_initial_x=" << 0.2 << "\n";
+    std::cout << "post_shock_density=" << 2.8181818181818183 << "\n";
+    std::cout << "post_shock_velocity=" << 1.6064386578049976 << "\n";
+    std::cout << "post_shock_pressure=" << 5.0 << "\n";
+    std::cout << "bubble_density=0.1\n";
+    std::cout << "bubble_center_x=" << 0.5 << "\n";
+    std::cout << "bubble_center_y=" << 0.0 << "\n";
+    std::cout << "bubble_radius=" << 0.2 << "\n";
+    std::cout << "boundary_left=fixed_post_shock_inflow\n";
+    std::cout << "boundary_right=outflow_zero_gradient\n";
+    std::cout << "boundary_bottom=reflective_symmetry\n";
+    std::cout << "boundary_top=outflow_zero_gradient\n";
+    std::cout << "snapshot_times=" << cfg.snapshot_times << "\n";
+    std::cout << "snapshot_status=" << snap_status << "\n";
+    std::cout << "snapshot_dir=" << cfg.snapshot_dir << "\n";
+    std::cout << "snapshots_written=" << snaps_written << "\n";
+    std::cout << "density_bubble_centroid_x=" << sm.density_bubble_centroid_x << "\n";
+    std::cout << "density_bubble_centroid_y=" << sm.density_bubble_centroid_y << "\n";
+    std::cout << "density_bubble_area_threshold=" << sm.density_bubble_area_threshold << "\n";
+    std::cout << "schlieren_max=" << sm.schlieren_max << "\n";
+    std::cout << "internal_energy_min=" << sm.internal_energy_min << "\n";
+    std::cout << "validation_claim=" << kSbClaim << "\n";
+    std::cout << "gfm_used=false\n";
+    std::cout << "level_set_used=false\n";
+    std::cout << "material_count=1\n";
+    std::cout << "cylindrical_source_used=false\n";
+    std::cout << "geometric_source_form=none\n";
   }
   HIP_OK(hipFree(A));
   HIP_OK(hipFree(B));

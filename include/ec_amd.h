@@ -462,6 +462,44 @@ ec_status ec_direct_step(ec_ctx* ctx, double* U, const ec_layout* layout, double
 ec_status ec_direct_dt(ec_ctx* ctx, const double* U, const ec_layout* layout, double imex_cfl,
                        double fixed_dt, double time, double stop_time, double* dt);
 
+/* ------------------------------------------------ whole-field metrics */
+/* compute_admissibility_diagnostics (numerics/diagnostic_metrics_timestep.hpp:
+ * 49-102): minima of rho, p and internal energy (-inf where a cell fails) and
+ * the failure counts, as the IMEX loop takes them after every step
+ * (app/driver_main.hpp:278-282). */
+typedef struct {
+  double rho_min, pressure_min, internal_energy_min;
+  int nonfinite_count, density_failure_count, pressure_failure_count, internal_energy_failure_count;
+} ec_admissibility_diag;
+ec_status ec_admissibility(ec_ctx* ctx, const double* U, const ec_layout* layout,
+                           ec_admissibility_diag* out);
+
+/* compute_gresho_metrics (numerics/diagnostic_metrics_timestep.hpp:147-225)
+ * against the Gresho vortex of problem (kind GRESHO, the context's grid). */
+typedef struct {
+  double density_l1_error, velocity_l1_error, pressure_l1_error;
+  double pressure_perturbation_l1_error, pressure_perturbation_l1_relative_error;
+  double kinetic_energy_initial, kinetic_energy_final, kinetic_energy_ratio;
+  double reference_sound_speed, velocity_scale;
+} ec_gresho_metrics_out;
+ec_status ec_gresho_metrics(ec_ctx* ctx, const double* U, const ec_layout* layout,
+                            const ec_problem_desc* problem, ec_gresho_metrics_out* out);
+
+/* compute_shock_density_bubble_metrics (io/output_cases.hpp:329-407): extrema,
+ * the area and centroid of rho < threshold (the reference uses 0.55) and the
+ * largest density gradient, the neighbours read through the context's ghost
+ * rule.  prob_lo: the domain's low corner (2 doubles). */
+typedef struct {
+  double rho_min, rho_max, pressure_min, pressure_max, internal_energy_min;
+  int nonfinite_count;
+  double density_bubble_area_threshold, density_bubble_area;
+  double density_bubble_centroid_x, density_bubble_centroid_y;
+  double schlieren_max;
+} ec_shock_bubble_metrics_out;
+ec_status ec_shock_bubble_metrics(ec_ctx* ctx, const double* U, const ec_layout* layout,
+                                  const double* prob_lo, double threshold,
+                                  ec_shock_bubble_metrics_out* out);
+
 #ifdef __cplusplus
 }
 #endif

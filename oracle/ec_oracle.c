@@ -800,3 +800,135 @@ void ecor_exact_riemann_sample(double x, double t, const double* L, const double
   const double a = 2.0 / gp1 * (a_r - 0.5 * gm1 * (R[1] - xi));
   set4(q, R[0] * pow(a / a_r, 2.0 / gm1), u, R[2], R[3] * pow(a / a_r, 2.0 * gamma / gm1));
 }
+
+/* ------------------------------------------------------------------------ */
+/* compute_admissibility_diagnostics (diagnostic_metrics_timestep.hpp:49-102),
+ * serially in cell order.  out: rho_min, p_min, ie_min; counts: nonfinite,
+ * density, pressure, internal energy failures. */
+void ecor_admissibility(const ecor_grid* g, const double* U, double* out, int* counts)
+{
+  const long n = (long)g->nx * g->ny;
+  double mins[3] = {INFINITY, INFINITY, INFINITY};
+  int c[4] = {0, 0, 0, 0};
+  for (long k = 0; k < n; ++k) {
+    const double r = U[k], mx = U[n + k], my = U[2 * n + k], E = U[3 * n + k];
+    const int cons_bad = !isfinite(r) || !isfinite(mx) || !isfinite(my) || !isfinite(E);
+    const int density_bad = !cons_bad && r <= 0.0;
+    double ie = -INFINITY, p = -INFINITY;
+    int thermo_bad = 0;
+    if (!cons_bad && !density_bad) {
+      const double kinetic = 0.5 * (mx * mx + my * my) / r;
+      ie = E - kinetic;
+      p = (g->gamma - 1.0) * ie;
+      thermo_bad = !isfinite(kinetic) || !isfinite(ie) || !isfinite(p);
+    }
+    const int bad = cons_bad || thermo_bad;
+    const double v[3] = {bad ? -INFINITY : r, (bad || density_bad) ? -INFINITY : p,
+                         (bad || density_bad) ? -INFINITY : ie};
+    for (int q = 0; q < 3; ++q) mins[q] = smin(mins[q], v[q]);
+    c[0] += bad;
+    c[1] += density_bad;
+    c[2] += !bad && !density_bad && p <= 0.0;
+    c[3] += !bad && !density_bad && ie <= 0.0;
+  }
+  memcpy(out, mins, sizeof mins);
+  memcpy(counts, c, sizeof c);
+}
+
+/* compute_gresho_metrics (diagnostic_metrics_timestep.hpp:147-225).  out[10]:
+ * density, velocity, pressure L1 errors, pressure-perturbation L1 error and
+ * its relative form, kinetic energy initial / final / ratio, reference sound
+ * speed, velocity scale. */
+void ecor_gresho_metrics(const ecor_grid* g, const double* U, double xlo, double ylo, double cx,
+                         double cy, double rho, double mach, double p0, double* out)
+{
+  const int nx = g->nx, ny = g->ny;
+  const long n = (long)nx * ny;
+  const double vol = g->dx * g->dy;
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < ny; ++j) {
+    for (int i = 0; i < nx; ++i) {
+      const long k = (long)j * nx + i;
+      const double x = xlo + ((double)i + 0.5) * g->dx, y = ylo + ((double)j + 0.5) * g->dy;
+      const double u[4] = {U[k], U[n + k], U[2 * n + k], U[3 * n + k]};
+      double q[4], e[4];
+      ecor_to_primitive(u, g->gamma, q);
+      gresho_prim(x, y, g->gamma, cx, cy, rho, mach, p0, e);
+      const double du = q[1] - e[1], dv = q[2] - e[2];
+      s[0] += fabs(q[0] - e[0]);
+      s[1] += sqrt(du * du + dv * dv);
+      s[2] += fabs(q[3] - e[3]);
+      s[3] += fabs((q[3] - p0) - (e[3] - p0));
+      s[4] += fabs(e[3] - p0);
+      s[5] += 0.5 * e[0] * (e[1] * e[1] + e[2] * e[2]) * vol;
+      s[6] += 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2]) * vol;
+    }
+  }
+  const double inv = 1.0 / (double)(nx * ny);
+  const double c0 = sqrt(g->gamma * p0 / rho);
+  out[0] = s[0] * inv;
+  out[1] = s[1] * inv;
+  out[2] = s[2] * inv;
+  out[3] = s[3] * inv;
+  out[4] = s[4] > 1.0e-30 ? s[3] / s[4] : NAN;
+  out[5] = s[5];
+  out[6] = s[6];
+  out[7] = s[5] > 0.0 ? s[6] / s[5] : NAN;
+  out[8] = c0;
+  out[9] = mach * c0;
+}
+
+/* compute_shock_density_bubble_metrics (io/output_cases.hpp:329-407), the
+ * neighbour densities through the grid's ghost rule.  out[10]: rho min/max,
+ * p min/max, internal energy min, area, centroid x/y, schlieren max,
+ * nonfinite count. */
+void ecor_shock_bubble_metrics(const ecor_grid* g, const double* U, double xlo, double ylo,
+                               double threshold, double* out)
+{
+  const int nx = g->nx, ny = g->ny;
+  const long n = (long)nx * ny;
+  const double vol = g->dx * g->dy;
+  double rmin = INFINITY, rmax = -INFINITY, pmin = INFINITY, pmax = -INFINITY, emin = INFINITY;
+  double area = 0.0, sx = 0.0, sy = 0.0, schl = 0.0;
+  int bad_count = 0;
+  for (int j = 0; j < ny; ++j) {
+    for (int i = 0; i < nx; ++i) {
+      const long k = (long)j * nx + i;
+      const double u[4] = {U[k], U[n + k], U[2 * n + k], U[3 * n + k]};
+      double q[4];
+      ecor_to_primitive(u, g->gamma, q);
+      const double kinetic = u[0] > 0.0 ? 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0] : NAN;
+      const double internal = u[3] - kinetic;
+      if (!isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2]) || !isfinite(u[3]) ||
+          !isfinite(q[3]) || !isfinite(internal)) {
+        ++bad_count;
+        continue;
+      }
+      rmin = smin(rmin, q[0]);
+      rmax = smax(rmax, q[0]);
+      pmin = smin(pmin, q[3]);
+      pmax = smax(pmax, q[3]);
+      emin = smin(emin, internal);
+      double a[4], b[4], c[4], d[4];
+      ecor_load(g, U, i + 1, j, a);
+      ecor_load(g, U, i - 1, j, b);
+      ecor_load(g, U, i, j + 1, c);
+      ecor_load(g, U, i, j - 1, d);
+      const double rx = (a[0] - b[0]) / (2.0 * g->dx), ry = (c[0] - d[0]) / (2.0 * g->dy);
+      const double grad = sqrt(rx * rx + ry * ry);
+      if (isfinite(grad)) schl = smax(schl, grad);
+      if (q[0] < threshold) {
+        const double x = xlo + ((double)i + 0.5) * g->dx, y = ylo + ((double)j + 0.5) * g->dy;
+        area += vol;
+        sx += x * vol;
+        sy += y * vol;
+      }
+    }
+  }
+  out[0] = rmin; out[1] = rmax; out[2] = pmin; out[3] = pmax; out[4] = emin;
+  out[5] = area;
+  out[6] = area > 0.0 ? sx / area : NAN;
+  out[7] = area > 0.0 ? sy / area : NAN;
+  out[8] = schl;
+  out[9] = bad_count;
+}

@@ -151,6 +151,13 @@ double ecor_imex_dt(const ecor_grid* g, const double* U, double cfl, double time
                     int acoustic_startup, double acoustic_cap, double* material_max,
                     double* acoustic_max);
 
+/* ---- whole-field metrics (diagnostic_metrics_timestep.hpp, output_cases.hpp) */
+void ecor_admissibility(const ecor_grid* g, const double* U, double* out, int* counts);
+void ecor_gresho_metrics(const ecor_grid* g, const double* U, double xlo, double ylo, double cx,
+                         double cy, double rho, double mach, double p0, double* out);
+void ecor_shock_bubble_metrics(const ecor_grid* g, const double* U, double xlo, double ylo,
+                               double threshold, double* out);
+
 /* ---- direct BDLTV20 T1/S2 step (ec_oracle_direct.c) */
 typedef struct {
   ecor_exact exact; /* problem and periodicity; kind TORO or GRESHO gives the

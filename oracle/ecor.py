@@ -151,6 +151,9 @@ def lib():
                                        C.c_int, _dp, C.POINTER(DirectResult)]
         L.ecor_direct_dt.argtypes = [G, D, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.ecor_direct_dt.restype = C.c_double
+        L.ecor_admissibility.argtypes = [G, _dp, _dp, C.POINTER(C.c_int)]
+        L.ecor_gresho_metrics.argtypes = [G, _dp] + [C.c_double] * 7 + [_dp]
+        L.ecor_shock_bubble_metrics.argtypes = [G, _dp, C.c_double, C.c_double, C.c_double, _dp]
         L.ecor_exact_ghost.argtypes = [G, C.c_int, C.c_int]
         L.ecor_exact_ghost.restype = C.c_int
         L.ecor_exact_state.argtypes = [C.POINTER(Exact), C.c_int, C.c_int, C.c_double, C.c_int,
@@ -320,6 +323,39 @@ def direct_boundary_state(g: Grid, d: Direct, i, j, t):
     u = np.zeros(4)
     lib().ecor_direct_boundary_state(C.byref(g), C.byref(d), i, j, t, _ptr(u))
     return u
+
+
+def admissibility(g: Grid, U):
+    """(rho_min, p_min, ie_min), (nonfinite, density, pressure, ie failure counts)"""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    o = np.zeros(3); c = (C.c_int * 4)()
+    lib().ecor_admissibility(C.byref(g), _ptr(U), _ptr(o), c)
+    return tuple(o), tuple(c)
+
+
+GRESHO_METRIC_KEYS = ("density_l1_error", "velocity_l1_error", "pressure_l1_error",
+                      "pressure_perturbation_l1_error", "pressure_perturbation_l1_relative_error",
+                      "kinetic_energy_initial", "kinetic_energy_final", "kinetic_energy_ratio",
+                      "reference_sound_speed", "velocity_scale")
+
+
+def gresho_metrics(g: Grid, U, xlo, ylo, cx, cy, rho, mach, p0) -> dict:
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    o = np.zeros(10)
+    lib().ecor_gresho_metrics(C.byref(g), _ptr(U), xlo, ylo, cx, cy, rho, mach, p0, _ptr(o))
+    return dict(zip(GRESHO_METRIC_KEYS, o))
+
+
+SB_METRIC_KEYS = ("rho_min", "rho_max", "pressure_min", "pressure_max", "internal_energy_min",
+                  "density_bubble_area", "density_bubble_centroid_x", "density_bubble_centroid_y",
+                  "schlieren_max", "nonfinite_count")
+
+
+def shock_bubble_metrics(g: Grid, U, xlo, ylo, threshold=0.55) -> dict:
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    o = np.zeros(10)
+    lib().ecor_shock_bubble_metrics(C.byref(g), _ptr(U), xlo, ylo, threshold, _ptr(o))
+    return dict(zip(SB_METRIC_KEYS, o))
 
 
 def max_rate(g: Grid, U):

@@ -151,3 +151,71 @@ def test_direct_bdltv20_toro_row(tmp_path, ecor):
     assert np.array_equal(np.array([float(r["exact_rho"]) for r in rows[:nx]]), ex)
     l1 = np.abs(rho - ex[None, :]).mean()
     assert float(kv["bdltv20_paper_density_l1_error"]) == pytest.approx(l1, rel=1e-9)
+
+
+def test_shock_bubble_snapshots_and_summary(tmp_path):
+    """The shock-density bubble row's on-disk outputs (io/output_cases.hpp:
+    408-592): snapshot CSVs named by index, time and step at the requested
+    times, the summary CSV, and the report keys -- the summary's extrema,
+    centroid and schlieren maximum (device reductions) agree with the last
+    snapshot's rows (host values)."""
+    import math
+    snap = tmp_path / "snaps"
+    summary = tmp_path / "summary.csv"
+    rc, kv, err = run(tmp_path, "amr.n_cell=160 40", "euler.problem=shock_density_bubble_2d",
+                      "euler.riemann=hllc", "euler.spatial_order=2", "euler.cfl=0.4", "stop_time=0.06",
+                      "max_step=100000", "euler.shock_density_bubble_snapshot_times=0,0.03,0.06",
+                      f"euler.shock_density_bubble_snapshot_dir={snap}", f"euler.final_csv={summary}")
+    assert rc == 0, err
+    assert kv["snapshot_status"] == "ok" and kv["snapshots_written"] == "3"
+    assert kv["scheme_name"] == "explicit_o2_hllc" and kv["order"] == "O2"
+    files = sorted(p.name for p in snap.iterdir())
+    assert len(files) == 3 and files[0] == "snapshot_00_t0p000000_step000000.csv"
+    assert files[1].startswith("snapshot_01_t0p030000_step") and files[2].startswith("snapshot_02_t0p060000_step")
+    rows = list(csv.DictReader(open(snap / files[2])))
+    assert len(rows) == 160 * 40
+    s = list(csv.DictReader(open(summary)))[0]
+    assert s["status"] == "ok" and int(s["steps_accepted"]) == int(files[2][-10:-4])
+    rho = np.array([float(r["rho"]) for r in rows])
+    assert float(s["rho_min"]) == rho.min() and float(s["rho_max"]) == rho.max()
+    schl = np.array([float(r["schlieren"]) for r in rows])
+    assert float(s["schlieren_max"]) == pytest.approx(schl.max(), rel=1e-15)
+    ind = np.array([int(r["density_bubble_indicator"]) for r in rows]) == 1
+    x = np.array([float(r["x"]) for r in rows]); y = np.array([float(r["y"]) for r in rows])
+    assert ind.any()
+    assert float(s["density_bubble_centroid_x"]) == pytest.approx(x[ind].mean(), rel=1e-12)
+    assert float(s["density_bubble_centroid_y"]) == pytest.approx(y[ind].mean(), rel=1e-12)
+    assert not math.isnan(float(kv["density_bubble_centroid_x"]))
+
+
+def test_gresho_final_csv_schema(tmp_path, ecor):
+    """write_gresho_final_csv (io/output_cases.hpp:16-71): the column set, the
+    exact Gresho state and errors per cell, the run labels, and NaN trial
+    minima for an explicit run."""
+    out_csv = tmp_path / "gresho.csv"
+    n, mach = 32, 0.1
+    rc, kv, err = run(tmp_path, f"amr.n_cell={n} {n}", "euler.problem=gresho_vortex",
+                      "euler.vortex_center=0.5 0.5", f"euler.mach={mach}",
+                      f"euler.pressure={1 / (1.4 * mach * mach)!r}", "euler.density_outer=1",
+                      "euler.riemann=xie_am_hllc_p", "euler.spatial_order=2", "stop_time=0.01",
+                      "max_step=1000", f"euler.final_csv={out_csv}")
+    assert rc == 0, err
+    rows = list(csv.DictReader(open(out_csv)))
+    assert list(rows[0].keys()) == [
+        "x", "y", "rho", "u", "v", "pressure", "exact_rho", "exact_u", "exact_v", "exact_pressure",
+        "density_error", "velocity_error", "pressure_error", "pressure_perturbation_error", "time", "step",
+        "mach", "method", "riemann", "imex_form", "imex_predictor_flux", "imex_predictor_dissipation",
+        "spatial_order", "slope_limiter", "imex_trial_density_run_min", "imex_high_trial_pressure_run_min",
+        "imex_high_trial_internal_energy_run_min"]
+    r0 = rows[0]
+    assert (r0["method"], r0["riemann"], r0["imex_form"], r0["slope_limiter"]) == \
+        ("explicit", "xie_am_hllc_p", "not_applicable", "minmod")
+    assert r0["imex_trial_density_run_min"] == "nan"
+    U0 = ecor.init_gresho(n, n, mach=mach, p0=1 / (1.4 * mach * mach))
+    p_ex = np.array([float(r["exact_pressure"]) for r in rows]).reshape(n, n)
+    p_init = 0.4 * (U0[3] - 0.5 * (U0[1] ** 2 + U0[2] ** 2) / U0[0])
+    assert np.abs(p_ex - p_init).max() <= 1e-13 * p_init.max()
+    # the errors are the differences of the columns
+    e = np.array([float(r["pressure_error"]) for r in rows])
+    assert np.array_equal(e, np.array([float(r["pressure"]) - float(r["exact_pressure"]) for r in rows]))
+    assert float(kv["kinetic_energy_ratio"]) < 1.0
