@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -108,6 +109,7 @@ struct Cfg {
   int imex_solver_max_iter = 100, imex_acoustic_startup = 1, imex_picard_iterations = 2;
   int n_cell[2] = {64, 64}, is_periodic[2] = {1, 1};
   double prob_lo[2] = {0.0, 0.0}, prob_hi[2] = {1.0, 1.0};
+  std::string plot_file = "plt";
   std::string problem = "advection_blob", riemann = "hllc", method = "explicit",
               imex_form = "bdltv20_t1_s2_source_map_picard", slope_limiter = "minmod",
               imex_predictor_dissipation = "material", field_boundary = "outflow",
@@ -200,6 +202,7 @@ Cfg read_config(const Params& pp)
   pp.query("ny", c.n_cell[1]);
   pp.query("amr.max_grid_size", c.max_grid_size);
   pp.query("amr.plot_int", c.plot_int);
+  pp.query("amr.plot_file", c.plot_file);
   for (int d = 0; d < 2; ++d) {
     pp.query("geometry.prob_lo", c.prob_lo[d], d);
     pp.query("geometry.prob_hi", c.prob_hi[d], d);
@@ -372,6 +375,147 @@ std::vector<double> parse_times(const std::string& s, double stop)
   t.erase(std::unique(t.begin(), t.end(), [](double a, double b) { return std::abs(a - b) < 1.0e-13; }),
           t.end());
   return t;
+}
+
+// The single-level AMReX plotfile of write_plotfile (io/output_cases.hpp:2-14):
+// WriteSingleLevelPlotfile (AMReX_PlotFileUtil.cpp:74-330) of the plot state
+// rho, momx, momy, E, pressure (imex/imex_source_map_step.hpp:618-643) over
+// the domain chopped by amr.max_grid_size (BoxList::maxSize,
+// AMReX_BoxList.cpp:765-815), each box a native-double FAB in
+// Level_0/Cell_D_00000 described by the VisMF header Level_0/Cell_H
+// (AMReX_VisMF.cpp:319-376).
+struct PlotBox { int ilo, jlo, ihi, jhi; };
+
+std::vector<PlotBox> max_size_boxes(int nx, int ny, int chunk)
+{
+  int len[2] = {nx, ny}, ratio[2] = {1, 1}, numblk[2] = {1, 1}, extra[2] = {0, 0}, sz[2] = {nx, ny};
+  for (int d = 0; d < 2; ++d) {
+    if (len[d] > chunk) {
+      int bs = chunk, nlen = len[d];
+      while (bs % 2 == 0 && nlen % 2 == 0) {
+        ratio[d] *= 2;
+        bs /= 2;
+        nlen /= 2;
+      }
+      numblk[d] = (nlen + bs - 1) / bs;
+      sz[d] = nlen / numblk[d];
+      extra[d] = nlen - sz[d] * numblk[d];
+    }
+  }
+  std::vector<PlotBox> out;
+  if (numblk[0] == 1 && numblk[1] == 1) {
+    out.push_back({0, 0, nx - 1, ny - 1});
+    return out;
+  }
+  auto lohi = [&](int d, int b, int& lo, int& hi) {
+    lo = b < extra[d] ? b * (sz[d] + 1) * ratio[d] : (b * sz[d] + extra[d]) * ratio[d];
+    hi = b < extra[d] ? lo + (sz[d] + 1) * ratio[d] - 1 : lo + sz[d] * ratio[d] - 1;
+  };
+  for (int j = 0; j < numblk[1]; ++j) {
+    for (int i = 0; i < numblk[0]; ++i) {
+      PlotBox b;
+      lohi(0, i, b.ilo, b.ihi);
+      lohi(1, j, b.jlo, b.jhi);
+      out.push_back(b);
+    }
+  }
+  return out;
+}
+
+bool write_plotfile(const Cfg& c, const std::vector<double>& U, int step, double time)
+{
+  const int nx = c.n_cell[0], ny = c.n_cell[1];
+  const size_t n = (size_t)nx * ny;
+  const double dx = (c.prob_hi[0] - c.prob_lo[0]) / nx, dy = (c.prob_hi[1] - c.prob_lo[1]) / ny;
+  std::ostringstream nm;
+  nm << c.plot_file << std::setfill('0') << std::setw(5) << step;
+  const std::string dir = nm.str();
+  std::error_code ec;
+  std::filesystem::create_directories(dir + "/Level_0", ec);
+  if (ec) return false;
+  const std::vector<PlotBox> boxes = max_size_boxes(nx, ny, c.max_grid_size > 0 ? c.max_grid_size : nx + ny);
+  auto box_str = [](int ilo, int jlo, int ihi, int jhi) {
+    std::ostringstream b;
+    b << "((" << ilo << ',' << jlo << ") (" << ihi << ',' << jhi << ") (0,0))";
+    return b.str();
+  };
+  const char* names[5] = {"rho", "momx", "momy", "E", "pressure"};
+  {
+    std::ofstream h(dir + "/Header", std::ios::out | std::ios::trunc | std::ios::binary);
+    h.precision(17);
+    h << "HyperCLaw-V1.1\n" << 5 << '\n';
+    for (const char* v : names) h << v << '\n';
+    h << 2 << '\n' << time << '\n' << 0 << '\n';
+    h << c.prob_lo[0] << ' ' << c.prob_lo[1] << ' ' << '\n';
+    h << c.prob_hi[0] << ' ' << c.prob_hi[1] << ' ' << '\n';
+    h << '\n';
+    h << box_str(0, 0, nx - 1, ny - 1) << ' ' << '\n';
+    h << step << ' ' << '\n';
+    h << dx << ' ' << dy << ' ' << '\n';
+    h << 0 << '\n' << "0\n";
+    h << 0 << ' ' << boxes.size() << ' ' << time << '\n' << step << '\n';
+    for (const PlotBox& b : boxes) {
+      h << c.prob_lo[0] + dx * (double)b.ilo << ' ' << c.prob_lo[0] + dx * (double)(b.ihi + 1) << '\n';
+      h << c.prob_lo[1] + dy * (double)b.jlo << ' ' << c.prob_lo[1] + dy * (double)(b.jhi + 1) << '\n';
+    }
+    h << "Level_0/Cell\n";
+    if (!h) return false;
+  }
+  std::vector<long long> offsets;
+  std::vector<std::array<double, 5>> mins, maxs;
+  {
+    std::ofstream d(dir + "/Level_0/Cell_D_00000", std::ios::out | std::ios::trunc | std::ios::binary);
+    for (const PlotBox& b : boxes) {
+      offsets.push_back((long long)d.tellp());
+      d << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(b.ilo, b.jlo, b.ihi, b.jhi)
+        << ' ' << 5 << '\n';
+      std::array<double, 5> lo, hi;
+      lo.fill(INFINITY);
+      hi.fill(-INFINITY);
+      std::vector<double> buf((size_t)(b.ihi - b.ilo + 1) * (b.jhi - b.jlo + 1));
+      for (int comp = 0; comp < 5; ++comp) {
+        size_t q = 0;
+        for (int j = b.jlo; j <= b.jhi; ++j) {
+          for (int i = b.ilo; i <= b.ihi; ++i) {
+            const size_t k = (size_t)j * nx + i;
+            double v;
+            if (comp < 4) {
+              v = U[comp * n + k];
+            } else {
+              const double rho = U[k], inv = 1.0 / rho, ux = U[n + k] * inv, uy = U[2 * n + k] * inv;
+              v = (c.gamma - 1.0) * (U[3 * n + k] - 0.5 * rho * (ux * ux + uy * uy));
+            }
+            buf[q++] = v;
+            lo[comp] = std::min(lo[comp], v);
+            hi[comp] = std::max(hi[comp], v);
+          }
+        }
+        d.write(reinterpret_cast<const char*>(buf.data()), (std::streamsize)(buf.size() * sizeof(double)));
+      }
+      mins.push_back(lo);
+      maxs.push_back(hi);
+    }
+    if (!d) return false;
+  }
+  std::ofstream m(dir + "/Level_0/Cell_H", std::ios::out | std::ios::trunc);
+  m << 1 << '\n' << 1 << '\n' << 5 << '\n' << 0 << '\n';
+  m << '(' << boxes.size() << ' ' << 0 << '\n';
+  for (const PlotBox& b : boxes) m << box_str(b.ilo, b.jlo, b.ihi, b.jhi) << '\n';
+  m << ')' << '\n';
+  m << boxes.size() << '\n';
+  for (size_t b = 0; b < boxes.size(); ++b) m << "FabOnDisk: Cell_D_00000 " << offsets[b] << '\n';
+  m << '\n';
+  m.setf(std::ios::scientific, std::ios::floatfield);
+  m.precision(17);
+  for (const auto* mm : {&mins, &maxs}) {
+    m << boxes.size() << ',' << 5 << '\n';
+    for (const auto& row : *mm) {
+      for (double v : row) m << v << ',';
+      m << '\n';
+    }
+    m << '\n';
+  }
+  return static_cast<bool>(m);
 }
 
 // Shock-bubble labels (io/output_cases.hpp:306-327, problems/problem_initialization.hpp:37-49)
@@ -818,6 +962,14 @@ int main(int argc, char** argv)
     }
   };
   if (sb) take_snapshots(false);
+  // plotfiles at step 0, every plot_int steps and at the end (app/driver_main.hpp:213,332,345-349)
+  int last_plot = -1;
+  auto plot_now = [&](bool download) {
+    if (download) HIP_OK(hipMemcpy(host.data(), A, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (!write_plotfile(cfg, host, step, time)) abort_msg("plotfile write failed");
+    last_plot = step;
+  };
+  if (cfg.plot_int >= 0) plot_now(false);
   ec_imex_diag last_imex{};
   int imex_iter_max = 0;
   // update_high_trial_run over the IMEX steps (numerics/diagnostic_metrics_timestep.hpp:122-131)
@@ -838,11 +990,17 @@ int main(int argc, char** argv)
     rs.stop_time = cfg.stop_time;
     rs.cfl = cfg.cfl;
     rs.max_step = cfg.max_step;
+    const int chunk = cfg.plot_int > 0 ? std::min(cfg.plot_int, 512) : 512;
     while (true) {
       int in_b = 0;
       const int before = rs.step;
-      EC_OK_OR_DIE(ctx, ec_explicit_advance(ctx, A, B, &lay, 512, &rs, &in_b));
+      EC_OK_OR_DIE(ctx, ec_explicit_advance(ctx, A, B, &lay, chunk, &rs, &in_b));
       if (in_b) std::swap(A, B);
+      if (cfg.plot_int > 0 && rs.step != before && rs.step % cfg.plot_int == 0) {
+        step = rs.step;
+        time = rs.time;
+        plot_now(true);
+      }
       if (rs.status != 0 || rs.step >= cfg.max_step || rs.time >= cfg.stop_time - 1.0e-14 ||
           rs.step == before) {
         break;
@@ -919,6 +1077,7 @@ int main(int argc, char** argv)
       time += dt;
       ++step;
       EC_OK_OR_DIE(ctx, ec_state_diagnostics(ctx, A, &lay, &diag));
+      if (cfg.plot_int > 0 && step % cfg.plot_int == 0) plot_now(true);
       if (diag.nonfinite_count > 0) {
         status = "failed";
         failure = "nonfinite_state";
@@ -938,6 +1097,7 @@ int main(int argc, char** argv)
   }
   if (sb && status == "ok") failure = "ok";
   HIP_OK(hipMemcpy(host.data(), A, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+  if (cfg.plot_int >= 0 && last_plot != step) plot_now(false);
   const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   double m1, e1;
   sums(host, m1, e1);

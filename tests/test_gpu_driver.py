@@ -31,7 +31,8 @@ euler.cfl = 0.4
 def run(tmp_path, *overrides):
     inp = tmp_path / "inputs"
     inp.write_text(INPUTS)
-    r = subprocess.run([str(EXE), str(inp), *overrides], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([str(EXE), str(inp), *overrides], capture_output=True, text=True, timeout=120,
+                       cwd=tmp_path)
     kv = dict(line.split("=", 1) for line in r.stdout.splitlines() if "=" in line)
     return r.returncode, kv, r.stderr
 
@@ -219,3 +220,64 @@ def test_gresho_final_csv_schema(tmp_path, ecor):
     e = np.array([float(r["pressure_error"]) for r in rows])
     assert np.array_equal(e, np.array([float(r["pressure"]) - float(r["exact_pressure"]) for r in rows]))
     assert float(kv["kinetic_energy_ratio"]) < 1.0
+
+
+def read_plotfile(path):
+    """Minimal reader of a single-level AMReX plotfile (the Header, the VisMF
+    header Level_0/Cell_H and native-double FABs in Cell_D files)."""
+    h = (path / "Header").read_text().split("\n")
+    nvar = int(h[1])
+    names = h[2:2 + nvar]
+    time = float(h[2 + nvar + 1])
+    dom = h[2 + nvar + 6].strip()
+    lo, hi = [tuple(int(v) for v in t.strip("()").split(",")) for t in dom.strip("()").split(") (")[:2]]
+    nx, ny = hi[0] + 1, hi[1] + 1
+    ch = (path / "Level_0" / "Cell_H").read_text().split("\n")
+    nbox = int(ch[4][1:].split()[0])
+    fods = [ln.split() for ln in ch[4 + nbox + 3: 4 + nbox + 3 + nbox]]
+    field = np.full((nvar, ny, nx), np.nan)
+    raw = {}
+    for _, fname, off in fods:
+        data = raw.setdefault(fname, (path / "Level_0" / fname).read_bytes())
+        off = int(off)
+        end = data.index(b"\n", off)
+        head = data[off:end].decode()
+        assert head.startswith("FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))")
+        box = head[head.index(")))") + 3:].rsplit(" ", 1)
+        ncomp = int(box[1])
+        (i0, j0), (i1, j1) = [tuple(int(v) for v in t.split(",")) for t in
+                              box[0].strip("()").split(") (")[:2]]
+        cnt = (i1 - i0 + 1) * (j1 - j0 + 1)
+        vals = np.frombuffer(data, dtype="<f8", count=cnt * ncomp, offset=end + 1)
+        field[:, j0:j1 + 1, i0:i1 + 1] = vals.reshape(ncomp, j1 - j0 + 1, i1 - i0 + 1)
+    return names, time, field, ch
+
+
+def test_plotfiles(tmp_path):
+    """write_plotfile (io/output_cases.hpp:2-14): plotfiles at step 0, every
+    plot_int steps and at the end, over the max_grid_size box layout; the
+    last one holds the final state of the CSV (bit for bit) and its VisMF
+    min/max records match the data."""
+    out_csv = tmp_path / "sod.csv"
+    rc, kv, err = run(tmp_path, "amr.n_cell=80 8", "amr.max_grid_size=32", "amr.plot_int=5",
+                      "geometry.prob_hi=1 0.1", "geometry.is_periodic=0 1", "euler.problem=toro1",
+                      "prob.test=1", "euler.cfl=0.8", "stop_time=0.05", "max_step=100000",
+                      "euler.riemann=hllc", "euler.spatial_order=2", f"euler.final_csv={out_csv}")
+    assert rc == 0, err
+    steps = int(kv["completed_steps"])
+    plts = sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("plt"))
+    want = sorted({f"plt{s:05d}" for s in list(range(0, steps + 1, 5)) + [steps]})
+    assert plts == want
+    names, t, field, ch = read_plotfile(tmp_path / plts[-1])
+    assert names == ["rho", "momx", "momy", "E", "pressure"]
+    assert t == pytest.approx(0.05, abs=1e-15)
+    # BoxList::maxSize halves 80 and 32 together to 5 blocks of 2, then
+    # splits 5 into 2 + 2 + 1: boxes 32, 32 and 16 cells wide
+    assert ch[4].startswith("(3 0") and ch[5] == "((0,0) (31,7) (0,0))" and ch[7] == "((64,0) (79,7) (0,0))"
+    rows = list(csv.DictReader(open(out_csv)))
+    rho = np.array([float(r["rho"]) for r in rows]).reshape(8, 80)
+    p = np.array([float(r["p"]) for r in rows]).reshape(8, 80)
+    assert np.array_equal(field[0], rho) and np.array_equal(field[4], p)
+    nb = 3
+    mins = [[float(v) for v in ln.rstrip(",").split(",")] for ln in ch[4 + nb + 3 + nb + 2: 4 + nb + 3 + nb + 2 + nb]]
+    assert mins[0][0] == field[0][:, :32].min() and mins[2][4] == field[4][:, 64:].min()
