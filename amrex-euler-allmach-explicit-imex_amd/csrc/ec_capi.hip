@@ -127,6 +127,8 @@ extern "C" ec_status ec_destroy(ec_ctx* ctx)
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->scratch_state) (void)hipFree(ctx->scratch_state);
+  if (ctx->metrics_part) (void)hipFree(ctx->metrics_part);
+  if (ctx->metrics_part_host) (void)hipHostFree(ctx->metrics_part_host);
   delete ctx;
   return EC_OK;
 }

@@ -87,6 +87,10 @@ struct ec_ctx {
   void* imex = nullptr;
   void* adjoint = nullptr;
   void* direct = nullptr;
+  // Whole-field metric reductions: workgroup partials (device) and their
+  // pinned host copy, allocated on first use.
+  double* metrics_part = nullptr;
+  double* metrics_part_host = nullptr;
   // Optional step-kernel timing: event pairs recorded around each explicit
   // step kernel launch on the context's stream.
   bool timing = false;

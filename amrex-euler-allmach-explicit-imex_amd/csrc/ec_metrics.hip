@@ -22,6 +22,7 @@ namespace ec {
 namespace {
 
 constexpr int MB = 256;
+constexpr int kMaxBlocks = 1024;
 enum Op { OP_MIN = 0, OP_MAX = 1, OP_SUM = 2 };
 
 __host__ __device__ __forceinline__ double combine(int op, double a, double b)
@@ -198,23 +199,32 @@ __global__ void __launch_bounds__(MB) shock_bubble_kernel(const double* __restri
 int blocks(long long n)
 {
   long long b = (n + MB - 1) / MB;
-  if (b > 1024) b = 1024;
+  if (b > kMaxBlocks) b = kMaxBlocks;
   return b < 1 ? 1 : (int)b;
 }
 
-// Launches through `launch`, then folds nb partials of K values on the host.
+constexpr int kMaxK = 16;
+
+// Launches through `launch` into the context's partial buffer, then folds
+// the nb partials of K values on the host in workgroup order.
 template <int K, typename L>
 ec_status reduce(ec_ctx* ctx, int nb, const int (&ops)[K], double (&out)[K], const L& launch)
 {
-  double* part = nullptr;
-  if (hipMalloc(&part, sizeof(double) * K * nb) != hipSuccess) return EC_ERR_OUT_OF_MEMORY;
-  launch(part);
-  std::vector<double> h((size_t)K * nb);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(double) * K * nb, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(part);
-  if (e != hipSuccess) return hip_fail(ctx, e, "metrics reduction");
+  static_assert(K <= kMaxK, "partial record too long");
+  if (!ctx->metrics_part) {
+    if (hipMalloc(&ctx->metrics_part, sizeof(double) * kMaxK * kMaxBlocks) != hipSuccess) {
+      return EC_ERR_OUT_OF_MEMORY;
+    }
+    if (hipHostMalloc(&ctx->metrics_part_host, sizeof(double) * kMaxK * kMaxBlocks) != hipSuccess) {
+      return EC_ERR_OUT_OF_MEMORY;
+    }
+  }
+  launch(ctx->metrics_part);
+  EC_HIP_TRY(ctx, hipGetLastError());
+  EC_HIP_TRY(ctx, hipMemcpyAsync(ctx->metrics_part_host, ctx->metrics_part, sizeof(double) * K * nb,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  EC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const double* h = ctx->metrics_part_host;
   for (int k = 0; k < K; ++k) {
     double r = h[k];
     for (int b = 1; b < nb; ++b) r = combine(ops[k], r, h[(size_t)b * K + k]);
