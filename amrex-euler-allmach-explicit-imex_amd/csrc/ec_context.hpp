@@ -100,6 +100,9 @@ struct ec_ctx {
   ec::ExactBc* exact_dev = nullptr;
   double* exact_ring = nullptr;
   int exact_kind = 0;
+  // The rule of each side where exact states are not read (the IMEX step):
+  // periodic on a periodic axis, else outflow.
+  int exact_plain_bc[4] = {0, 0, 0, 0};
 };
 
 namespace ec {

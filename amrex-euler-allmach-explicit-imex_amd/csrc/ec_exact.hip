@@ -115,6 +115,7 @@ extern "C" ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* p
   for (int s = 0; s < 4; ++s) {
     ctx->grid.bc[s] = bc[s];
     ctx->desc.bc[s] = bc[s];
+    ctx->exact_plain_bc[s] = (s < 2 ? px : py) ? BC_PERIODIC : BC_OUTFLOW;
   }
   ctx->grid.exact = nullptr;
   if (px && py) return EC_OK;

@@ -39,8 +39,6 @@
 
 namespace ec {
 
-bool has_exact_side(const GridParams& g);
-
 constexpr int RB = 256;        // threads per reduction workgroup
 constexpr int MAXK = 64;       // largest GMRES restart length
 
@@ -1954,10 +1952,28 @@ ec_status imex_solve_pressure(ec_ctx* ctx, const int bc[4], double cx, double cy
 // ------------------------------------------------------------------ C-ABI
 using namespace ec;
 
-static ImexGrid imex_grid(const ec_ctx* ctx, double dt)
+// The step's ghost rule.  Exact-Dirichlet sides read as outflow on a
+// non-periodic axis and as periodic on a periodic one: the source-map step
+// fills the state's ghosts with the periodic images and clamps the physical
+// sides before the star update for every problem but the shock bubble
+// (imex/imex_source_map_step.hpp:405-409, problems/boundary_conditions.hpp:
+// 2-25), and its host grid reads valid cells only, wrapped or clamped
+// (imex/imex_source_map_helpers.hpp:79-95,117-135), so the exact ghosts it
+// writes after the star update (imex_source_map_helpers.hpp:67) and at the
+// end of the step (imex_source_map_step.hpp:275) are never read.
+static GridParams imex_ghost_rule(const ec_ctx* ctx)
+{
+  GridParams g = ctx->grid;
+  for (int s = 0; s < 4; ++s)
+    if (g.bc[s] == BC_EXACT) g.bc[s] = ctx->exact_plain_bc[s];
+  g.exact = nullptr;
+  g.time_ptr = nullptr;
+  return g;
+}
+
+static ImexGrid imex_grid(const GridParams& g, double dt)
 {
   ImexGrid G{};
-  const GridParams& g = ctx->grid;
   G.nx = g.nx;
   G.ny = g.ny;
   for (int s = 0; s < 4; ++s) {
@@ -1997,12 +2013,6 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
   if (lay->row_stride < ctx->grid.nx || lay->comp_stride < lay->row_stride * ctx->grid.ny) {
     return EC_ERR_INVALID_ARGUMENT;
   }
-  if (has_exact_side(ctx->grid)) {
-    // The source-map step fills exact ghosts at time 0 (imex/
-    // imex_source_map_step.hpp:275,406); that boundary is not built here.
-    ctx->last_error = "the IMEX step does not take exact-Dirichlet sides";
-    return EC_ERR_UNSUPPORTED;
-  }
   if (opts->picard_iterations < 1 || opts->restart < 1 || opts->restart > MAXK ||
       !(opts->tol > 0.0) || opts->max_iter < 0 ||
       (opts->precision != EC_IMEX_PRECISION_FP64 && opts->precision != EC_IMEX_PRECISION_MIXED) ||
@@ -2023,11 +2033,12 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
   std::memset(&h, 0, sizeof(GmCtl));
   EC_HIP_TRY(ctx, hipMemcpyAsync(w->ctl, &h, sizeof(GmCtl), hipMemcpyHostToDevice, ctx->stream));
   HostCtl hc{ctx, w};
-  const ImexGrid G = imex_grid(ctx, dt);
+  const GridParams g = imex_ghost_rule(ctx);
+  const ImexGrid G = imex_grid(g, dt);
 
   // star state and initial fields
   hipLaunchKernelGGL(imex_star_kernel, dim3(nb), dim3(RB), 0, ctx->stream, U, lay->row_stride,
-                     lay->comp_stride, ctx->grid, dt, opts->predictor_dissipation, w->star);
+                     lay->comp_stride, g, dt, opts->predictor_dissipation, w->star);
   hipLaunchKernelGGL(imex_init_kernel, dim3(nb), dim3(RB), 0, ctx->stream, U, lay->row_stride,
                      lay->comp_stride, ctx->grid.gamma, w->nx, w->ny, w->star, w->p, w->mom, w->ctl);
   EC_HIP_TRY(ctx, hipGetLastError());
@@ -2131,7 +2142,7 @@ extern "C" ec_status ec_imex_dt(ec_ctx* ctx, const double* U, const ec_layout* l
   ImexWork* w = static_cast<ImexWork*>(ctx->imex);
   const long long n = (long long)w->nx * w->ny;
   hipLaunchKernelGGL(imex_rates_kernel, dim3(grid_blocks(n)), dim3(RB), 0, ctx->stream, U,
-                     lay->row_stride, lay->comp_stride, ctx->grid, w->partials, w->ctl);
+                     lay->row_stride, lay->comp_stride, imex_ghost_rule(ctx), w->partials, w->ctl);
   hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, ctx->stream, w->partials,
                      grid_blocks(n), 1, &w->ctl->mat_max, &w->ctl->aco_max);
   EC_HIP_TRY(ctx, hipGetLastError());

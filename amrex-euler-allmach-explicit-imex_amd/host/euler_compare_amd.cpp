@@ -329,10 +329,6 @@ Cfg read_config(const Params& pp)
       abort_msg("BDLTV20 advection-blob mode requires problem=advection_blob, periodicity=1 1, and at least 8x8 cells.");
     }
   }
-  if (c.field_boundary == "exact_dirichlet" && c.method == "imex" && c.bdltv20_paper_t1_s2 == "off" &&
-      (!c.is_periodic[0] || !c.is_periodic[1])) {
-    abort_msg("euler.field_boundary=exact_dirichlet with euler.method=imex is not part of this build.");
-  }
   if (c.method == "imex") {
     // core/config_reader.hpp:177-196
     if (c.spatial_order != 2) abort_msg("euler.method=imex requires euler.spatial_order=2 in this code pack.");

@@ -407,8 +407,11 @@ ec_status ec_post_shock_state(double gamma, double mach, double rho1, double p1,
  * built here, the Toro exact solution and the advected blob at the step's
  * time -- the time set by ec_set_boundary_time for ec_explicit_step and
  * ec_explicit_step_inplace, the run's own time in ec_explicit_advance.
- * ec_imex_step and the adjoint entry points return EC_ERR_UNSUPPORTED on
- * such a context.  Synchronises the context's stream. */
+ * ec_imex_step reads EC_BC_EXACT sides as outflow (periodic on a periodic
+ * axis), as the source-map step clamps the physical ghosts before its star
+ * update whatever the field boundary (imex/imex_source_map_step.hpp:405-409);
+ * the adjoint entry points return EC_ERR_UNSUPPORTED on such a context.
+ * Synchronises the context's stream. */
 ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* problem, const int* periodic);
 /* The time argument of euler_fv_one_step (numerics/explicit_update.hpp:58-65)
  * for the exact ghost states of the next host-dt steps (default 0). */

@@ -82,6 +82,30 @@ def test_imex_row(tmp_path):
     assert float(kv["imex_solver_relative_residual_max"]) <= 1e-10
 
 
+def test_imex_exact_dirichlet_row_equals_outflow(tmp_path):
+    """euler.method=imex with euler.field_boundary=exact_dirichlet on a
+    non-periodic Gresho box: the source-map step clamps the physical ghosts
+    before its star update for every problem but the shock bubble
+    (imex/imex_source_map_step.hpp:405-409), so the run and its final CSV
+    equal the outflow run's."""
+    outs = []
+    for fb in ("exact_dirichlet", "outflow"):
+        out_csv = tmp_path / f"{fb}.csv"
+        rc, kv, err = run(tmp_path, "amr.n_cell=48 40", "geometry.prob_hi=1.0 0.8",
+                          "geometry.is_periodic=0 0", "euler.problem=gresho_vortex",
+                          "euler.vortex_center=0.4 0.35", "euler.mach=0.01",
+                          "euler.pressure=7142.857142857143", "euler.method=imex",
+                          "euler.spatial_order=2", "euler.imex_acoustic_startup=0",
+                          "euler.imex_solver_tol=1e-10", "euler.imex_solver_max_iter=2000",
+                          "max_step=3", "stop_time=1.0", f"euler.field_boundary={fb}",
+                          f"euler.final_csv={out_csv}")
+        assert rc == 0, err
+        assert kv["imex_step_count"] == "3", kv
+        assert kv["field_boundary"] == fb
+        outs.append(out_csv.read_text())
+    assert outs[0] == outs[1]
+
+
 def test_toro_exact_dirichlet_row(tmp_path, ecor):
     """A Riemann-strip row of the convergence study: geometry.is_periodic=0 0,
     euler.field_boundary=exact_dirichlet, 5 cells across (the device loop with

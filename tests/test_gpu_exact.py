@@ -164,7 +164,44 @@ def test_exact_boundary_rejections(ecor, ecamd, gpu):
     assert lib().ec_set_exact_boundary(s.ctx, C.byref(bad), per) == 1
     iv = problems.problem_desc("isentropic_vortex", g.nx, g.ny)
     assert lib().ec_set_exact_boundary(s.ctx, C.byref(iv), per) == 1
-    # IMEX and the adjoint do not take exact sides
+    # the adjoint does not take exact sides (the reference has no adjoint)
+    from ecamd import adjoint
     Ud = torch.from_numpy(U).to(gpu)
     with pytest.raises(RuntimeError, match="unsupported|exact"):
-        s.imex_step(Ud, 1e-4)
+        adjoint.adjoint_step(s, Ud, torch.zeros_like(Ud), 1e-4)
+
+
+@pytest.mark.parametrize("name,periodic", [("gresho", (0, 0)), ("toro", (0, 1)), ("toro", (1, 0))])
+def test_imex_step_reads_exact_sides_as_the_source_map_step(ecor, ecamd, gpu, name, periodic):
+    """The source-map step wraps periodic ghosts and clamps physical ones
+    before its star update whatever euler.field_boundary says
+    (imex/imex_source_map_step.hpp:405-409), and reads valid cells only
+    afterwards.  A context with exact sides therefore takes the same IMEX step
+    as one with outflow sides (periodic ones on a periodic axis, Toro's x
+    included): bit for bit, and the star density (no solve in it) bit for bit
+    against the oracle's step with those sides."""
+    s, g, U = setup(ecor, ecamd, name, periodic, 3, nx=32, ny=24)
+    nx, ny = g.nx, g.ny
+    bc = tuple(0 if periodic[a // 2] else 1 for a in range(4))
+    plain = ecamd.ExplicitSolver(ecamd.Grid(nx, ny, g.dx, g.dy, GAMMA, "xie_am_hllc_p", "minmod", 2,
+                                            tuple("periodic" if b == 0 else "outflow" for b in bc)))
+    go = ecor.make_grid(nx, ny, g.dx, g.dy, riemann=3, bc=bc)
+    dt, _, _ = ecor.imex_dt(go, U, 0.8, 0.0, 1.0)
+    opts = dict(tol=1e-10, max_iter=3000, picard_iterations=2, restart=64)
+    out = []
+    for solver in (s, plain):
+        Ud = torch.from_numpy(U.copy()).to(gpu)
+        p = torch.empty(nx * ny, dtype=torch.float64, device=gpu)
+        d = solver.imex_step(Ud, dt, solver.imex_opts(**opts), p)
+        assert ecamd.IMEX_STATUS[d.status] in ("converged_absolute", "converged_relative"), d.status
+        out.append((Ud.cpu().numpy(), p.cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    ref, _, r = ecor.imex_step(go, U, dt, picard=2, tol=1e-10, max_iter=3000)
+    assert r.status in (0, 1)
+    assert np.array_equal(out[0][0][0], ref[0])
+    if name != "gresho":
+        return           # Toro's ghosts at t = 0 equal the clamped end states
+    # the vortex reaches the sides: the exact ghosts would change the star
+    ge = ecor.set_exact(ecor.make_grid(nx, ny, g.dx, g.dy, riemann=3, bc=bc),
+                        oracle_exact(ecor, name, periodic, (1.0, 0.7)), 0.0)
+    assert not np.array_equal(ecor.imex_star(ge, U, dt)[0], ecor.imex_star(go, U, dt)[0])
