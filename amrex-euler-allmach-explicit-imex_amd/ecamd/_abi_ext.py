@@ -126,6 +126,8 @@ def declare(L) -> None:
     L.ec_shock_bubble_metrics.argtypes = [vp, vp, C.POINTER(Layout), dp, C.c_double,
                                           C.POINTER(ShockBubbleMetricsOut)]
     L.ec_shock_bubble_metrics.restype = st
+    L.ec_cylindrical_source.argtypes = [vp, vp, C.POINTER(Layout), C.c_double, C.c_double]
+    L.ec_cylindrical_source.restype = st
     L.ec_exact_state_at.argtypes = [C.POINTER(ProblemDesc), C.POINTER(C.c_int), C.c_double,
                                     C.c_double, C.c_double, dp]
     L.ec_exact_state_at.restype = st

@@ -299,6 +299,14 @@ class ExplicitSolver:
               "ec_shock_bubble_metrics")
         return m
 
+    def cylindrical_source(self, U: torch.Tensor, dt: float, y_lo: float = 0.0) -> None:
+        """apply_shock_density_bubble_cylindrical_source
+        (problems/boundary_conditions.hpp:75-131), in place on U."""
+        self._check_state(U)
+        check(self._ctx, lib().ec_cylindrical_source(self._ctx, _ptr(U), C.byref(self.layout), float(dt),
+                                                      float(y_lo)),
+              "ec_cylindrical_source")
+
     def imex_dt(self, U: torch.Tensor, cfl: float, time: float, stop_time: float,
                 acoustic_startup: int = 0, acoustic_cfl_cap: float = 0.0):
         self._check_state(U)

@@ -121,6 +121,9 @@ struct Cfg {
   // transmissive-boundary systems that the unrestarted solve converges).
   int imex_gmres_restart = 64;
   int problem_kind = EC_PROBLEM_ADVECTION_BLOB, riemann_kind = 1, limiter_kind = 0;
+  // ShockDensityBubbleCylindrical: the shock bubble plus the radial geometric
+  // source after every update (core/solver_types.hpp:8-9)
+  bool cylindrical = false;
   // the direct BDLTV20 driver (core/run_config.hpp:33-35, 51-52)
   double bdltv20_paper_t1_s2_dt = -1.0, bdltv20_paper_epsilon = 1.0;
   int bdltv20_paper_t1_s2_max_steps = 4096;
@@ -128,6 +131,12 @@ struct Cfg {
 };
 
 // core/config_parse_helpers.hpp:11-42
+bool is_cylindrical_name(const std::string& l)
+{
+  return l == "shock_density_bubble_cylindrical_clawpack" || l == "shock_density_bubble_cylindrical" ||
+         l == "clawpack_shock_density_bubble";
+}
+
 int parse_problem(const std::string& v)
 {
   const std::string l = lower(v);
@@ -142,12 +151,14 @@ int parse_problem(const std::string& v)
       l == "shock_density_bubble") {
     return EC_PROBLEM_SHOCK_DENSITY_BUBBLE;
   }
+  if (is_cylindrical_name(l)) return EC_PROBLEM_SHOCK_DENSITY_BUBBLE;
   if (l == "shock_bubble_mach") return EC_PROBLEM_SHOCK_BUBBLE_MACH;
   abort_msg("Unknown euler.problem=" + v);
 }
 
-const char* problem_name(int k)
+const char* problem_name(int k, bool cylindrical = false)
 {
+  if (cylindrical) return "shock_density_bubble_cylindrical_clawpack";
   switch (k) {
     case EC_PROBLEM_TORO: return "toro1";
     case EC_PROBLEM_GRESHO: return "gresho_vortex";
@@ -194,12 +205,14 @@ const char* limiter_name(int k) { return k == 1 ? "mc" : (k == 2 ? "vanleer" : "
 Cfg read_config(const Params& pp)
 {
   Cfg c;
-  pp.query("max_step", c.max_step);
+  // core/config_reader.hpp:9-33 (whether max_step and the grid were given)
+  const bool max_step_set = pp.query("max_step", c.max_step);
   pp.query("stop_time", c.stop_time);
-  pp.query("amr.n_cell", c.n_cell[0], 0);
-  pp.query("amr.n_cell", c.n_cell[1], 1);
-  pp.query("nx", c.n_cell[0]);
-  pp.query("ny", c.n_cell[1]);
+  const bool n_cell0 = pp.query("amr.n_cell", c.n_cell[0], 0);
+  const bool n_cell1 = pp.query("amr.n_cell", c.n_cell[1], 1);
+  const bool nx_set = pp.query("nx", c.n_cell[0]);
+  const bool ny_set = pp.query("ny", c.n_cell[1]);
+  const bool grid_size_set = (n_cell0 && n_cell1) || nx_set || ny_set;
   pp.query("amr.max_grid_size", c.max_grid_size);
   pp.query("amr.plot_int", c.plot_int);
   pp.query("amr.plot_file", c.plot_file);
@@ -268,6 +281,7 @@ Cfg read_config(const Params& pp)
   pp.query("prob.y0", c.riemann_interface_y);
 
   c.problem_kind = parse_problem(c.problem);
+  c.cylindrical = is_cylindrical_name(lower(c.problem));
   c.riemann_kind = parse_riemann(c.riemann);
   c.limiter_kind = parse_limiter(c.slope_limiter);
   c.method = lower(c.method);
@@ -345,9 +359,47 @@ Cfg read_config(const Params& pp)
     }
   }
   if (c.problem_kind == EC_PROBLEM_SHOCK_DENSITY_BUBBLE) {
-    // the reference's fixed shock-bubble geometry (core/config_reader.hpp:288-295)
-    c.prob_lo[0] = 0.0; c.prob_lo[1] = 0.0; c.prob_hi[0] = 2.0; c.prob_hi[1] = 0.5;
-    c.is_periodic[0] = 0; c.is_periodic[1] = 0;
+    // core/config_reader.hpp:262-320: the shock bubble's defaults where the
+    // inputs keep the generic ones, then its fixed geometry and checks
+    if (!max_step_set && c.max_step == 8) c.max_step = 200000;
+    if (!grid_size_set && c.n_cell[0] == 64 && c.n_cell[1] == 64) {
+      c.n_cell[0] = 160;
+      c.n_cell[1] = 40;
+    }
+    if (c.prob_lo[0] == 0.0 && c.prob_lo[1] == 0.0 && c.prob_hi[0] == 1.0 && c.prob_hi[1] == 1.0) {
+      c.prob_hi[0] = 2.0;
+      c.prob_hi[1] = 0.5;
+    }
+    if (c.stop_time == 0.05) c.stop_time = 0.3;
+    if (c.cfl == 0.4) c.cfl = 0.45;
+    if (c.spatial_order == 1) c.spatial_order = 2;
+    c.is_periodic[0] = 0;
+    c.is_periodic[1] = 0;
+    const bool valid_domain = std::abs(c.prob_lo[0] - 0.0) < 1.0e-14 && std::abs(c.prob_hi[0] - 2.0) < 1.0e-14 &&
+                              std::abs(c.prob_lo[1] - 0.0) < 1.0e-14 && std::abs(c.prob_hi[1] - 0.5) < 1.0e-14;
+    if (!valid_domain) abort_msg("shock_density_bubble requires domain x=[0,2], y=[0,0.5].");
+    if (std::abs(c.gamma - 1.4) > 1.0e-14) {
+      abort_msg("shock_density_bubble is a same-gamma ideal-gas test with gamma=1.4.");
+    }
+    if (c.n_cell[0] < 16 || c.n_cell[1] < 4) abort_msg("shock_density_bubble requires at least 16x4 cells.");
+    if (!(c.stop_time > 0.0) || !std::isfinite(c.stop_time)) {
+      abort_msg("shock_density_bubble requires positive finite final_time/stop_time.");
+    }
+    if (!(c.cfl > 0.0) || !std::isfinite(c.cfl)) abort_msg("shock_density_bubble requires positive finite CFL.");
+    if (c.method == "explicit") {
+      if (c.spatial_order != 2) abort_msg("shock_density_bubble explicit rows require order/spatial_order=2.");
+      if (c.riemann_kind != EC_RIEMANN_HLLC && c.riemann_kind != EC_RIEMANN_XIE_AM_HLLC_P) {
+        abort_msg("shock_density_bubble explicit rows support hllc or xie_am_hllc_p.");
+      }
+    } else {
+      if (c.cylindrical) {
+        abort_msg("cylindrical shock_density_bubble IMEX is not part of the cleaned Cartesian code pack.");
+      }
+      // (every parsed imex_form is the source-map form: core/config_parse_helpers.hpp:198-201)
+      if (c.spatial_order != 2) {
+        abort_msg("shock_density_bubble IMEX rows require bdltv20_t1_s2_source_map_picard and spatial_order=2.");
+      }
+    }
   }
   return c;
 }
@@ -515,8 +567,23 @@ bool write_plotfile(const Cfg& c, const std::vector<double>& U, int step, double
 }
 
 // Shock-bubble labels (io/output_cases.hpp:306-327, problems/problem_initialization.hpp:37-49)
-const char* kSbCaseId = "same_gamma_shock_density_bubble_clawpack_style_v1";
-const char* kSbClaim = "qualitative_project_high_speed_2d_only";
+const char* sb_case_id(const Cfg& c)
+{
+  return c.cylindrical ? "same_gamma_shock_density_bubble_clawpack_cylindrical_v1"
+                       : "same_gamma_shock_density_bubble_clawpack_style_v1";
+}
+
+const char* sb_claim(const Cfg& c)
+{
+  return c.cylindrical ? "qualitative_project_clawpack_style_cylindrical_high_speed_2d_only"
+                       : "qualitative_project_high_speed_2d_only";
+}
+
+// The CSV rows' last two columns (io/output_cases.hpp:497-502, 582-587).
+const char* sb_source_columns(const Cfg& c)
+{
+  return c.cylindrical ? "true,clawpack_radial_symmetry_fractional_source_step" : "false,none";
+}
 
 std::string sb_scheme_name(const Cfg& c)
 {
@@ -585,11 +652,12 @@ bool write_sb_snapshot(const Cfg& c, const std::vector<double>& U, double dx, do
       const double internal = E - 0.5 * (mx * mx + my * my) / r;
       const double rx = (rho_at(i + 1, j) - rho_at(i - 1, j)) / (2.0 * dx);
       const double ry = (rho_at(i, j + 1) - rho_at(i, j - 1)) / (2.0 * dy);
-      out << kSbCaseId << ',' << step << ',' << time << ',' << i << ',' << j << ','
+      out << sb_case_id(c) << ',' << step << ',' << time << ',' << i << ',' << j << ','
           << c.prob_lo[0] + ((double)i + 0.5) * dx << ',' << c.prob_lo[1] + ((double)j + 0.5) * dy << ','
           << r << ',' << u << ',' << v << ',' << p << ',' << internal << ',' << std::sqrt(rx * rx + ry * ry)
           << ',' << (r < threshold ? 1 : 0) << ',' << threshold << ',' << g << ',' << c.method << ','
-          << scheme << ',' << rl << ',' << c.spatial_order << ',' << kSbClaim << ",false,false,1,false,none\n";
+          << scheme << ',' << rl << ',' << c.spatial_order << ',' << sb_claim(c) << ",false,false,1,"
+          << sb_source_columns(c) << '\n';
     }
   }
   return static_cast<bool>(out);
@@ -979,7 +1047,8 @@ int main(int argc, char** argv)
   io.restart = cfg.imex_gmres_restart;
   io.precision = lower(cfg.imex_gmres_precision) == "mixed" ? EC_IMEX_PRECISION_MIXED : EC_IMEX_PRECISION_FP64;
 
-  const bool device_loop = cfg.method == "explicit" && snaps.empty();
+  // the cylindrical source runs between updates: host-dt loop
+  const bool device_loop = cfg.method == "explicit" && snaps.empty() && !cfg.cylindrical;
   if (device_loop && status == "ok") {
     // the explicit branch entirely on the device, in chunks
     ec_run_state rs{};
@@ -1066,6 +1135,8 @@ int main(int argc, char** argv)
         std::swap(A, B);
         degenerate += sd.degenerate_count;
       }
+      // app/driver_main.hpp:291
+      if (cfg.cylindrical) EC_OK_OR_DIE(ctx, ec_cylindrical_source(ctx, A, &lay, dt, cfg.prob_lo[1]));
       dt_min_run = std::min(dt_min_run, dt);
       dt_max_run = std::max(dt_max_run, dt);
       dt_sum_run += dt;
@@ -1200,7 +1271,7 @@ int main(int argc, char** argv)
   const bool imex = cfg.method == "imex";
   std::cout << "project_amrex_euler_compare_status=" << status << "\n";
   std::cout << "failure_category=" << failure << "\n";
-  std::cout << "problem=" << problem_name(cfg.problem_kind) << "\n";
+  std::cout << "problem=" << problem_name(cfg.problem_kind, cfg.cylindrical) << "\n";
   std::cout << "method=" << cfg.method << "\n";
   std::cout << "riemann=" << (imex ? "not_applicable" : riemann_name(cfg.riemann_kind)) << "\n";
   std::cout << "imex_form=" << (imex ? "bdltv20_t1_s2_source_map_picard" : "not_applicable") << "\n";
@@ -1286,7 +1357,7 @@ int main(int argc, char** argv)
              "validation_claim,gfm_used,level_set_used,material_count,"
              "cylindrical_source_used,geometric_source_form\n";
       const std::string run_id = scheme + "_" + std::to_string(nx) + "x" + std::to_string(ny);
-      out << kSbCaseId << ',' << csv_escape(run_id) << ",amrex_single_level," << cfg.method << ','
+      out << sb_case_id(cfg) << ',' << csv_escape(run_id) << ",amrex_single_level," << cfg.method << ','
           << csv_escape(scheme) << ',' << csv_escape(order) << ',' << rl << ',' << cfg.gamma << ',' << nx << ','
           << ny << ',' << cfg.prob_lo[0] << ',' << cfg.prob_hi[0] << ',' << cfg.prob_lo[1] << ','
           << cfg.prob_hi[1] << ',' << 0.2 << ',' << 2.8181818181818183 << ',' << 1.6064386578049976 << ','
@@ -1298,9 +1369,10 @@ int main(int argc, char** argv)
           << ',' << sm.pressure_max << ',' << sm.internal_energy_min << ',' << sm.nonfinite_count << ','
           << sm.density_bubble_centroid_x << ',' << sm.density_bubble_centroid_y << ','
           << sm.density_bubble_area_threshold << ',' << sm.schlieren_max << ',' << snap_status << ','
-          << csv_escape(cfg.snapshot_dir) << ',' << kSbClaim << ",false,false,1,false,none\n";
+          << csv_escape(cfg.snapshot_dir) << ',' << sb_claim(cfg) << ",false,false,1,"
+          << sb_source_columns(cfg) << '\n';
     }
-    std::cout << "source_case_id=" << kSbCaseId << "\n";
+    std::cout << "source_case_id=" << sb_case_id(cfg) << "\n";
     std::cout << "scheme_name=" << scheme << "\n";
     std::cout << "order=" << order << "\n";
     std::cout << "gamma=" << cfg.gamma << "\n";
@@ -1327,12 +1399,14 @@ int main(int argc, char** argv)
     std::cout << "density_bubble_area_threshold=" << sm.density_bubble_area_threshold << "\n";
     std::cout << "schlieren_max=" << sm.schlieren_max << "\n";
     std::cout << "internal_energy_min=" << sm.internal_energy_min << "\n";
-    std::cout << "validation_claim=" << kSbClaim << "\n";
+    std::cout << "validation_claim=" << sb_claim(cfg) << "\n";
     std::cout << "gfm_used=false\n";
     std::cout << "level_set_used=false\n";
     std::cout << "material_count=1\n";
-    std::cout << "cylindrical_source_used=false\n";
-    std::cout << "geometric_source_form=none\n";
+    // app/driver_main.hpp:625-633
+    std::cout << "cylindrical_source_used=" << (cfg.cylindrical ? "true" : "false") << "\n";
+    std::cout << "geometric_source_form="
+              << (cfg.cylindrical ? "radial_symmetry_fractional_source_step" : "none") << "\n";
   }
   HIP_OK(hipFree(A));
   HIP_OK(hipFree(B));

@@ -503,6 +503,17 @@ ec_status ec_shock_bubble_metrics(ec_ctx* ctx, const double* U, const ec_layout*
                                   const double* prob_lo, double threshold,
                                   ec_shock_bubble_metrics_out* out);
 
+/* ----------------------------------------------------- geometric source */
+/* Replaces void apply_shock_density_bubble_cylindrical_source(MultiFab&,
+ * const Geometry&, const RunConfig&, Real dt)
+ * (problems/boundary_conditions.hpp:75-131), which the driver calls after
+ * every update of the cylindrical shock-density bubble
+ * (app/driver_main.hpp:291).  Reads y as the radius r = y_lo + (j + 1/2) dy
+ * and integrates dq/dt = -(rho v, rho u v, rho v^2, v (E + p)) / r over dt
+ * in place with the midpoint rule; rows with r <= 0 are left alone.  Queued
+ * on the context's stream; returns without synchronising. */
+ec_status ec_cylindrical_source(ec_ctx* ctx, double* U, const ec_layout* layout, double dt, double y_lo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -932,3 +932,38 @@ void ecor_shock_bubble_metrics(const ecor_grid* g, const double* U, double xlo, 
   out[8] = schl;
   out[9] = bad_count;
 }
+
+/* apply_shock_density_bubble_cylindrical_source
+ * (problems/boundary_conditions.hpp:75-131): y is the radius; the source
+ * -(rho v, rho u v, rho v^2, v (E + p)) / r is integrated over dt by the
+ * midpoint rule, evaluated at q0 for the half step and at the half-step
+ * state for the full step from q0.  Rows with r <= 0 are skipped. */
+void ecor_cylindrical_source(const ecor_grid* g, double* U, double dt, double ylo)
+{
+  const int nx = g->nx, ny = g->ny;
+  const long plane = (long)nx * ny;
+  const double gm1 = g->gamma - 1.0;
+  const double half_dt = 0.5 * dt;
+  for (int j = 0; j < ny; ++j) {
+    const double r = ylo + ((double)j + 0.5) * g->dy;
+    if (!(r > 0.0)) continue;
+    for (int i = 0; i < nx; ++i) {
+      const long k = (long)j * nx + i;
+      const double rho0 = U[k], mx0 = U[plane + k], my0 = U[2 * plane + k], E0 = U[3 * plane + k];
+      const double ir0 = rho0 > 0.0 ? 1.0 / rho0 : 0.0;
+      const double u0 = mx0 * ir0, v0 = my0 * ir0;
+      const double p0 = gm1 * (E0 - 0.5 * (mx0 * mx0 + my0 * my0) * ir0);
+      const double rhos = rho0 - half_dt * my0 / r;
+      const double mxs = mx0 - half_dt * rho0 * u0 * v0 / r;
+      const double mys = my0 - half_dt * rho0 * v0 * v0 / r;
+      const double Es = E0 - half_dt * v0 * (E0 + p0) / r;
+      const double irs = rhos > 0.0 ? 1.0 / rhos : 0.0;
+      const double us = mxs * irs, vs = mys * irs;
+      const double ps = gm1 * (Es - 0.5 * (mxs * mxs + mys * mys) * irs);
+      U[k] = rho0 - dt * mys / r;
+      U[plane + k] = mx0 - dt * rhos * us * vs / r;
+      U[2 * plane + k] = my0 - dt * rhos * vs * vs / r;
+      U[3 * plane + k] = E0 - dt * vs * (Es + ps) / r;
+    }
+  }
+}

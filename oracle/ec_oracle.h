@@ -158,6 +158,10 @@ void ecor_gresho_metrics(const ecor_grid* g, const double* U, double xlo, double
 void ecor_shock_bubble_metrics(const ecor_grid* g, const double* U, double xlo, double ylo,
                                double threshold, double* out);
 
+/* ---- the cylindrical shock bubble's geometric source, in place
+ * (problems/boundary_conditions.hpp:75-131) */
+void ecor_cylindrical_source(const ecor_grid* g, double* U, double dt, double ylo);
+
 /* ---- direct BDLTV20 T1/S2 step (ec_oracle_direct.c) */
 typedef struct {
   ecor_exact exact; /* problem and periodicity; kind TORO or GRESHO gives the

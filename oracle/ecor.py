@@ -154,6 +154,7 @@ def lib():
         L.ecor_admissibility.argtypes = [G, _dp, _dp, C.POINTER(C.c_int)]
         L.ecor_gresho_metrics.argtypes = [G, _dp] + [C.c_double] * 7 + [_dp]
         L.ecor_shock_bubble_metrics.argtypes = [G, _dp, C.c_double, C.c_double, C.c_double, _dp]
+        L.ecor_cylindrical_source.argtypes = [G, _dp, C.c_double, C.c_double]
         L.ecor_exact_ghost.argtypes = [G, C.c_int, C.c_int]
         L.ecor_exact_ghost.restype = C.c_int
         L.ecor_exact_state.argtypes = [C.POINTER(Exact), C.c_int, C.c_int, C.c_double, C.c_int,
@@ -356,6 +357,13 @@ def shock_bubble_metrics(g: Grid, U, xlo, ylo, threshold=0.55) -> dict:
     o = np.zeros(10)
     lib().ecor_shock_bubble_metrics(C.byref(g), _ptr(U), xlo, ylo, threshold, _ptr(o))
     return dict(zip(SB_METRIC_KEYS, o))
+
+
+def cylindrical_source(g: Grid, U, dt, ylo=0.0):
+    """The cylindrical shock bubble's geometric source over dt (a new array)."""
+    U = np.array(U, dtype=np.float64, order="C", copy=True)
+    lib().ecor_cylindrical_source(C.byref(g), _ptr(U), dt, ylo)
+    return U
 
 
 def max_rate(g: Grid, U):
