@@ -55,6 +55,12 @@ struct GmCtl {
   unsigned int counter;
   int floor_count, nonfinite, guard_count, mom_bad;
   double coupled_res, mat_max, aco_max;
+  // closeout evaluation (imex/imex_source_map_step.hpp:75-258): the finite
+  // star density's minimum (negated), the record of CloseCol, the eq78
+  // divergence maximum and the count of cells it covers
+  double star_rho_neg_max, star_pad;
+  double close[8];
+  double eq78, eq78_count;
 };
 
 template <typename T>
@@ -370,6 +376,86 @@ __global__ void __launch_bounds__(RB) finalize_pair_kernel(const double* partial
   }
 }
 
+// The closeout's record per workgroup: maxima and minima over the finite
+// cells (evaluate_source_map_final, imex/imex_source_map_step.hpp:151-166).
+enum CloseCol {
+  CL_RES = 0,        // max |p/(g-1) - E + K|
+  CL_RHO_MIN,        // min rho
+  CL_P_MIN,          // min solved p
+  CL_P_MAX,          // max solved p
+  CL_EOS_MIN,        // min (g-1)(E - K)
+  CL_INTERNAL_MIN,   // min E - K
+  CL_MOM_CORR,       // max |m - m*|
+  CL_E_DELTA,        // max |E - E*|
+  CLOSE_K
+};
+__host__ __device__ constexpr bool close_is_min(int c)
+{
+  return c == CL_RHO_MIN || c == CL_P_MIN || c == CL_EOS_MIN || c == CL_INTERNAL_MIN;
+}
+
+__device__ __forceinline__ void close_record_init(double (&v)[CLOSE_K])
+{
+#pragma unroll
+  for (int c = 0; c < CLOSE_K; ++c) v[c] = close_is_min(c) ? INFINITY : -INFINITY;
+  v[CL_RES] = 0.0;
+  v[CL_MOM_CORR] = 0.0;
+  v[CL_E_DELTA] = 0.0;
+}
+
+__device__ __forceinline__ double close_fold(int c, double a, double b)
+{
+  return close_is_min(c) ? smin(a, b) : smax(a, b);
+}
+
+// Folds the record over the workgroup (wave shuffles, then the waves in
+// order) and stores it as partials[CLOSE_K * blockIdx.x + c].
+__device__ void close_record_store(double (&v)[CLOSE_K], double* partials)
+{
+  __shared__ double sh[RB / 64][CLOSE_K];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < CLOSE_K; ++c)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[c] = close_fold(c, v[c], __shfl_xor(v[c], off));
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < CLOSE_K; ++c) sh[wave][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x < CLOSE_K) {
+    const int c = threadIdx.x;
+    double a = sh[0][c];
+    for (int w = 1; w < RB / 64; ++w) a = close_fold(c, a, sh[w][c]);
+    partials[(long long)CLOSE_K * blockIdx.x + c] = a;
+  }
+}
+
+// One workgroup folds the closeout records of nparts workgroups into out.
+__global__ void __launch_bounds__(RB) finalize_close_kernel(const double* partials, int nparts, double* out)
+{
+  double v[CLOSE_K];
+  close_record_init(v);
+  for (int q = threadIdx.x; q < nparts; q += RB)
+#pragma unroll
+    for (int c = 0; c < CLOSE_K; ++c) v[c] = close_fold(c, v[c], partials[(long long)CLOSE_K * q + c]);
+  __shared__ double sh[RB / 64][CLOSE_K];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < CLOSE_K; ++c)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[c] = close_fold(c, v[c], __shfl_xor(v[c], off));
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < CLOSE_K; ++c) sh[wave][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x < CLOSE_K) {
+    const int c = threadIdx.x;
+    double a = sh[0][c];
+    for (int w = 1; w < RB / 64; ++w) a = close_fold(c, a, sh[w][c]);
+    out[c] = a;
+  }
+}
+
 // ------------------------------------------------------------ star update
 // reconstruction_fluxes.hpp:26-52 and :462-515; the explicit ghost rule of
 // the state is the same as the explicit step's.
@@ -440,9 +526,10 @@ __global__ void __launch_bounds__(256) imex_star_kernel(const double* __restrict
 // density floor count (imex_source_map_step.hpp:418-451).
 __global__ void __launch_bounds__(256) imex_init_kernel(const double* __restrict__ U, long long rs, long long cs, double gamma,
                                  int nx, int ny, const double* __restrict__ S, double* __restrict__ P,
-                                 double* __restrict__ M, GmCtl* ctl)
+                                 double* __restrict__ M, double* partials, GmCtl* ctl)
 {
   const long long n = (long long)nx * ny;
+  double neg = -INFINITY;  // -min of the finite star densities
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(t % nx), j = (int)(t / nx);
@@ -452,7 +539,9 @@ __global__ void __launch_bounds__(256) imex_init_kernel(const double* __restrict
     M[n + t] = S[2 * n + t];
     const double rho = S[t];
     if (!isfinite(rho) || rho <= 1.0e-14) atomicAdd(&ctl->floor_count, 1);
+    if (isfinite(rho)) neg = smax(neg, -rho);
   }
+  reduce_maxsum_store(neg, 0.0, partials);
 }
 
 // Face enthalpy (and, for the closeout, energy flux) on every face
@@ -1410,7 +1499,8 @@ __global__ void __launch_bounds__(256) closeout_kernel(ImexGrid G, const double*
 {
   const long long n = (long long)G.nx * G.ny;
   const double gm1 = G.gamma - 1.0;
-  double rmax = 0.0;
+  double v[CLOSE_K];
+  close_record_init(v);
   int bad = 0;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
@@ -1419,19 +1509,83 @@ __global__ void __launch_bounds__(256) closeout_kernel(ImexGrid G, const double*
     const double fn = fy[(long long)(j + 1) * G.nx + i], fs = fy[(long long)j * G.nx + i];
     const double div = (fe - fw) * G.inv_dx + (fn - fs) * G.inv_dy;
     const double E = S[3 * n + t] - G.dt * div;
-    const double mx = M[t], my = M[n + t], rho = S[t];
+    const double mx = M[t], my = M[n + t], rho = S[t], p = P[t];
     const double kin = 0.5 * (mx * mx + my * my) / rho;
-    const double res = P[t] / gm1 - E + kin;
-    if (isfinite(res)) rmax = smax(rmax, fabs(res));
-    else ++bad;
+    const double internal = E - kin;
+    const double eos = gm1 * internal;
+    const double res = p / gm1 - E + kin;
     const long long k = (long long)j * rs + i;
     U[k] = rho;
     U[cs + k] = mx;
     U[2 * cs + k] = my;
     U[3 * cs + k] = E;
+    if (!isfinite(rho) || !isfinite(mx) || !isfinite(my) || !isfinite(E) || !isfinite(p) || !isfinite(kin) ||
+        !isfinite(internal) || !isfinite(eos) || !isfinite(res) || !isfinite(div)) {
+      ++bad;
+      continue;
+    }
+    const double dmx = mx - S[n + t], dmy = my - S[2 * n + t];
+    v[CL_RES] = smax(v[CL_RES], fabs(res));
+    v[CL_RHO_MIN] = smin(v[CL_RHO_MIN], rho);
+    v[CL_P_MIN] = smin(v[CL_P_MIN], p);
+    v[CL_P_MAX] = smax(v[CL_P_MAX], p);
+    v[CL_EOS_MIN] = smin(v[CL_EOS_MIN], eos);
+    v[CL_INTERNAL_MIN] = smin(v[CL_INTERNAL_MIN], internal);
+    v[CL_MOM_CORR] = smax(v[CL_MOM_CORR], sqrt(dmx * dmx + dmy * dmy));
+    v[CL_E_DELTA] = smax(v[CL_E_DELTA], fabs(E - S[3 * n + t]));
   }
   if (bad) atomicAdd(&ctl->nonfinite, bad);
-  reduce_maxsum_store(rmax, 0.0, partials);
+  close_record_store(v, partials);
+}
+
+// The eq78 velocity divergence of the closed state (imex/
+// imex_source_map_step.hpp:207-227): central differences of the velocities,
+// the neighbours wrapped on a periodic axis and mirrored at a physical side
+// (source_map_index, imex/imex_source_map_helpers.hpp:169-188); a velocity
+// reads NaN where the density is zero or either value non-finite, and cells
+// whose divergence is not finite are skipped.  Partials: (max |div|, count).
+__device__ __forceinline__ int source_map_mirror(int i, int n, bool periodic)
+{
+  if (periodic) return ((i % n) + n) % n;
+  if (i < 0) return min(-i - 1, n - 1);
+  if (i >= n) return max(n - (i - n + 1), 0);
+  return i;
+}
+
+__device__ __forceinline__ double closed_velocity(const ImexGrid& G, const double* U, long long rs, long long cs,
+                                                  int i, int j, int axis)
+{
+  const int ii = source_map_mirror(i, G.nx, G.bc[0] == BC_PERIODIC);
+  const int jj = source_map_mirror(j, G.ny, G.bc[2] == BC_PERIODIC);
+  const long long k = (long long)jj * rs + ii;
+  const double rho = U[k], m = U[(axis == 0 ? cs : 2 * cs) + k];
+  if (!isfinite(rho) || rho == 0.0 || !isfinite(m)) return NAN;
+  return m / rho;
+}
+
+__global__ void __launch_bounds__(256) eq78_kernel(ImexGrid G, const double* __restrict__ U, long long rs,
+                                                   long long cs, double* partials)
+{
+  const long long n = (long long)G.nx * G.ny;
+  const double den_scale = smax(G.inv_dx, G.inv_dy);
+  double dmax = 0.0, count = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % G.nx), j = (int)(t / G.nx);
+    const double ue = closed_velocity(G, U, rs, cs, i + 1, j, 0);
+    const double uw = closed_velocity(G, U, rs, cs, i - 1, j, 0);
+    const double vn = closed_velocity(G, U, rs, cs, i, j + 1, 1);
+    const double vs = closed_velocity(G, U, rs, cs, i, j - 1, 1);
+    const double div = 0.5 * (ue - uw) * G.inv_dx + 0.5 * (vn - vs) * G.inv_dy;
+    if (!isfinite(div)) continue;
+    // velocity_scale >= 1 and finite here (every velocity is finite when div is)
+    const double scale = smax(smax(smax(smax(fabs(ue), fabs(uw)), fabs(vn)), fabs(vs)), 1.0);
+    const double den = scale * den_scale;
+    if (!isfinite(den) || den <= 0.0) continue;
+    dmax = smax(dmax, fabs(div));
+    count += 1.0;
+  }
+  reduce_maxsum_store(dmax, count, partials);
 }
 
 // compute_imex_dt reductions (diagnostic_metrics_timestep.hpp:262-326): the
@@ -2025,6 +2179,10 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
   ImexWork* w = static_cast<ImexWork*>(ctx->imex);
   std::memset(diag, 0, sizeof *diag);
   diag->status = EC_IMEX_NOT_RUN;
+  // core/imex_diagnostics.hpp:111-163 defaults, kept by the early returns
+  diag->solved_pressure_min = diag->solved_pressure_max = NAN;
+  diag->trial_density_min = diag->trial_pressure_min = diag->trial_internal_energy_min = NAN;
+  diag->eq78_divergence_linf = NAN;
   const long long n = (long long)w->nx * w->ny;
   const int nb = grid_blocks(n);
   const long long nf = (long long)(w->nx + 1) * w->ny + (long long)w->nx * (w->ny + 1);
@@ -2040,9 +2198,14 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
   hipLaunchKernelGGL(imex_star_kernel, dim3(nb), dim3(RB), 0, ctx->stream, U, lay->row_stride,
                      lay->comp_stride, g, dt, opts->predictor_dissipation, w->star);
   hipLaunchKernelGGL(imex_init_kernel, dim3(nb), dim3(RB), 0, ctx->stream, U, lay->row_stride,
-                     lay->comp_stride, ctx->grid.gamma, w->nx, w->ny, w->star, w->p, w->mom, w->ctl);
+                     lay->comp_stride, ctx->grid.gamma, w->nx, w->ny, w->star, w->p, w->mom, w->partials,
+                     w->ctl);
+  hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, ctx->stream, w->partials, nb, 1,
+                     &w->ctl->star_rho_neg_max, &w->ctl->star_pad);
   EC_HIP_TRY(ctx, hipGetLastError());
   if ((st = hc.pull()) != EC_OK) return st;
+  // imex/imex_source_map_step.hpp:436-444: the finite star densities' minimum
+  diag->trial_density_min = -h.star_rho_neg_max;
   diag->density_floor_count = h.floor_count;
   if (h.floor_count > 0) {
     diag->status = EC_IMEX_DENSITY_FLOOR;
@@ -2116,8 +2279,12 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
                      w->hx, w->hy, nullptr, nullptr, w->fx, w->fy, w->ctl);
   hipLaunchKernelGGL(closeout_kernel, dim3(nb), dim3(RB), 0, ctx->stream, G, w->star, w->mom, w->p,
                      w->fx, w->fy, U, lay->row_stride, lay->comp_stride, w->partials, w->ctl);
+  hipLaunchKernelGGL(finalize_close_kernel, dim3(1), dim3(RB), 0, ctx->stream, w->partials, nb,
+                     w->ctl->close);
+  hipLaunchKernelGGL(eq78_kernel, dim3(nb), dim3(RB), 0, ctx->stream, G, U, lay->row_stride, lay->comp_stride,
+                     w->partials);
   hipLaunchKernelGGL(finalize_pair_kernel, dim3(1), dim3(RB), 0, ctx->stream, w->partials, nb, 0,
-                     &w->ctl->coupled_res, &w->ctl->r2);
+                     &w->ctl->eq78, &w->ctl->eq78_count);
   if (p_out) {
     EC_HIP_TRY(ctx, hipMemcpyAsync(p_out, w->p, n * sizeof(double), hipMemcpyDeviceToDevice,
                                    ctx->stream));
@@ -2126,7 +2293,19 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
   if ((st = hc.pull()) != EC_OK) return st;
   diag->guard_count = h.guard_count;
   diag->nonfinite_count = h.nonfinite;
-  diag->coupled_residual_linf = h.coupled_res;
+  // imex/imex_source_map_step.hpp:229-257: the residual and the divergence
+  // read infinite once a cell is not finite; extrema without a finite cell
+  // read NaN
+  auto finite_or_nan = [](double v) { return std::isfinite(v) ? v : NAN; };
+  diag->coupled_residual_linf = h.nonfinite > 0 ? INFINITY : h.close[CL_RES];
+  diag->trial_density_min = finite_or_nan(h.close[CL_RHO_MIN]);
+  diag->solved_pressure_min = finite_or_nan(h.close[CL_P_MIN]);
+  diag->solved_pressure_max = finite_or_nan(h.close[CL_P_MAX]);
+  diag->trial_pressure_min = finite_or_nan(h.close[CL_EOS_MIN]);
+  diag->trial_internal_energy_min = finite_or_nan(h.close[CL_INTERNAL_MIN]);
+  diag->momentum_correction_linf = h.close[CL_MOM_CORR];
+  diag->energy_correction_linf = h.close[CL_E_DELTA];
+  diag->eq78_divergence_linf = (h.nonfinite > 0 || h.eq78_count == 0.0) ? INFINITY : h.eq78;
   return EC_OK;
 }
 

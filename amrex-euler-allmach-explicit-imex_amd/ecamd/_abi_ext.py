@@ -28,7 +28,11 @@ class ImexDiag(C.Structure):
                 ("density_floor_count", C.c_int), ("guard_count", C.c_int),
                 ("nonfinite_count", C.c_int), ("residual_linf", C.c_double),
                 ("relative_residual_linf", C.c_double), ("rhs_norm", C.c_double),
-                ("coupled_residual_linf", C.c_double)]
+                ("coupled_residual_linf", C.c_double), ("trial_density_min", C.c_double),
+                ("trial_pressure_min", C.c_double), ("trial_internal_energy_min", C.c_double),
+                ("solved_pressure_min", C.c_double), ("solved_pressure_max", C.c_double),
+                ("momentum_correction_linf", C.c_double), ("energy_correction_linf", C.c_double),
+                ("eq78_divergence_linf", C.c_double)]
 
 
 class DirectDiag(C.Structure):

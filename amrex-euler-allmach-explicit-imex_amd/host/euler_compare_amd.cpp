@@ -120,6 +120,9 @@ struct Cfg {
   // the longest device restart comes closest (GMRES(30) stagnates on some
   // transmissive-boundary systems that the unrestarted solve converges).
   int imex_gmres_restart = 64;
+  // core/run_config.hpp:29-49
+  std::string imex_pressure_solver_path = "host_single_level_gmres";
+  int imex_picard_extra_iterations = 1, imex_solver_verbose = 0;
   int problem_kind = EC_PROBLEM_ADVECTION_BLOB, riemann_kind = 1, limiter_kind = 0;
   // ShockDensityBubbleCylindrical: the shock bubble plus the radial geometric
   // source after every update (core/solver_types.hpp:8-9)
@@ -235,6 +238,9 @@ Cfg read_config(const Params& pp)
   pp.query("euler.imex_picard_iterations", c.imex_picard_iterations);
   pp.query("euler.imex_gmres_precision", c.imex_gmres_precision);
   pp.query("euler.imex_gmres_restart", c.imex_gmres_restart);
+  pp.query("euler.imex_pressure_solver_path", c.imex_pressure_solver_path);
+  pp.query("euler.imex_picard_extra_iterations", c.imex_picard_extra_iterations);
+  pp.query("euler.imex_solver_verbose", c.imex_solver_verbose);
   pp.query("euler.problem", c.problem);
   pp.query("euler.riemann", c.riemann);
   pp.query("euler.method", c.method);
@@ -262,6 +268,7 @@ Cfg read_config(const Params& pp)
   pp.query("gamma", c.gamma);
   pp.query("cfl", c.cfl);
   pp.query("final_time", c.stop_time);
+  pp.query("snapshot_dir", c.snapshot_dir);
   pp.query("snapshot_times", c.snapshot_times);
   for (int d = 0; d < 2; ++d) {
     pp.query("euler.velocity", c.velocity[d], d);
@@ -291,6 +298,7 @@ Cfg read_config(const Params& pp)
     abort_msg("euler.field_boundary must be outflow or exact_dirichlet.");
   }
   if (c.spatial_order != 1 && c.spatial_order != 2) abort_msg("euler.spatial_order must be 1 or 2.");
+  if (c.imex_picard_extra_iterations < 0) abort_msg("euler.imex_picard_extra_iterations must be non-negative.");
   if (c.imex_picard_iterations < 1) abort_msg("euler.imex_picard_iterations must be at least one.");
   c.bdltv20_paper_t1_s2 = lower(c.bdltv20_paper_t1_s2);
   c.bdltv20_paper_pressure_solver = lower(c.bdltv20_paper_pressure_solver);
@@ -346,6 +354,9 @@ Cfg read_config(const Params& pp)
   if (c.method == "imex") {
     // core/config_reader.hpp:177-196
     if (c.spatial_order != 2) abort_msg("euler.method=imex requires euler.spatial_order=2 in this code pack.");
+    if (c.imex_pressure_solver_path != "host_single_level_gmres") {
+      abort_msg("IMEX rows require euler.imex_pressure_solver_path=host_single_level_gmres.");
+    }
     if (c.imex_pressure_stabilization != "off") {
       abort_msg("IMEX rows in this code pack use euler.imex_pressure_stabilization=off.");
     }
@@ -1035,10 +1046,27 @@ int main(int argc, char** argv)
   };
   if (cfg.plot_int >= 0) plot_now(false);
   ec_imex_diag last_imex{};
+  // core/imex_diagnostics.hpp:92-163 defaults (a run that takes no IMEX step prints them)
+  last_imex.residual_linf = last_imex.relative_residual_linf = std::nan("");
+  last_imex.solved_pressure_min = last_imex.solved_pressure_max = std::nan("");
+  last_imex.trial_density_min = last_imex.trial_pressure_min = std::nan("");
+  last_imex.trial_internal_energy_min = last_imex.eq78_divergence_linf = std::nan("");
+  last_imex.status = EC_IMEX_NOT_RUN;
   int imex_iter_max = 0;
   // update_high_trial_run over the IMEX steps (numerics/diagnostic_metrics_timestep.hpp:122-131)
   double adm_rho_min = INFINITY, adm_p_min = INFINITY, adm_e_min = INFINITY;
-  double imex_res_max = 0.0, imex_rel_max = 0.0, mat_max_run = 0.0, aco_max_run = 0.0;
+  double mat_max_run = 0.0, aco_max_run = 0.0;
+  // update_imex_maxima (app/driver_main.hpp:49-150): residual and eq78 maxima
+  // skip non-finite values and start undefined; the corrections start at 0
+  auto max_real = [](double current, double trial) {
+    if (!std::isfinite(current)) return trial;
+    if (!std::isfinite(trial)) return current;
+    return std::max(current, trial);
+  };
+  const double nan = std::nan("");
+  double imex_res_max = nan, imex_rel_max = nan, eq78_max = nan;
+  double mom_corr_max = 0.0, e_corr_max = 0.0;
+  int solves_max = 0;
   ec_imex_opts io;
   ec_imex_default_opts(&io);
   io.picard_iterations = cfg.imex_picard_iterations;
@@ -1114,8 +1142,12 @@ int main(int argc, char** argv)
         EC_OK_OR_DIE(ctx, ec_imex_step(ctx, A, &lay, dt, &io, &last_imex, nullptr));
         ++imex_steps;
         imex_iter_max = std::max(imex_iter_max, last_imex.iterations);
-        imex_res_max = std::max(imex_res_max, last_imex.residual_linf);
-        imex_rel_max = std::max(imex_rel_max, last_imex.relative_residual_linf);
+        imex_res_max = max_real(imex_res_max, last_imex.residual_linf);
+        imex_rel_max = max_real(imex_rel_max, last_imex.relative_residual_linf);
+        solves_max = std::max(solves_max, last_imex.picard_solves);
+        mom_corr_max = std::max(mom_corr_max, last_imex.momentum_correction_linf);
+        e_corr_max = std::max(e_corr_max, last_imex.energy_correction_linf);
+        eq78_max = max_real(eq78_max, last_imex.eq78_divergence_linf);
         if (last_imex.status != EC_IMEX_CONVERGED_ABSOLUTE && last_imex.status != EC_IMEX_CONVERGED_RELATIVE) {
           status = "failed";
           failure = sb ? "imex_pressure_solve_nonconverged"
@@ -1297,7 +1329,14 @@ int main(int argc, char** argv)
   std::cout << "hllc_degenerate_star_count=" << degenerate << "\n";
   std::cout << "imex_step_count=" << imex_steps << "\n";
   if (imex) {
+    // app/driver_main.hpp:451-544 (IMEX rows run with acoustic startup off
+    // and no cap, so the eq79 rule applies and neither was used)
     std::cout << "imex_cfl=" << cfg.imex_cfl << "\n";
+    std::cout << "imex_acoustic_cfl_cap=" << cfg.imex_acoustic_cfl_cap << "\n";
+    std::cout << "imex_timestep_rule=material_speed_eq79_zdrop_2d\n";
+    std::cout << "imex_acoustic_startup_enabled=" << cfg.imex_acoustic_startup << "\n";
+    std::cout << "imex_acoustic_startup_steps=0\n";
+    std::cout << "imex_acoustic_cap_steps=0\n";
     std::cout << "imex_timestep_material_rate_max=" << mat_max_run << "\n";
     std::cout << "imex_timestep_acoustic_rate_max=" << aco_max_run << "\n";
     std::cout << "imex_solver=device_gmres_matrix_free_source_map_pressure_row\n";
@@ -1308,8 +1347,45 @@ int main(int argc, char** argv)
     std::cout << "imex_solver_residual_max=" << imex_res_max << "\n";
     std::cout << "imex_solver_relative_residual_last=" << last_imex.relative_residual_linf << "\n";
     std::cout << "imex_solver_relative_residual_max=" << imex_rel_max << "\n";
-    std::cout << "imex_gmres_precision=" << (io.precision ? "mixed" : "fp64") << "\n";
+    std::cout << "imex_pressure_solver_path=device_single_level_gmres\n";
+    std::cout << "imex_pressure_operator_label="
+              << "bdltv20_o1_source_picard_eq49_50_52_eq76_77_zdrop_2d_pressure_row_no_eq51_claim\n";
+    std::cout << "imex_pressure_row_stencil="
+              << "eq49_76_lhs_enthalpy_eq50_77_rhs_star_energy_lagged_kinetic_star_momentum_eq52_"
+              << "host_sparse_zdrop_2d_no_eq51_claim\n";
+    std::cout << "imex_pressure_energy_form=bdltv20_t1_s2_source_map_picard\n";
+    std::cout << "imex_source_primary_paper=bdltv20_2020_jcp_415_109486\n";
+    std::cout << "imex_source_equation_set="
+              << "eq25_28_split_eq43_55_source_map_eq69_79_zdrop_2d_plus_eq87_89_conserved_minmod_"
+              << "explicit_advection\n";
+    std::cout << "imex_source_time_order=first_order_time_t1\n";
+    std::cout << "imex_source_space_order=second_order_space_s2\n";
+    std::cout << "imex_source_advection_reconstruction="
+              << "bdltv20_eq87_89_conserved_variable_minmod_explicit_toro_vazquez_star_update\n";
+    // the kinetic lag is refreshed once per Picard solve (imex/imex_source_map_step.hpp:558-564)
     std::cout << "imex_picard_pressure_solves_used=" << last_imex.picard_solves << "\n";
+    std::cout << "imex_picard_pressure_solves_used_max=" << solves_max << "\n";
+    std::cout << "imex_kinetic_lag_update_count=" << last_imex.picard_solves << "\n";
+    std::cout << "imex_kinetic_lag_update_count_max=" << solves_max << "\n";
+    std::cout << "imex_momentum_correction_linf_last=" << last_imex.momentum_correction_linf << "\n";
+    std::cout << "imex_momentum_correction_linf_max=" << mom_corr_max << "\n";
+    std::cout << "imex_energy_correction_linf_last=" << last_imex.energy_correction_linf << "\n";
+    std::cout << "imex_energy_correction_linf_max=" << e_corr_max << "\n";
+    std::cout << "imex_face_enthalpy_policy="
+              << "bdltv20_eq46_corrected_momentum_weighted_face_enthalpy_with_tolerance_guard\n";
+    std::cout << "imex_face_momentum_policy=centered_eq54_eq71_72_corrected_normal_momentum\n";
+    std::cout << "imex_eq78_divergence_linf_last=" << last_imex.eq78_divergence_linf << "\n";
+    std::cout << "imex_eq78_divergence_linf_max=" << eq78_max << "\n";
+    // the closeout delta is the energy correction (imex/imex_source_map_step.hpp:163-165)
+    std::cout << "imex_energy_closeout_delta_linf_last=" << last_imex.energy_correction_linf << "\n";
+    std::cout << "imex_energy_closeout_delta_linf_max=" << e_corr_max << "\n";
+    std::cout << "imex_solved_pressure_min=" << last_imex.solved_pressure_min << "\n";
+    std::cout << "imex_solved_pressure_max=" << last_imex.solved_pressure_max << "\n";
+    std::cout << "imex_trial_density_min=" << last_imex.trial_density_min << "\n";
+    std::cout << "imex_trial_pressure_min=" << last_imex.trial_pressure_min << "\n";
+    std::cout << "imex_trial_internal_energy_min=" << last_imex.trial_internal_energy_min << "\n";
+    std::cout << "imex_nonfinite_count_last=" << last_imex.nonfinite_count << "\n";
+    std::cout << "imex_gmres_precision=" << (io.precision ? "mixed" : "fp64") << "\n";
   }
   // app/driver_main.hpp:550-571 and :609-620: the problem metrics of the final state
   if (cfg.problem_kind == EC_PROBLEM_GRESHO) {

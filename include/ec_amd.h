@@ -215,7 +215,19 @@ typedef struct {
   double residual_linf;      /* |A p - b|_inf of the last solve */
   double relative_residual_linf;
   double rhs_norm;           /* max(|b|_inf, 1) */
-  double coupled_residual_linf; /* max |p/(g-1) - E + K| after the closeout */
+  double coupled_residual_linf; /* max |p/(g-1) - E + K| after the closeout (inf once a
+                                   cell is not finite) */
+  /* the closeout's evaluation (evaluate_source_map_final,
+   * imex/imex_source_map_step.hpp:75-258), over the cells whose values are
+   * finite; NaN where no cell is (or the step ended before the closeout) */
+  double trial_density_min;       /* min rho (before the closeout: of the star) */
+  double trial_pressure_min;      /* min (g-1)(E - K) */
+  double trial_internal_energy_min; /* min E - K */
+  double solved_pressure_min, solved_pressure_max;
+  double momentum_correction_linf; /* max |m - m*| (0 before the closeout) */
+  double energy_correction_linf;  /* max |E - E*|, also the closeout delta (0 before) */
+  double eq78_divergence_linf;    /* max |central velocity divergence| (inf once a cell
+                                     is not finite) */
 } ec_imex_diag;
 
 /* Defaults of core/run_config.hpp:2-55 (2 Picard solves, tol 1e-10,

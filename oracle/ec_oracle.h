@@ -134,6 +134,10 @@ typedef struct {
   int nonfinite_count;
   double residual_linf, relative_residual_linf, rhs_norm;
   double coupled_residual_linf;
+  /* evaluate_source_map_final (imex/imex_source_map_step.hpp:75-258) */
+  double trial_density_min, trial_pressure_min, trial_internal_energy_min;
+  double solved_pressure_min, solved_pressure_max;
+  double momentum_correction_linf, energy_correction_linf, eq78_divergence_linf;
 } ecor_imex_result;
 
 void ecor_imex_star(const ecor_grid* g, const double* U, double dt, int acoustic, double* S);

@@ -494,6 +494,53 @@ static void fixed_pressures(const ecor_grid* g, double* fp)
   }
 }
 
+
+/* The closed state's eq78 velocity divergence (imex/
+ * imex_source_map_step.hpp:207-244): central differences, neighbours wrapped
+ * on a periodic axis and mirrored at a physical side (source_map_index,
+ * imex/imex_source_map_helpers.hpp:169-188), a velocity NaN where the density
+ * is zero or a value non-finite; infinite when no cell has a finite value. */
+static int mirror_index(int i, int n, int periodic)
+{
+  if (periodic) return ((i % n) + n) % n;
+  if (i < 0) { const int a = -i - 1; return a < n - 1 ? a : n - 1; }
+  if (i >= n) { const int a = n - (i - n + 1); return a > 0 ? a : 0; }
+  return i;
+}
+
+static double closed_velocity(const ecor_grid* g, const double* U, int i, int j, int axis)
+{
+  const long plane = (long)g->nx * g->ny;
+  const int ii = mirror_index(i, g->nx, g->bc[0] == ECOR_BC_PERIODIC);
+  const int jj = mirror_index(j, g->ny, g->bc[2] == ECOR_BC_PERIODIC);
+  const long k = (long)jj * g->nx + ii;
+  const double rho = U[k], m = U[(axis == 0 ? 1 : 2) * plane + k];
+  if (!isfinite(rho) || rho == 0.0 || !isfinite(m)) return NAN;
+  return m / rho;
+}
+
+static double eq78_divergence(const ecor_grid* g, const double* U)
+{
+  const double inv_dx = 1.0 / g->dx, inv_dy = 1.0 / g->dy;
+  double linf = 0.0;
+  long count = 0;
+  for (int j = 0; j < g->ny; ++j) {
+    for (int i = 0; i < g->nx; ++i) {
+      const double ue = closed_velocity(g, U, i + 1, j, 0), uw = closed_velocity(g, U, i - 1, j, 0);
+      const double vn = closed_velocity(g, U, i, j + 1, 1), vs = closed_velocity(g, U, i, j - 1, 1);
+      const double div = 0.5 * (ue - uw) * inv_dx + 0.5 * (vn - vs) * inv_dy;
+      double scale = 1.0;
+      const double a[4] = {fabs(ue), fabs(uw), fabs(vn), fabs(vs)};
+      for (int q = 0; q < 4; ++q) scale = scale < a[q] ? a[q] : scale;
+      const double den = scale * (inv_dx < inv_dy ? inv_dy : inv_dx);
+      if (!isfinite(div) || !isfinite(den) || den <= 0.0) continue;
+      linf = smax(linf, fabs(div));
+      ++count;
+    }
+  }
+  return count > 0 ? linf : INFINITY;
+}
+
 /* imex_source_map_step.hpp:300-616 */
 void ecor_imex_step(const ecor_grid* g, double* U, double dt, int picard, double tol, int max_iter,
                     double* p_out, ecor_imex_result* out)
@@ -517,7 +564,15 @@ void ecor_imex_step(const ecor_grid* g, double* U, double dt, int picard, double
   c.p = p;
   fixed_pressures(g, c.fixed_p);
 
+  /* core/imex_diagnostics.hpp:111-163 defaults */
+  out->trial_density_min = out->trial_pressure_min = out->trial_internal_energy_min = NAN;
+  out->solved_pressure_min = out->solved_pressure_max = NAN;
+  out->eq78_divergence_linf = NAN;
   ecor_imex_star(g, U, dt, 0, S);
+  /* imex/imex_source_map_step.hpp:436-444 */
+  out->trial_density_min = INFINITY;
+  for (long k = 0; k < n; ++k)
+    if (isfinite(S[k]) && S[k] < out->trial_density_min) out->trial_density_min = S[k];
   for (long k = 0; k < n; ++k) {
     mom[k] = S[plane + k];
     mom[plane + k] = S[2 * plane + k];
@@ -566,6 +621,8 @@ void ecor_imex_step(const ecor_grid* g, double* U, double dt, int picard, double
     corrected_momentum(&c, dt, mom);
     double* E = (double*)malloc((size_t)n * sizeof(double));
     double res_linf = 0.0;
+    double rho_min = INFINITY, p_min = INFINITY, p_max = -INFINITY, eos_min = INFINITY,
+           int_min = INFINITY, mom_corr = 0.0, e_delta = 0.0;
     for (int j = 0; j < ny; ++j) {
       for (int i = 0; i < nx; ++i) {
         double fe, fw, fn, fs;
@@ -577,13 +634,36 @@ void ecor_imex_step(const ecor_grid* g, double* U, double dt, int picard, double
         const long k = (long)j * nx + i;
         const double div = (fe - fw) * inv_dx + (fn - fs) * inv_dy;
         E[k] = S[3 * plane + k] - dt * div;
-        const double kin = 0.5 * (mom[k] * mom[k] + mom[plane + k] * mom[plane + k]) / S[k];
+        const double mx = mom[k], my = mom[plane + k], rho = S[k];
+        const double kin = 0.5 * (mx * mx + my * my) / rho;
+        const double internal = E[k] - kin;
+        const double eos = gm1 * internal;
         const double r = p[k] / gm1 - E[k] + kin;
-        if (isfinite(r)) res_linf = smax(res_linf, fabs(r));
-        else ++out->nonfinite_count;
+        if (!isfinite(rho) || !isfinite(mx) || !isfinite(my) || !isfinite(E[k]) || !isfinite(p[k]) ||
+            !isfinite(kin) || !isfinite(internal) || !isfinite(eos) || !isfinite(r) || !isfinite(div)) {
+          ++out->nonfinite_count;
+          continue;
+        }
+        rho_min = rho < rho_min ? rho : rho_min;
+        p_min = p[k] < p_min ? p[k] : p_min;
+        p_max = p_max < p[k] ? p[k] : p_max;
+        eos_min = eos < eos_min ? eos : eos_min;
+        int_min = internal < int_min ? internal : int_min;
+        const double dmx = mx - S[plane + k], dmy = my - S[2 * plane + k];
+        mom_corr = smax(mom_corr, sqrt(dmx * dmx + dmy * dmy));
+        e_delta = smax(e_delta, fabs(E[k] - S[3 * plane + k]));
+        res_linf = smax(res_linf, fabs(r));
       }
     }
-    out->coupled_residual_linf = res_linf;
+    /* imex/imex_source_map_step.hpp:229-257 */
+    out->coupled_residual_linf = out->nonfinite_count > 0 ? INFINITY : res_linf;
+    out->trial_density_min = isfinite(rho_min) ? rho_min : NAN;
+    out->solved_pressure_min = isfinite(p_min) ? p_min : NAN;
+    out->solved_pressure_max = isfinite(p_max) ? p_max : NAN;
+    out->trial_pressure_min = isfinite(eos_min) ? eos_min : NAN;
+    out->trial_internal_energy_min = isfinite(int_min) ? int_min : NAN;
+    out->momentum_correction_linf = mom_corr;
+    out->energy_correction_linf = e_delta;
     for (long k = 0; k < n; ++k) {
       U[k] = S[k];
       U[plane + k] = mom[k];
@@ -591,6 +671,8 @@ void ecor_imex_step(const ecor_grid* g, double* U, double dt, int picard, double
       U[3 * plane + k] = E[k];
     }
     free(E);
+    out->eq78_divergence_linf = eq78_divergence(g, U);
+    if (out->nonfinite_count > 0) out->eq78_divergence_linf = INFINITY;
   }
 done:
   if (p_out) memcpy(p_out, p, (size_t)n * sizeof(double));

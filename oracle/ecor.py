@@ -101,7 +101,11 @@ class ImexResult(C.Structure):
                 ("picard_solves", C.c_int), ("density_floor_count", C.c_int),
                 ("guard_count", C.c_int), ("nonfinite_count", C.c_int),
                 ("residual_linf", C.c_double), ("relative_residual_linf", C.c_double),
-                ("rhs_norm", C.c_double), ("coupled_residual_linf", C.c_double)]
+                ("rhs_norm", C.c_double), ("coupled_residual_linf", C.c_double),
+                ("trial_density_min", C.c_double), ("trial_pressure_min", C.c_double),
+                ("trial_internal_energy_min", C.c_double), ("solved_pressure_min", C.c_double),
+                ("solved_pressure_max", C.c_double), ("momentum_correction_linf", C.c_double),
+                ("energy_correction_linf", C.c_double), ("eq78_divergence_linf", C.c_double)]
 
 
 IMEX_STATUS = {0: "converged_absolute", 1: "converged_relative", 2: "linear_solve_not_converged",

@@ -80,6 +80,24 @@ def test_imex_row(tmp_path):
     assert kv["imex_step_count"] == "3"
     assert kv["imex_solver_status"] in ("converged_absolute", "converged_relative")
     assert float(kv["imex_solver_relative_residual_max"]) <= 1e-10
+    # the closeout's evaluation keys (app/driver_main.hpp:451-544)
+    assert kv["imex_picard_pressure_solves_used"] == kv["imex_kinetic_lag_update_count"] == "2"
+    assert kv["imex_timestep_rule"] == "material_speed_eq79_zdrop_2d"
+    assert kv["imex_acoustic_startup_steps"] == kv["imex_acoustic_cap_steps"] == "0"
+    assert kv["imex_nonfinite_count_last"] == "0"
+    assert kv["imex_trial_density_min"] == kv["rho_min"]     # both of the final state
+    pmin, pmax = float(kv["imex_solved_pressure_min"]), float(kv["imex_solved_pressure_max"])
+    assert 714285.0 < pmin <= pmax < 714287.0
+    assert float(kv["imex_trial_pressure_min"]) == pytest.approx(pmin, rel=1e-9)
+    assert kv["imex_energy_closeout_delta_linf_max"] == kv["imex_energy_correction_linf_max"]
+    for k in ("imex_momentum_correction_linf_last", "imex_eq78_divergence_linf_last",
+              "imex_eq78_divergence_linf_max"):
+        assert 0.0 < float(kv[k]) < 1e3, k
+    assert kv["imex_source_space_order"] == "second_order_space_s2"
+    # IMEX rows name the reference's host solve path in the inputs
+    rc, _, err = run(tmp_path, "euler.method=imex", "euler.spatial_order=2", "euler.imex_acoustic_startup=0",
+                     "euler.imex_pressure_solver_path=eigen")
+    assert rc != 0 and "imex_pressure_solver_path" in err
 
 
 def test_imex_exact_dirichlet_row_equals_outflow(tmp_path):

@@ -247,3 +247,46 @@ def test_non_periodic_pressure_solves_the_reference_rows(ecor, ecamd, gpu, preci
     ref, pref, r = ecor.imex_step(g, U, dt, picard=1, tol=1e-10, max_iter=3000)
     assert r.status in (0, 1)
     assert np.array_equal(Ud.cpu().numpy()[0], ref[0])   # the star density, no solver error
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_closeout_diagnostics(ecor, ecamd, gpu, precision):
+    """The closeout's evaluation (imex/imex_source_map_step.hpp:75-258) on
+    the device: each value is the reduction of the device's own closed state
+    and solved pressure (bit for bit; the eq78 divergence to rounding of
+    1/dx), and the star density's minimum is the oracle's.  The other values
+    follow the solve: the closed state's EOS pressure moves by 3e-4 relative
+    between solves to 1e-8 and 1e-10 on this grid, so against the oracle only
+    the solved pressure's extrema are compared (relative 1e-9 at tol 1e-10).
+    x sides outflow (mirrored neighbours), y periodic."""
+    nx, ny = 40, 24
+    U = smooth_low_mach(nx, ny)
+    g = ecor.make_grid(nx, ny, 1 / nx, 1 / ny, bc=(1, 1, 0, 0))
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    s = ecamd.Solver(ecamd.Grid(nx, ny, 1 / nx, 1 / ny, 1.4, "hllc", "minmod", 2,
+                                ("outflow", "outflow", "periodic", "periodic")), gpu)
+    Ud = torch.from_numpy(U.copy()).to(gpu)
+    p = torch.empty(nx * ny, dtype=torch.float64, device=gpu)
+    d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-10, max_iter=4000, precision=precision,
+                                        picard_iterations=2, restart=64), p)
+    assert ecamd.IMEX_STATUS[d.status] in ("converged_absolute", "converged_relative"), d.status
+    assert d.nonfinite_count == 0
+    V, P = Ud.cpu().numpy(), p.cpu().numpy()
+    S = ecor.imex_star(g, U, dt)
+    assert d.solved_pressure_min == P.min() and d.solved_pressure_max == P.max()
+    assert d.trial_density_min == V[0].min() == S[0].min()
+    kin = 0.5 * (V[1] * V[1] + V[2] * V[2]) / V[0]
+    assert d.trial_internal_energy_min == (V[3] - kin).min()
+    assert d.trial_pressure_min == ((1.4 - 1.0) * (V[3] - kin)).min()
+    assert d.momentum_correction_linf == np.sqrt((V[1] - S[1]) ** 2 + (V[2] - S[2]) ** 2).max()
+    assert d.energy_correction_linf == np.abs(V[3] - S[3]).max()
+    u, v = V[1] / V[0], V[2] / V[0]
+    i = np.arange(nx)
+    ie, iw = np.minimum(i + 1, nx - 1), np.maximum(i - 1, 0)
+    div = 0.5 * (u[:, ie] - u[:, iw]) * nx + 0.5 * (np.roll(v, -1, 0) - np.roll(v, 1, 0)) * ny
+    assert d.eq78_divergence_linf == pytest.approx(np.abs(div).max(), rel=1e-14)
+    _, _, r = ecor.imex_step(g, U, dt, picard=2, tol=1e-10, max_iter=4000)
+    assert r.status in (0, 1)
+    assert d.trial_density_min == r.trial_density_min
+    assert d.solved_pressure_min == pytest.approx(r.solved_pressure_min, rel=1e-9)
+    assert d.solved_pressure_max == pytest.approx(r.solved_pressure_max, rel=1e-9)

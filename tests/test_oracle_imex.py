@@ -10,6 +10,11 @@
   and agreement with a dense solve on an SPD system.
 * A uniform flow is an exact fixed point of the step; a periodic step
   conserves mass and energy.
+* The closeout's evaluation (imex/imex_source_map_step.hpp:75-258): zero
+  corrections and divergence for a uniform flow; otherwise the extrema of the
+  returned state and solved pressure, the corrections against the star state,
+  and the eq78 divergence with the neighbours mirrored at a physical side
+  (not wrapped).
 """
 import numpy as np
 import pytest
@@ -143,3 +148,61 @@ def test_imex_dt_material_rate(ecor):
     dt0, _, _ = ecor.imex_dt(g, np.broadcast_to(ecor.to_conserved([1, 0, 0, 1])[:, None, None],
                                                 (4, n, n)).copy(), 0.8, 0.25, 1.0)
     assert dt0 == 0.75                                # no material rate: stop - time
+
+
+def test_closeout_diagnostics_of_uniform_flow(ecor):
+    n = 16
+    g = gresho_grid(ecor, n)
+    q = np.array([1.3, 0.2, -0.1, 5.0])
+    U = np.broadcast_to(ecor.to_conserved(q)[:, None, None], (4, n, n)).copy()
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    _, _, r = ecor.imex_step(g, U, dt, picard=2, tol=1e-10, max_iter=100)
+    assert r.trial_density_min == 1.3
+    assert r.solved_pressure_min == pytest.approx(5.0, rel=1e-13)
+    assert r.solved_pressure_max == pytest.approx(5.0, rel=1e-13)
+    assert r.trial_pressure_min == pytest.approx(5.0, rel=1e-12)
+    assert r.trial_internal_energy_min == pytest.approx(5.0 / 0.4, rel=1e-12)
+    assert r.momentum_correction_linf <= 1e-12 and r.energy_correction_linf <= 1e-12
+    assert r.eq78_divergence_linf <= 1e-10
+
+
+def smooth_state(nx, ny):
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    rho = 1.0 + 0.1 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    u = 0.5 * np.sin(np.pi * Y) * np.cos(np.pi * X) + 0.2
+    v = 0.3 * np.cos(np.pi * Y) * np.sin(2 * np.pi * X)
+    p = 1.0 / (1.4 * 1e-4) + np.cos(2 * np.pi * X)
+    return np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+
+
+def test_closeout_diagnostics_are_reductions_of_the_closed_state(ecor):
+    nx, ny = 24, 20
+    g = ecor.make_grid(nx, ny, 1.0 / nx, 1.0 / ny, bc=(1, 1, 0, 0))   # outflow x, periodic y
+    U = smooth_state(nx, ny)
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    U2, p, r = ecor.imex_step(g, U, dt, picard=2, tol=1e-10, max_iter=2000)
+    assert r.status in (0, 1) and r.nonfinite_count == 0
+    S = ecor.imex_star(g, U, dt)
+    assert r.solved_pressure_min == p.min() and r.solved_pressure_max == p.max()
+    assert r.trial_density_min == U2[0].min()
+    kin = 0.5 * (U2[1] * U2[1] + U2[2] * U2[2]) / U2[0]
+    assert r.trial_internal_energy_min == (U2[3] - kin).min()
+    assert r.trial_pressure_min == ((1.4 - 1.0) * (U2[3] - kin)).min()
+    dm = np.sqrt((U2[1] - S[1]) ** 2 + (U2[2] - S[2]) ** 2)
+    assert r.momentum_correction_linf == dm.max() > 0
+    assert r.energy_correction_linf == np.abs(U2[3] - S[3]).max() > 0
+    u, v = U2[1] / U2[0], U2[2] / U2[0]
+
+    def div(xwrap):
+        i = np.arange(nx)
+        ie, iw = (i + 1) % nx, (i - 1) % nx
+        if not xwrap:             # mirrored: cell -1 reads 0, cell nx reads nx-1
+            ie, iw = np.minimum(i + 1, nx - 1), np.maximum(i - 1, 0)
+        d = 0.5 * (u[:, ie] - u[:, iw]) * nx + 0.5 * (np.roll(v, -1, 0) - np.roll(v, 1, 0)) * ny
+        return np.abs(d).max()
+
+    assert r.eq78_divergence_linf == pytest.approx(div(False), rel=1e-14)
+    assert div(False) != pytest.approx(div(True), rel=1e-3)     # the side rule matters here
+
