@@ -34,6 +34,7 @@
 #include <string>
 #include <vector>
 
+#include "ec_exact_api.hpp"
 #include "ec_tile.hpp"
 
 namespace ec {
@@ -143,7 +144,8 @@ __device__ bool member_round(typename TileStep<TX, TY, NT, RIEMANN, ORDER>::Smem
     Partial* out = A.part + (long long)p * A.ntiles + md.tile0;
     for (int t = ta; t < tb; ++t) {
       const int i0 = (t % md.tiles_x) * TX, j0 = (t / md.tiles_x) * TY;
-      const Red rr = T::run(sm, uin, uout, rs, cs, md.g, i0, j0, dt);
+      // exact-Dirichlet ghosts take the step's start time, held in LDS
+      const Red rr = T::run(sm, uin, uout, rs, cs, md.g, i0, j0, dt, true, &rsh.c.time);
       store_partial(out, t, rr);
       __syncthreads();
     }
@@ -369,6 +371,7 @@ struct ec_batch {
   int last_mode = 0, last_shape = 0, last_rounds = 0;
   double last_ms = 0.0;
   unsigned spin_limit = 1u << 20;
+  std::vector<ec::ExactDevice> exact;  // [nm]: exact-Dirichlet sides of a member
   std::string last_error;
 };
 
@@ -428,6 +431,8 @@ ec_status configure_tables(ec_batch* b, int shape)
       for (int c = 0; c < 4; ++c) gd.fixed_state[s][c] = d.fixed_state[s][c];
     }
     md[m].g = make_grid_params(gd);
+    md[m].g.exact = m < (int)b->exact.size() ? b->exact[m].desc : nullptr;
+    md[m].g.time_ptr = nullptr;
     md[m].off = b->off[m];
     md[m].ntiles = tiles_of(d, shape, &md[m].tiles_x);
     md[m].tile0 = (int)tm.size();
@@ -768,6 +773,7 @@ extern "C" ec_status ec_batch_destroy(ec_batch* b)
   if (b->tile_member) (void)hipFree(b->tile_member);
   if (b->tiles) (void)hipFree(b->tiles);
   if (b->ranges) (void)hipFree(b->ranges);
+  for (auto& e : b->exact) ec::exact_free(e);
   if (b->ev_user) (void)hipEventDestroy(b->ev_user);
   if (b->ev_a) (void)hipEventDestroy(b->ev_a);
   if (b->ev_b) (void)hipEventDestroy(b->ev_b);
@@ -804,6 +810,35 @@ static ec_status member_copy(ec_batch* b, int m, double* dst, const double* src,
     }
   }
   BT_TRY(b, hipStreamSynchronize(b->stream));
+  return EC_OK;
+}
+
+extern "C" ec_status ec_batch_set_exact_boundary(ec_batch* b, int member, const ec_problem_desc* pd,
+                                                 const int* periodic)
+{
+  if (!b || !pd || !periodic || member < 0 || member >= b->nm) return EC_ERR_INVALID_ARGUMENT;
+  ec_member_desc& d = b->desc[member];
+  if (pd->nx != d.nx || pd->ny != d.ny || pd->gamma != b->gamma) {
+    b->last_error = "ec_batch_set_exact_boundary: problem grid or gamma differs from the member's";
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  ec_status st = enter(b);
+  if (st != EC_OK) return st;
+  // Work in flight may read the member's current descriptor.
+  BT_TRY(b, hipStreamSynchronize(b->stream));
+  int bc[4], plain[4];
+  ec::ExactDevice e;
+  std::string err;
+  st = ec::exact_build(pd, periodic[0], periodic[1], bc, plain, &e, &err);
+  if (st != EC_OK) {
+    if (!err.empty()) b->last_error = "ec_batch_set_exact_boundary: " + err;
+    return st;
+  }
+  if (b->exact.empty()) b->exact.resize(b->nm);
+  ec::exact_free(b->exact[member]);
+  b->exact[member] = e;
+  for (int s = 0; s < 4; ++s) d.bc[s] = bc[s];
+  b->cfg_shape = -1;  // the device tables hold the old sides
   return EC_OK;
 }
 

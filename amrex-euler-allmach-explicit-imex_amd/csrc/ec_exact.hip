@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "ec_context.hpp"
+#include "ec_exact_api.hpp"
 #include "ec_ghost.hpp"
 #include "ec_problems.hpp"
 
@@ -56,6 +57,13 @@ void toro_constants(ExactBc& e, const prob::P4& L, const prob::P4& R, double gam
   }
 }
 
+void exact_free(ExactDevice& d)
+{
+  if (d.desc) (void)hipFree(d.desc);
+  if (d.ring) (void)hipFree(d.ring);
+  d = ExactDevice{};
+}
+
 void exact_release(ec_ctx* ctx)
 {
   if (ctx->exact_dev) (void)hipFree(ctx->exact_dev);
@@ -72,39 +80,29 @@ bool has_exact_side(const GridParams& g)
   return false;
 }
 
-}  // namespace ec
-
-using namespace ec;
-
-extern "C" ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* pd, const int* periodic)
+ec_status exact_build(const ec_problem_desc* pd, int px, int py, int bc[4], int plain_bc[4], ExactDevice* out,
+                      std::string* err)
 {
-  if (!ctx || !pd || !periodic) return EC_ERR_INVALID_ARGUMENT;
-  if (pd->nx != ctx->grid.nx || pd->ny != ctx->grid.ny || pd->gamma != ctx->grid.gamma) {
-    ctx->last_error = "ec_set_exact_boundary: problem grid or gamma differs from the context's";
-    return EC_ERR_INVALID_ARGUMENT;
-  }
+  *out = ExactDevice{};
   const int kinds[] = {EC_PROBLEM_GRESHO, EC_PROBLEM_TORO, EC_PROBLEM_RIEMANN_QUADRANT,
                        EC_PROBLEM_ADVECTION_BLOB};
   bool known = false;
   for (int k : kinds) known = known || pd->kind == k;
   if (!known) {
-    ctx->last_error = "ec_set_exact_boundary: the problem has no exact-Dirichlet ghost state";
+    *err = "the problem has no exact-Dirichlet ghost state";
     return EC_ERR_INVALID_ARGUMENT;
   }
   prob::P4 tl{}, tr{};
   if (pd->kind == EC_PROBLEM_TORO && !prob::toro_state(pd->toro_test, tl, tr)) {
-    ctx->last_error = "ec_set_exact_boundary: unknown Toro test id";
+    *err = "unknown Toro test id";
     return EC_ERR_INVALID_ARGUMENT;
   }
-  EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  // Kernels in flight may read the current descriptor.
-  EC_HIP_TRY(ctx, ctx->stream ? hipStreamSynchronize(ctx->stream) : hipDeviceSynchronize());
-  exact_release(ctx);
-  const int px = periodic[0] ? 1 : 0, py = periodic[1] ? 1 : 0;
+  px = px ? 1 : 0;
+  py = py ? 1 : 0;
   // The exact fill applies when an axis is not periodic
   // (use_*_exact_dirichlet_boundary, problems/boundary_conditions.hpp:165-191);
   // otherwise every side is periodic.
-  int bc[4] = {BC_PERIODIC, BC_PERIODIC, BC_PERIODIC, BC_PERIODIC};
+  for (int s = 0; s < 4; ++s) bc[s] = BC_PERIODIC;
   if (!px || !py) {
     // Toro's fill writes the x ghosts whatever the x periodicity
     // (problems/problem_initialization.hpp:329-330).
@@ -112,12 +110,7 @@ extern "C" ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* p
     bc[0] = bc[1] = x_exact;
     bc[2] = bc[3] = py ? BC_PERIODIC : BC_EXACT;
   }
-  for (int s = 0; s < 4; ++s) {
-    ctx->grid.bc[s] = bc[s];
-    ctx->desc.bc[s] = bc[s];
-    ctx->exact_plain_bc[s] = (s < 2 ? px : py) ? BC_PERIODIC : BC_OUTFLOW;
-  }
-  ctx->grid.exact = nullptr;
+  for (int s = 0; s < 4; ++s) plain_bc[s] = (s < 2 ? px : py) ? BC_PERIODIC : BC_OUTFLOW;
   if (px && py) return EC_OK;
 
   ExactBc e;
@@ -168,21 +161,62 @@ extern "C" ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* p
         prob::to_cons(q, pd->gamma, &ring[4 * ring_index(i, j, pd->nx, pd->ny)]);
       }
     }
-    if (hipMalloc(&ctx->exact_ring, sizeof(double) * ring.size()) != hipSuccess) {
-      exact_release(ctx);
+    if (hipMalloc(&out->ring, sizeof(double) * ring.size()) != hipSuccess) {
+      exact_free(*out);
       return EC_ERR_OUT_OF_MEMORY;
     }
-    EC_HIP_TRY(ctx, hipMemcpy(ctx->exact_ring, ring.data(), sizeof(double) * ring.size(),
-                              hipMemcpyHostToDevice));
-    e.ring = ctx->exact_ring;
+    if (hipMemcpy(out->ring, ring.data(), sizeof(double) * ring.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      exact_free(*out);
+      *err = "copying the exact ghost ring";
+      return EC_ERR_HIP;
+    }
+    e.ring = out->ring;
   }
-  if (hipMalloc(&ctx->exact_dev, sizeof(ExactBc)) != hipSuccess) {
-    exact_release(ctx);
+  if (hipMalloc(&out->desc, sizeof(ExactBc)) != hipSuccess) {
+    exact_free(*out);
     return EC_ERR_OUT_OF_MEMORY;
   }
-  EC_HIP_TRY(ctx, hipMemcpy(ctx->exact_dev, &e, sizeof(ExactBc), hipMemcpyHostToDevice));
-  ctx->exact_kind = e.kind;
-  ctx->grid.exact = ctx->exact_dev;
+  if (hipMemcpy(out->desc, &e, sizeof(ExactBc), hipMemcpyHostToDevice) != hipSuccess) {
+    exact_free(*out);
+    *err = "copying the exact descriptor";
+    return EC_ERR_HIP;
+  }
+  out->kind = e.kind;
+  return EC_OK;
+}
+
+}  // namespace ec
+
+using namespace ec;
+
+extern "C" ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* pd, const int* periodic)
+{
+  if (!ctx || !pd || !periodic) return EC_ERR_INVALID_ARGUMENT;
+  if (pd->nx != ctx->grid.nx || pd->ny != ctx->grid.ny || pd->gamma != ctx->grid.gamma) {
+    ctx->last_error = "ec_set_exact_boundary: problem grid or gamma differs from the context's";
+    return EC_ERR_INVALID_ARGUMENT;
+  }
+  EC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // Kernels in flight may read the current descriptor.
+  EC_HIP_TRY(ctx, ctx->stream ? hipStreamSynchronize(ctx->stream) : hipDeviceSynchronize());
+  int bc[4], plain[4];
+  ExactDevice d;
+  std::string err;
+  const ec_status st = exact_build(pd, periodic[0], periodic[1], bc, plain, &d, &err);
+  if (st != EC_OK) {
+    if (!err.empty()) ctx->last_error = "ec_set_exact_boundary: " + err;
+    return st;
+  }
+  exact_release(ctx);
+  for (int s = 0; s < 4; ++s) {
+    ctx->grid.bc[s] = bc[s];
+    ctx->desc.bc[s] = bc[s];
+    ctx->exact_plain_bc[s] = plain[s];
+  }
+  ctx->exact_dev = d.desc;
+  ctx->exact_ring = d.ring;
+  ctx->exact_kind = d.kind;
+  ctx->grid.exact = d.desc;
   return EC_OK;
 }
 

@@ -134,10 +134,16 @@ EC_HD Cons fixed_state(const GridParams& g, int side)
   return Cons{g.fixed[side][0], g.fixed[side][1], g.fixed[side][2], g.fixed[side][3]};
 }
 
+// time_src: where exact ghost states read the step's time, if not from g
+// (the ensembles keep each member's step time in LDS).
 EC_HD Cons load_cell(const double* __restrict__ u, long long rs, long long cs,
-                                          const GridParams& g, int i, int j)
+                                          const GridParams& g, int i, int j,
+                                          const double* time_src = nullptr)
 {
-  if (g.exact && exact_cell(g, i, j)) return exact_state(*g.exact, g.time_ptr ? *g.time_ptr : g.time, i, j);
+  if (g.exact && exact_cell(g, i, j)) {
+    const double t = time_src ? *time_src : (g.time_ptr ? *g.time_ptr : g.time);
+    return exact_state(*g.exact, t, i, j);
+  }
   if (i < 0 && g.bc[0] == BC_FIXED) return fixed_state(g, 0);
   if (i >= g.nx && g.bc[1] == BC_FIXED) return fixed_state(g, 1);
   if (j < 0 && g.bc[2] == BC_FIXED) return fixed_state(g, 2);

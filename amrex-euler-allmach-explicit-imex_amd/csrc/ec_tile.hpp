@@ -236,7 +236,7 @@ struct TileStep {
   // count): a replay with known dts needs only the state.
   __device__ static Red run(Smem& sm, const double* __restrict__ uin, double* __restrict__ uout,
                             long long rs, long long cs, const GridParams& g, int i0, int j0,
-                            double dt, bool diag = true)
+                            double dt, bool diag = true, const double* time_src = nullptr)
   {
     double* lds = sm.lds;
     unsigned char* degx = sm.degx;
@@ -276,7 +276,7 @@ struct TileStep {
     } else {
       for (int t = tid; t < NR; t += NT) {
         const int li = t % RX, lj = t / RX;
-        const Prim q = to_prim(load_cell(uin, rs, cs, g, i0 + li - 2, j0 + lj - 2), gamma);
+        const Prim q = to_prim(load_cell(uin, rs, cs, g, i0 + li - 2, j0 + lj - 2, time_src), gamma);
         lds[OFF_Q + t] = q.r;
         lds[OFF_Q + NR + t] = q.u;
         lds[OFF_Q + 2 * NR + t] = q.v;

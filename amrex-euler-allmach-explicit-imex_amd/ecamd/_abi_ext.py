@@ -154,6 +154,8 @@ def declare(L) -> None:
     L.ec_batch_last_error.restype = C.c_char_p
     L.ec_batch_set_state.argtypes = [vp, C.c_int, vp, C.POINTER(Layout)]
     L.ec_batch_set_state.restype = st
+    L.ec_batch_set_exact_boundary.argtypes = [vp, C.c_int, C.POINTER(ProblemDesc), C.POINTER(C.c_int)]
+    L.ec_batch_set_exact_boundary.restype = st
     L.ec_batch_get_state.argtypes = [vp, C.c_int, vp, C.POINTER(Layout)]
     L.ec_batch_get_state.restype = st
     L.ec_batch_tile_count.argtypes = []

@@ -102,6 +102,14 @@ class Ensemble:
         except Exception:
             pass
 
+    def set_exact_boundary(self, m: int, problem, periodic) -> None:
+        """Exact-Dirichlet sides for member m (euler.field_boundary=exact_dirichlet):
+        problem is an ecamd.problems.problem_desc of the member's grid,
+        periodic the geometry.is_periodic pair."""
+        per = (C.c_int * 2)(*[int(x) for x in periodic])
+        self._check(lib().ec_batch_set_exact_boundary(self._b, m, C.byref(problem), per),
+                    "ec_batch_set_exact_boundary")
+
     def set_state(self, m: int, U) -> None:
         """U: (4, ny, nx) float64, numpy (host) or torch (host or this device)."""
         mem = self.members[m]

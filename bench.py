@@ -131,6 +131,26 @@ def copy_calibration(dev, n: int, steps: int) -> int:
     return 0
 
 
+def stream_copy_gbs(dev, like, copies: int = 20) -> float:
+    """Achievable HBM bandwidth on this device: a device-to-device copy of a
+    buffer the size of one state (read + write bytes over the copy time,
+    median of `copies`)."""
+    import torch
+    b = torch.empty_like(like)
+    b.copy_(like)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(copies):
+        a0, a1 = _events()
+        a0.record()
+        b.copy_(like)
+        a1.record()
+        torch.cuda.synchronize()
+        times.append(a0.elapsed_time(a1) / 1e3)
+    s = sorted(times)[len(times) // 2]
+    return 2 * like.numel() * like.element_size() / s / 1e9
+
+
 def _events():
     import torch
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -479,6 +499,7 @@ def main() -> int:
     solver.disarm_kernel_timing()
     kernel_s = k_ms / 1e3 / max(k_n, 1)
     achieved = BYTES_PER_CELL_UPDATE * cells / kernel_s / 1e9
+    copy_gbs = stream_copy_gbs(dev, A)
     traffic = None
     if TRAFFIC_FILE.exists() and n == 2048 and args.riemann == "xie_am_hllc_p" and args.order == 2:
         try:
@@ -507,7 +528,10 @@ def main() -> int:
                      "kernel_us_avg": kernel_s * 1e6, "kernel_launches_timed": k_n,
                      "bytes_per_cell_update": BYTES_PER_CELL_UPDATE,
                      "bytes_per_launch": BYTES_PER_CELL_UPDATE * cells,
-                     "fp64_valu": valu_bound(args.riemann, args.order, n, kernel_s)},
+                     "fp64_valu": valu_bound(args.riemann, args.order, n, kernel_s),
+                     "stream_copy": {"achieved": copy_gbs, "unit": "GB/s",
+                                     "frac_of_peak": copy_gbs / HBM_PEAK_GBS,
+                                     "kernel_frac_of_copy": achieved / copy_gbs}},
         "wall_s": wall,
         "sim_time_advanced": run.time,
     }

@@ -428,6 +428,13 @@ ec_status ec_set_exact_boundary(ec_ctx* ctx, const ec_problem_desc* problem, con
 /* The time argument of euler_fv_one_step (numerics/explicit_update.hpp:58-65)
  * for the exact ghost states of the next host-dt steps (default 0). */
 ec_status ec_set_boundary_time(ec_ctx* ctx, double time);
+/* The same exact-Dirichlet sides for one member of a batched ensemble
+ * (problem: the member's nx, ny and the batch's gamma).  The ghost states
+ * take each step's start time, as the explicit branch of the driver loop
+ * passes it (app/driver_main.hpp:289, numerics/explicit_update.hpp:58-65).
+ * Replaces the member's bc; synchronises the batch's stream. */
+ec_status ec_batch_set_exact_boundary(ec_batch* b, int member, const ec_problem_desc* problem,
+                                      const int* periodic);
 
 /* ------------------------------------------ direct BDLTV20 T1/S2 driver */
 /* The cases of euler.bdltv20_paper_t1_s2 (core/config_reader.hpp:136-142),
