@@ -154,6 +154,26 @@ def test_exact_boundary_both_periodic_is_periodic(ecor, ecamd, gpu):
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("nx,ny", [(1, 1), (1, 4), (2, 3), (3, 2), (5, 1)])
+@pytest.mark.parametrize("name,periodic,t", [("gresho", (0, 0), 0.0), ("quadrant", (1, 0), 0.0),
+                                             ("toro", (0, 1), 0.3), ("blob", (0, 0), 0.1)])
+def test_exact_step_on_tiny_grids(ecor, ecamd, gpu, nx, ny, name, periodic, t):
+    """Grids smaller than the ghost ring: every ghost (and every unused lane
+    of the partial tile) reads a ring cell or the exact solution, bit for bit
+    with the oracle."""
+    print(f"exact tiny {name} {periodic} {nx}x{ny}", flush=True)
+    s, g, U = setup(ecor, ecamd, name, periodic, 3, nx=nx, ny=ny, hi=(1.0, 0.7))
+    g.time = t
+    dt = 0.1 * 0.8 / ecor.max_rate(g, U)
+    ref, deg = ecor.fv_step(g, U, dt)
+    s.set_boundary_time(t)
+    Ud = torch.from_numpy(U).to(gpu)
+    out = torch.empty_like(Ud)
+    d = s.step(Ud, out, dt)
+    check(out.cpu().numpy(), ref, True)
+    assert d.degenerate_count == deg
+
+
 def test_exact_boundary_rejections(ecor, ecamd, gpu):
     from ecamd import problems
     from ecamd._lib import lib
