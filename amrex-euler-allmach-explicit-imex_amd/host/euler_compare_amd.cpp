@@ -291,6 +291,23 @@ Cfg read_config(const Params& pp)
   c.cylindrical = is_cylindrical_name(lower(c.problem));
   c.riemann_kind = parse_riemann(c.riemann);
   c.limiter_kind = parse_limiter(c.slope_limiter);
+  // core/config_parse_helpers.hpp:76-114
+  {
+    const std::string f = lower(c.imex_form);
+    if (f != "bdltv20_t1_s2_source_map_picard" && f != "bdltv20-t1-s2-source-map-picard") {
+      abort_msg("Unknown euler.imex_form=" + c.imex_form +
+                "; this code pack supports bdltv20_t1_s2_source_map_picard.");
+    }
+    c.imex_form = "bdltv20_t1_s2_source_map_picard";
+    const std::string d = lower(c.imex_predictor_dissipation);
+    if (d == "material" || d == "material_speed") {
+      c.imex_predictor_dissipation = "material";
+    } else if (d == "acoustic" || d == "acoustic_speed") {
+      c.imex_predictor_dissipation = "acoustic";
+    } else {
+      abort_msg("Unknown euler.imex_predictor_dissipation=" + c.imex_predictor_dissipation);
+    }
+  }
   c.method = lower(c.method);
   c.field_boundary = lower(c.field_boundary);
   if (c.method != "explicit" && c.method != "imex") abort_msg("Unknown euler.method=" + c.method);
@@ -321,6 +338,9 @@ Cfg read_config(const Params& pp)
     if (c.bdltv20_paper_pressure_solver != "gmres") abort_msg("euler.bdltv20_paper_pressure_solver must be gmres.");
     if (!(c.bdltv20_paper_epsilon > 0.0) || !std::isfinite(c.bdltv20_paper_epsilon)) {
       abort_msg("euler.bdltv20_paper_epsilon must be finite and positive.");
+    }
+    if (c.bdltv20_paper_t1_s2_max_steps < 1) {
+      abort_msg("euler.bdltv20_paper_t1_s2_max_steps must be at least one.");
     }
     // core/config_reader.hpp:228-255
     const std::string& m = c.bdltv20_paper_t1_s2;

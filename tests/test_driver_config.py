@@ -23,6 +23,12 @@ SB = ("euler.problem=shock_density_bubble_2d",)
     (SB + ("euler.gamma=1.3",), "shock_density_bubble is a same-gamma ideal-gas test with gamma=1.4."),
     (SB + ("amr.n_cell=8 40",), "shock_density_bubble requires at least 16x4 cells."),
     (SB + ("euler.riemann=rusanov",), "shock_density_bubble explicit rows support hllc or xie_am_hllc_p."),
+    (("euler.imex_form=bdltv20_o1",), "Unknown euler.imex_form=bdltv20_o1"),
+    (("euler.imex_predictor_dissipation=rusanov",), "Unknown euler.imex_predictor_dissipation=rusanov"),
+    (IMEX + ("euler.bdltv20_paper_t1_s2=toro_xy_exact_dirichlet_2d", "euler.problem=toro1",
+             "geometry.is_periodic=0 0", "euler.field_boundary=exact_dirichlet",
+             "euler.bdltv20_paper_t1_s2_max_steps=0"),
+     "euler.bdltv20_paper_t1_s2_max_steps must be at least one."),
     (("euler.problem=shock_density_bubble_cylindrical",) + IMEX,
      "cylindrical shock_density_bubble IMEX is not part of the cleaned Cartesian code pack."),
 ])
