@@ -16,7 +16,17 @@
  *                               x/y rotation symmetry
  *   xie_hllcp                   consistency; reduces to HLLC for f=1 and a
  *                               locally supersonic state (theta=1); rotation
- *                               symmetry; f-affine structure
+ *                               symmetry; f-affine structure; Roe's property
+ *                               of the average recovered from beta_p (pins
+ *                               u^, v^, H^, c^2, the (1+|M^|) factor and the
+ *                               sign); the low-Mach pressure correction
+ *                               against Toro's p*; the paper's arrangement
+ *                               (HLLC + correction + dissipation)
+ *   5-face sensor               f recovered from face fluxes = cube of the
+ *                               minimum ratio over the face stencil, O1/O2
+ *   low-Mach HLLC blend         z = 1 gives HLLC; z = 1/2 blend
+ *   (tests/test_oracle_hllcp.py; tests/test_oracle_mutants.py plants a slip
+ *   in each of these terms and checks that a pin rejects it)
  *   exact_riemann               Toro's Table 4.2 star values (tests 1-5)
  *   fv_step / run_explicit      Sod 400x4 HLLC O2 CFL 0.8 t=0.2 takes exactly
  *                               the 347 steps the reference run reports;
