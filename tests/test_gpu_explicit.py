@@ -113,9 +113,11 @@ def test_sod_full_run_device_loop(ecor, ecamd, gpu):
 
 
 @pytest.mark.parametrize("riemann", [1, 3])
-def test_gresho_device_loop_parity(ecor, ecamd, gpu, riemann):
-    """BASELINE config 2 at 64^2, M=0.01, CFL 0.4, N=100 steps."""
-    n = 64
+@pytest.mark.parametrize("n", [64, 256])
+def test_gresho_device_loop_parity(ecor, ecamd, gpu, riemann, n):
+    """Gresho vortex, M=0.01, CFL 0.4, N=100 steps of the device loop against
+    the oracle (about 13 s of serial CPU for HLLC-P at 256^2, the smallest
+    roofline sweep size)."""
     g = ecor.make_grid(n, n, 1.0 / n, 1.0 / n, riemann=riemann)
     U = ecor.init_gresho(n, n, mach=0.01)
     ref, res, dts = ecor.run_explicit(g, U, 0.4, 1.0, 100)
@@ -189,6 +191,22 @@ def test_run_status_paths(ecor, ecamd, gpu):
     r, p, nf = ecor.diagnostics(g, bad)
     assert (d.rho_min, d.pressure_min, d.nonfinite_count) == (r, p, nf)
     assert d.max_rate == ecor.max_rate(g, bad)
+
+
+def test_sweep_size_1024_bitwise(ecor, ecamd, gpu):
+    """The 1024^2 roofline sweep point (Gresho, M=0.01, HLLC-P O2, CFL 0.4):
+    five device-loop steps against the oracle (about 10 s of serial CPU), bit
+    for bit with the same dt sequence and degenerate count."""
+    n = 1024
+    g = ecor.make_grid(n, n, 1.0 / n, 1.0 / n, riemann=3)
+    U = ecor.init_gresho(n, n, mach=0.01)
+    ref, res, dts = ecor.run_explicit(g, U, 0.4, 1.0, 5)
+    s = solver_for(ecamd, g)
+    got, run = ecamd.run_explicit(s, to_dev(U, gpu), 0.4, 1.0, 5)
+    assert run.step == 5 and res.steps == 5
+    assert run.time == res.time and run.degenerate_total == res.degenerate_total
+    assert np.all(rel_l1(got.cpu().numpy(), ref) <= TOL)
+    assert np.array_equal(got.cpu().numpy(), ref)
 
 
 def test_headline_workload_full_size_bitwise(ecor, ecamd, gpu):
