@@ -15,11 +15,17 @@
 //      max CFL spectral rate, min rho, min p, non-finite and degenerate-star
 //      counts, reduced per wavefront with cross-lane shuffles and per
 //      workgroup in LDS.
+// Stage A also checks that every state the tile reads lies in the magnitude
+// window of ec_exact_ops.hpp.  When it does (always, for physical states of
+// moderate size), stages C and D use the window operators of
+// ec_flux_window.hpp, whose results equal the plain IEEE arithmetic bit for
+// bit; otherwise they use the plain operators.
 #pragma once
 
 #include <type_traits>
 
 #include "ec_context.hpp"
+#include "ec_flux_window.hpp"
 #include "ec_ghost.hpp"
 #include "ec_state.hpp"
 #include "ec_wave.hpp"
@@ -213,6 +219,7 @@ struct TileStep {
     unsigned char degx[NXF];
     unsigned char degy[NYF];
     Red red_scratch[NT / 64];
+    int wave_ok[NT / 64];
   };
 
   // Copies the tile through unchanged (a stopped run keeps its ping-pong
@@ -263,6 +270,7 @@ struct TileStep {
     // halo lies inside the domain reads it directly (the ghost rule is the
     // identity there); boundary tiles go through the rule.
     const bool interior = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= g.nx && j0 + TY + 2 <= g.ny;
+    bool ok = pos_in_window(gamma) && pos_in_window(gamma - 1.0);
     if (interior) {
       for (int t = tid; t < NR; t += NT) {
         const int li = t % RX, lj = t / RX;
@@ -272,6 +280,7 @@ struct TileStep {
         lds[OFF_Q + NR + t] = q.u;
         lds[OFF_Q + 2 * NR + t] = q.v;
         lds[OFF_Q + 3 * NR + t] = q.p;
+        ok = ok && pos_in_window(q.r) && pos_in_window(q.p) && vel_in_window(q.u) && vel_in_window(q.v);
       }
     } else {
       for (int t = tid; t < NR; t += NT) {
@@ -281,9 +290,17 @@ struct TileStep {
         lds[OFF_Q + NR + t] = q.u;
         lds[OFF_Q + 2 * NR + t] = q.v;
         lds[OFF_Q + 3 * NR + t] = q.p;
+        ok = ok && pos_in_window(q.r) && pos_in_window(q.p) && vel_in_window(q.u) && vel_in_window(q.v);
       }
     }
+    {
+      const int wave_all = __all(ok ? 1 : 0);
+      if ((tid & 63) == 0) sm.wave_ok[tid >> 6] = wave_all;
+    }
     __syncthreads();
+    bool win = true;
+  #pragma unroll
+    for (int w = 0; w < NT / 64; ++w) win = win && sm.wave_ok[w] != 0;
 
     // B. limited slopes (reconstruction_fluxes.hpp:2-12).
     if constexpr (O2) {
@@ -326,15 +343,34 @@ struct TileStep {
     };
 
     // C. face pressure ratios for the HLLC-P sensor (reconstruction_fluxes.hpp:390-419).
+    // Inside the window both pressures are finite and positive, so the floor
+    // is a max, and the ratio min/max lies in (0, 1] already.
+    auto ratio_w = [](double pl, double pr) {
+      const double a = fmax(pl, 1.0e-14), b = fmax(pr, 1.0e-14);
+      return quot(fmin(a, b), recip(fmax(a, b)));
+    };
     if constexpr (HP) {
-      for (int t = tid; t < NXR + NYR; t += NT) {
-        if (t < NXR) {
-          const int fi = t % XRX, j = t / XRX - 1;
-          lds[OFF_RX + t] = pressure_ratio(face_p(fi - 1, j, 0, 0.5), face_p(fi, j, 0, -0.5));
-        } else {
-          const int s = t - NXR;
-          const int i = s % YRX - 1, fj = s / YRX;
-          lds[OFF_RY + s] = pressure_ratio(face_p(i, fj - 1, 1, 0.5), face_p(i, fj, 1, -0.5));
+      if (win) {
+        for (int t = tid; t < NXR + NYR; t += NT) {
+          if (t < NXR) {
+            const int fi = t % XRX, j = t / XRX - 1;
+            lds[OFF_RX + t] = ratio_w(face_p(fi - 1, j, 0, 0.5), face_p(fi, j, 0, -0.5));
+          } else {
+            const int s = t - NXR;
+            const int i = s % YRX - 1, fj = s / YRX;
+            lds[OFF_RY + s] = ratio_w(face_p(i, fj - 1, 1, 0.5), face_p(i, fj, 1, -0.5));
+          }
+        }
+      } else {
+        for (int t = tid; t < NXR + NYR; t += NT) {
+          if (t < NXR) {
+            const int fi = t % XRX, j = t / XRX - 1;
+            lds[OFF_RX + t] = pressure_ratio(face_p(fi - 1, j, 0, 0.5), face_p(fi, j, 0, -0.5));
+          } else {
+            const int s = t - NXR;
+            const int i = s % YRX - 1, fj = s / YRX;
+            lds[OFF_RY + s] = pressure_ratio(face_p(i, fj - 1, 1, 0.5), face_p(i, fj, 1, -0.5));
+          }
         }
       }
       __syncthreads();
@@ -347,8 +383,13 @@ struct TileStep {
     // it fold away.
     const int lim_x = g.nx - i0, lim_y = g.ny - j0;  // cells of this tile inside the domain
     constexpr int NPX = (NXF + NT - 1) / NT, NPY = (NYF + NT - 1) / NT;
-    auto face = [&](auto axis_c, int f, Cons& out, unsigned char& dflag) {
+    // W: the window operators (every state of the tile inside the window).
+    auto face = [&](auto axis_c, auto w_c, int f, Cons& out, unsigned char& dflag) {
       constexpr int AX = decltype(axis_c)::value;
+      constexpr bool W = decltype(w_c)::value;
+      // inside the window the ratios are finite and in (0, 1]: min is fmin and
+      // clamp01 the identity
+      auto mn = [](double a, double b) { return W ? fmin(a, b) : smin(a, b); };
       int li, lj, ri, rj;
       bool needed;
       double sensor = 1.0;
@@ -358,11 +399,11 @@ struct TileStep {
         needed = fi <= lim_x && j < lim_y;
         if constexpr (HP) {
           double m = rx(fi, j);
-          m = smin(m, ry(fi - 1, j));
-          m = smin(m, ry(fi - 1, j + 1));
-          m = smin(m, ry(fi, j));
-          m = smin(m, ry(fi, j + 1));
-          const double c = clamp01(m);
+          m = mn(m, ry(fi - 1, j));
+          m = mn(m, ry(fi - 1, j + 1));
+          m = mn(m, ry(fi, j));
+          m = mn(m, ry(fi, j + 1));
+          const double c = W ? m : clamp01(m);
           sensor = c * c * c;
         }
       } else {
@@ -371,11 +412,11 @@ struct TileStep {
         needed = i < lim_x && fj <= lim_y;
         if constexpr (HP) {
           double m = ry(i, fj);
-          m = smin(m, rx(i, fj - 1));
-          m = smin(m, rx(i + 1, fj - 1));
-          m = smin(m, rx(i, fj));
-          m = smin(m, rx(i + 1, fj));
-          const double c = clamp01(m);
+          m = mn(m, rx(i, fj - 1));
+          m = mn(m, rx(i + 1, fj - 1));
+          m = mn(m, rx(i, fj));
+          m = mn(m, rx(i + 1, fj));
+          const double c = W ? m : clamp01(m);
           sensor = c * c * c;
         }
       }
@@ -384,24 +425,34 @@ struct TileStep {
       if (needed) {
         const Prim L = face_state(li, lj, AX, 0.5);
         const Prim R = face_state(ri, rj, AX, -0.5);
-        out = interface_flux(L, R, AX, gamma, RIEMANN, sensor);
-        if constexpr (RIEMANN != 0) {
-          if (diag) dflag = face_degenerate(L, R, AX, gamma, RIEMANN) ? 1 : 0;
+        if constexpr (W) {
+          bool degen;
+          out = interface_flux_w<AX, RIEMANN>(L, R, gamma, recip(gamma - 1.0), sensor, degen);
+          dflag = degen ? 1 : 0;
+        } else {
+          out = interface_flux(L, R, AX, gamma, RIEMANN, sensor);
+          if constexpr (RIEMANN != 0) {
+            if (diag) dflag = face_degenerate(L, R, AX, gamma, RIEMANN) ? 1 : 0;
+          }
         }
       }
     };
     Cons fxv[NPX], fyv[NPY];
     unsigned char dfx[NPX], dfy[NPY];
+    auto all_faces = [&](auto w_c) {
   #pragma unroll
-    for (int p = 0; p < NPX; ++p) {
-      const int f = tid + p * NT;
-      if (f < NXF) face(std::integral_constant<int, 0>{}, f, fxv[p], dfx[p]);
-    }
+      for (int p = 0; p < NPX; ++p) {
+        const int f = tid + p * NT;
+        if (f < NXF) face(std::integral_constant<int, 0>{}, w_c, f, fxv[p], dfx[p]);
+      }
   #pragma unroll
-    for (int p = 0; p < NPY; ++p) {
-      const int f = tid + p * NT;
-      if (f < NYF) face(std::integral_constant<int, 1>{}, f, fyv[p], dfy[p]);
-    }
+      for (int p = 0; p < NPY; ++p) {
+        const int f = tid + p * NT;
+        if (f < NYF) face(std::integral_constant<int, 1>{}, w_c, f, fyv[p], dfy[p]);
+      }
+    };
+    if (win) all_faces(std::true_type{});
+    else all_faces(std::false_type{});
     __syncthreads();  // stage A-C storage is dead from here on
   #pragma unroll
     for (int p = 0; p < NPX; ++p) {
