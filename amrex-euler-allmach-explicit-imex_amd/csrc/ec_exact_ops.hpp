@@ -90,7 +90,7 @@ __device__ __forceinline__ bool pos_in_window(double x)
 __device__ __forceinline__ bool vel_in_window(double x)
 {
   const unsigned e = ((unsigned)(__double_as_longlong(x) >> 52)) & 0x7ffu;
-  return x == 0.0 || e - 923u < 140u;
+  return (x == 0.0) | (e - 923u < 140u);
 }
 
 }  // namespace ec

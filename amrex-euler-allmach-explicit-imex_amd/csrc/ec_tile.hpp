@@ -60,32 +60,64 @@ __device__ __forceinline__ Red red_identity()
   return Red{-INFINITY, INFINITY, INFINITY, 0, 0};
 }
 
+// The reduced values are never NaN (a NaN rate is dropped, a non-finite
+// state's minima are -inf), so max and min are v_max_f64 / v_min_f64.
 __device__ __forceinline__ void red_combine(Red& a, const Red& b)
 {
-  a.rate = smax(a.rate, b.rate);
-  a.rmin = smin(a.rmin, b.rmin);
-  a.pmin = smin(a.pmin, b.pmin);
+  a.rate = fmax(a.rate, b.rate);
+  a.rmin = fmin(a.rmin, b.rmin);
+  a.pmin = fmin(a.pmin, b.pmin);
   a.bad += b.bad;
   a.deg += b.deg;
 }
 
 // Diagnostics and CFL rate of one cell (diagnostic_metrics_timestep.hpp:30-38
 // and :240-245).  A NaN rate is dropped, as the reference's sequential
-// max(running, value) drops it.
+// max(running, value) drops it: fmax returns its non-NaN operand, and the
+// running rate is never NaN.
+__device__ __forceinline__ double cell_rate(double ux, double uy, double c, const GridParams& g)
+{
+  // division by a power-of-two cell size is the multiplication by its exact
+  // reciprocal, bit for bit
+  const double ax = fabs(ux) + c, ay = fabs(uy) + c;
+  return (g.dx_pow2 ? ax * g.inv_dx : ax / g.dx) + (g.dy_pow2 ? ay * g.inv_dy : ay / g.dy);
+}
+
 __device__ __forceinline__ void red_cell(Red& r, const Cons& u, const GridParams& g)
 {
   const Prim q = to_prim(u, g.gamma);
   const bool bad = !isfinite(q.r) || !isfinite(q.u) || !isfinite(q.v) || !isfinite(q.p) ||
                    !isfinite(u.E);
-  r.rmin = smin(r.rmin, bad ? -INFINITY : q.r);
-  r.pmin = smin(r.pmin, bad ? -INFINITY : q.p);
+  r.rmin = fmin(r.rmin, bad ? -INFINITY : q.r);
+  r.pmin = fmin(r.pmin, bad ? -INFINITY : q.p);
   r.bad += bad ? 1 : 0;
-  const double c = sound_speed(q, g.gamma);
-  // division by a power-of-two cell size is the multiplication by its exact
-  // reciprocal, bit for bit
-  const double ax = fabs(q.u) + c, ay = fabs(q.v) + c;
-  const double rate = (g.dx_pow2 ? ax * g.inv_dx : ax / g.dx) + (g.dy_pow2 ? ay * g.inv_dy : ay / g.dy);
-  if (!isnan(rate)) r.rate = smax(r.rate, rate);
+  r.rate = fmax(r.rate, cell_rate(q.u, q.v, sound_speed(q, g.gamma), g));
+}
+
+// red_cell with the window operators where every active lane's density (for
+// 1/rho) and then pressure (for the sound speed) lie in the window: the same
+// results, bit for bit.  Both choices are wavefront-uniform.
+__device__ __forceinline__ void red_cell_w(Red& r, const Cons& u, const GridParams& g)
+{
+  if (!__all(pos_in_window(u.r) ? 1 : 0)) {
+    red_cell(r, u, g);
+    return;
+  }
+  const Recip rr = recip(u.r);
+  const double inv_rho = quot(1.0, rr);
+  const double ux = u.mx * inv_rho;
+  const double uy = u.my * inv_rho;
+  const double kinetic = 0.5 * u.r * (ux * ux + uy * uy);
+  const double p = (g.gamma - 1.0) * (u.E - kinetic);
+  // rho is finite inside the window
+  const bool bad = !isfinite(ux) || !isfinite(uy) || !isfinite(p) || !isfinite(u.E);
+  r.rmin = fmin(r.rmin, bad ? -INFINITY : u.r);
+  r.pmin = fmin(r.pmin, bad ? -INFINITY : p);
+  r.bad += bad ? 1 : 0;
+  double c;
+  if (__all(pos_in_window(p) ? 1 : 0)) c = sqrt_core(quot(g.gamma * p, rr));
+  else c = sqrt(g.gamma * p / u.r);
+  r.rate = fmax(r.rate, cell_rate(ux, uy, c, g));
 }
 
 // One step of the wavefront tree: combine with the value DPP control CTRL
@@ -117,11 +149,36 @@ __device__ __forceinline__ Red wave_reduce(Red r)
   return r;
 }
 
-// Workgroup reduction; result valid in thread 0.
-template <int NT>
+// wave_reduce for per-lane counts small enough to share one 32-bit word
+// (bad < 2^15 and deg < 2^16 summed over the wavefront, as in a tile's
+// epilogue): one integer move per step instead of two.
+__device__ __forceinline__ Red wave_reduce_small(Red r)
+{
+  int k = (r.bad << 16) | r.deg;
+  auto step = [&](auto ctrl_c, auto mask_c) {
+    constexpr int CTRL = decltype(ctrl_c)::value, MASK = decltype(mask_c)::value;
+    r.rate = fmax(r.rate, dpp_f64_id<CTRL, MASK>(r.rate, -INFINITY));
+    r.rmin = fmin(r.rmin, dpp_f64_id<CTRL, MASK>(r.rmin, INFINITY));
+    r.pmin = fmin(r.pmin, dpp_f64_id<CTRL, MASK>(r.pmin, INFINITY));
+    k += dpp_i32<CTRL, MASK>(k);
+  };
+  step(std::integral_constant<int, 0xB1>{}, std::integral_constant<int, 0xF>{});
+  step(std::integral_constant<int, 0x4E>{}, std::integral_constant<int, 0xF>{});
+  step(std::integral_constant<int, 0x141>{}, std::integral_constant<int, 0xF>{});
+  step(std::integral_constant<int, 0x140>{}, std::integral_constant<int, 0xF>{});
+  step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{});
+  step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{});
+  r.bad = k >> 16;
+  r.deg = k & 0xffff;
+  return r;
+}
+
+// Workgroup reduction; result valid in thread 0.  SMALL: the per-lane counts
+// of a tile epilogue (see wave_reduce_small).
+template <int NT, bool SMALL = false>
 __device__ __forceinline__ Red block_reduce(Red r, Red* scratch)
 {
-  r = wave_reduce(r);
+  r = SMALL ? wave_reduce_small(r) : wave_reduce(r);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 63) scratch[wave] = r;
   __syncthreads();
@@ -272,15 +329,29 @@ struct TileStep {
     const bool interior = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= g.nx && j0 + TY + 2 <= g.ny;
     bool ok = pos_in_window(gamma) && pos_in_window(gamma - 1.0);
     if (interior) {
-      for (int t = tid; t < NR; t += NT) {
-        const int li = t % RX, lj = t / RX;
-        const long long k = (long long)(j0 + lj - 2) * rs + (i0 + li - 2);
-        const Prim q = to_prim(Cons{uin[k], uin[cs + k], uin[2 * cs + k], uin[3 * cs + k]}, gamma);
-        lds[OFF_Q + t] = q.r;
-        lds[OFF_Q + NR + t] = q.u;
-        lds[OFF_Q + 2 * NR + t] = q.v;
-        lds[OFF_Q + 3 * NR + t] = q.p;
-        ok = ok && pos_in_window(q.r) && pos_in_window(q.p) && vel_in_window(q.u) && vel_in_window(q.v);
+      // every load of the thread first, then the conversions
+      constexpr int NA = (NR + NT - 1) / NT;
+      Cons raw[NA];
+      const double* tb = uin + (long long)(j0 - 2) * rs + (i0 - 2);
+  #pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const int t = tid + a * NT;
+        if (t < NR) {
+          const long long k = (long long)(t / RX) * rs + t % RX;
+          raw[a] = Cons{tb[k], tb[cs + k], tb[2 * cs + k], tb[3 * cs + k]};
+        }
+      }
+  #pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const int t = tid + a * NT;
+        if (t < NR) {
+          const Prim q = to_prim(raw[a], gamma);
+          lds[OFF_Q + t] = q.r;
+          lds[OFF_Q + NR + t] = q.u;
+          lds[OFF_Q + 2 * NR + t] = q.v;
+          lds[OFF_Q + 3 * NR + t] = q.p;
+          ok = ok & pos_in_window(q.r) & pos_in_window(q.p) & vel_in_window(q.u) & vel_in_window(q.v);
+        }
       }
     } else {
       for (int t = tid; t < NR; t += NT) {
@@ -290,7 +361,7 @@ struct TileStep {
         lds[OFF_Q + NR + t] = q.u;
         lds[OFF_Q + 2 * NR + t] = q.v;
         lds[OFF_Q + 3 * NR + t] = q.p;
-        ok = ok && pos_in_window(q.r) && pos_in_window(q.p) && vel_in_window(q.u) && vel_in_window(q.v);
+        ok = ok & pos_in_window(q.r) & pos_in_window(q.p) & vel_in_window(q.u) & vel_in_window(q.v);
       }
     }
     {
@@ -498,12 +569,12 @@ struct TileStep {
         }
         if (diag) {
           mine.deg += degx[xl] + degx[xl + 1] + degy[yb] + degy[yb + TX];
-          red_cell(mine, Cons{nu[0], nu[1], nu[2], nu[3]}, g);
+          red_cell_w(mine, Cons{nu[0], nu[1], nu[2], nu[3]}, g);
         }
       }
     }
     if (!diag) return mine;
-    return block_reduce<NT>(mine, sm.red_scratch);
+    return block_reduce<NT, true>(mine, sm.red_scratch);
   }
 };
 
