@@ -4,18 +4,13 @@ a temporary directory, on the CPU) and requires that at least one check of
 tests/hllcp_pins.py rejects it, while the unmodified copy passes them all.
 """
 import ctypes as C
-import shutil
-import subprocess
 from concurrent.futures import ThreadPoolExecutor
-from pathlib import Path
 
 import numpy as np
 import pytest
 
 import hllcp_pins as P
-
-ORACLE = Path(__file__).resolve().parent.parent / "oracle"
-SOURCES = ("ec_oracle.c", "ec_oracle_imex.c", "ec_oracle_direct.c")
+from oracle_copy import build_copy, load
 
 # name -> list of (text in ec_oracle.c, replacement); each text occurs once.
 MUTANTS = {
@@ -40,35 +35,17 @@ MUTANTS = {
 }
 
 
-def _build(root: Path, name: str, edits) -> Path:
-    d = root / name
-    d.mkdir()
-    for s in SOURCES + ("ec_oracle.h",):
-        shutil.copy(ORACLE / s, d / s)
-    src = (d / "ec_oracle.c").read_text()
-    for old, new in edits:
-        assert src.count(old) == 1, (name, old)
-        src = src.replace(old, new)
-    (d / "ec_oracle.c").write_text(src)
-    so = d / "libmut.so"
-    subprocess.run(["gcc", "-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-shared", "-o", str(so),
-                    *[str(d / s) for s in SOURCES], "-lm"], check=True)
-    return so
-
-
 @pytest.fixture(scope="module")
 def libs(tmp_path_factory, ecor):
     root = tmp_path_factory.mktemp("mutants")
     with ThreadPoolExecutor(max_workers=4) as ex:
-        sos = dict(zip(MUTANTS, ex.map(lambda kv: _build(root, *kv), MUTANTS.items())))
+        sos = dict(zip(MUTANTS, ex.map(lambda kv: build_copy(root, *kv), MUTANTS.items())))
     return sos
 
 
-def _bind(so: Path, ecor):
-    L = C.CDLL(str(so))
+def _bind(so, ecor):
+    L = load(so, ecor)
     dp = C.POINTER(C.c_double)
-    L.ecor_interface_flux.argtypes = [dp, dp, C.c_int, C.c_double, C.c_int, C.c_double, dp]
-    L.ecor_face_flux.argtypes = [C.POINTER(ecor.Grid), dp, C.c_int, C.c_int, C.c_int, dp]
 
     def ptr(a):
         return a.ctypes.data_as(dp)
