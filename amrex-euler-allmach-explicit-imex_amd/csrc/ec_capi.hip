@@ -54,6 +54,8 @@ static bool desc_ok(const ec_grid_desc* d)
   return true;
 }
 
+bool grid_desc_valid(const ec_grid_desc* d) { return desc_ok(d); }
+
 }  // namespace ec
 
 using namespace ec;
@@ -66,6 +68,7 @@ extern "C" const char* ec_status_string(ec_status s)
     case EC_ERR_HIP: return "HIP runtime error";
     case EC_ERR_OUT_OF_MEMORY: return "out of device memory";
     case EC_ERR_UNSUPPORTED: return "unsupported";
+    case EC_ERR_TOO_LARGE: return "grid exceeds the fixed workspace of the allocation-free entry point";
   }
   return "unknown status";
 }

@@ -153,48 +153,71 @@ int step_partials_needed(const ec_ctx* ctx, int tile_id)
   return tx * ty;
 }
 
-// device_dt: 0 host dt, 1 device-loop dt, 2 host dt without the diagnostics
-// epilogue (state only; the run control is left untouched).
-ec_status launch_explicit_step_on(ec_ctx* ctx, hipStream_t stream, const double* uin, double* uout,
-                                  long long rs, long long cs, double dt, int device_dt)
+int tiles_needed(int nx, int ny, int tile_id)
 {
-  const TileShape s = kShapes[ctx->tile_id];
+  const TileShape s = kShapes[tile_id];
+  return ((nx + s.tx - 1) / s.tx) * ((ny + s.ty - 1) / s.ty);
+}
+
+// The step kernel and its finalize on a caller-supplied workspace (run
+// control block and max_partials tile partials).  ev0/ev1: optional events
+// recorded around the step kernel.  device_dt as below.
+ec_status launch_step_raw(const GridParams& grid, int tile_id, hipStream_t stream, const double* uin,
+                          double* uout, long long rs, long long cs, double dt, int device_dt,
+                          DevCtrl* ctrl, Partial* partials, int max_partials, double* dt_log,
+                          hipEvent_t ev0, hipEvent_t ev1)
+{
+  const TileShape s = kShapes[tile_id];
   StepArgs a;
   a.uin = uin;
   a.uout = uout;
   a.rs = rs;
   a.cs = cs;
-  a.g = ctx->grid;
+  a.g = grid;
   // Exact ghost states at the step's start time: the run control's time in
-  // the device loop (read by the kernel), the context's boundary time else.
-  a.g.time_ptr = device_dt == 1 ? &ctx->ctrl->time : nullptr;
+  // the device loop (read by the kernel), the boundary time else.
+  a.g.time_ptr = device_dt == 1 ? &ctrl->time : nullptr;
   a.dt = dt;
   a.device_dt = device_dt == 1 ? 1 : 0;
   a.diag = device_dt == 2 ? 0 : 1;
-  a.ctrl = ctx->ctrl;
-  a.partials = ctx->partials;
-  a.dt_log = device_dt ? ctx->dt_log : nullptr;
-  a.tiles_x = (ctx->grid.nx + s.tx - 1) / s.tx;
-  const int tiles_y = (ctx->grid.ny + s.ty - 1) / s.ty;
+  a.ctrl = ctrl;
+  a.partials = partials;
+  a.dt_log = device_dt ? dt_log : nullptr;
+  a.tiles_x = (grid.nx + s.tx - 1) / s.tx;
+  const int tiles_y = (grid.ny + s.ty - 1) / s.ty;
   a.nblocks = a.tiles_x * tiles_y;
-  if (a.nblocks > ctx->max_partials) {
+  if (a.nblocks > max_partials) return EC_ERR_INVALID_ARGUMENT;
+  if (ev0 && hipEventRecord(ev0, stream) != hipSuccess) return EC_ERR_HIP;
+  step_fn(tile_id, grid.riemann, grid.order)(a, dim3(a.nblocks), dim3(s.nt), stream);
+  if (ev1 && hipEventRecord(ev1, stream) != hipSuccess) return EC_ERR_HIP;
+  if (a.diag) {
+    hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0, stream,
+                       partials, a.nblocks, ctrl, device_dt ? 1 : 0, a.dt_log);
+  }
+  return hipGetLastError() == hipSuccess ? EC_OK : EC_ERR_HIP;
+}
+
+// device_dt: 0 host dt, 1 device-loop dt, 2 host dt without the diagnostics
+// epilogue (state only; the run control is left untouched).
+ec_status launch_explicit_step_on(ec_ctx* ctx, hipStream_t stream, const double* uin, double* uout,
+                                  long long rs, long long cs, double dt, int device_dt)
+{
+  if (tiles_needed(ctx->grid.nx, ctx->grid.ny, ctx->tile_id) > ctx->max_partials) {
     ctx->last_error = "partials buffer too small for the tile shape";
     return EC_ERR_INVALID_ARGUMENT;
   }
   const bool timed =
       ctx->timing && stream == ctx->stream && ctx->tused + 2 <= (int)ctx->tev.size();
-  if (timed) EC_HIP_TRY(ctx, hipEventRecord(ctx->tev[ctx->tused], stream));
-  step_fn(ctx->tile_id, ctx->grid.riemann, ctx->grid.order)(a, dim3(a.nblocks), dim3(s.nt), stream);
-  if (timed) {
-    EC_HIP_TRY(ctx, hipEventRecord(ctx->tev[ctx->tused + 1], stream));
-    ctx->tused += 2;
+  hipEvent_t e0 = timed ? ctx->tev[ctx->tused] : nullptr;
+  hipEvent_t e1 = timed ? ctx->tev[ctx->tused + 1] : nullptr;
+  const ec_status st = launch_step_raw(ctx->grid, ctx->tile_id, stream, uin, uout, rs, cs, dt, device_dt,
+                                       ctx->ctrl, ctx->partials, ctx->max_partials, ctx->dt_log, e0, e1);
+  if (st == EC_ERR_HIP) {
+    ctx->last_error = std::string("step launch: ") + hipGetErrorString(hipGetLastError());
+    return st;
   }
-  if (a.diag) {
-    hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0, stream,
-                       ctx->partials, a.nblocks, ctx->ctrl, device_dt ? 1 : 0, a.dt_log);
-  }
-  EC_HIP_TRY(ctx, hipGetLastError());
-  return EC_OK;
+  if (st == EC_OK && timed) ctx->tused += 2;
+  return st;
 }
 
 ec_status launch_explicit_step(ec_ctx* ctx, const double* uin, double* uout, long long rs,

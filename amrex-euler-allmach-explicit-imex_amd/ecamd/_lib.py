@@ -13,7 +13,8 @@ LIB_PATH = PKG_DIR / "libec_amd.so"
 
 EC_OK = 0
 STATUS_TEXT = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of device memory",
-               4: "unsupported"}
+               4: "unsupported", 5: "grid exceeds the fixed workspace of the allocation-free entry point"}
+EC_ERR_TOO_LARGE = 5
 
 # euler.riemann spellings of core/config_parse_helpers.hpp:44-61
 RIEMANN = {"rusanov": 0, "local_lax_friedrichs": 0, "llf": 0, "hllc": 1, "lowmach_hllc": 2,
@@ -86,6 +87,9 @@ def lib() -> C.CDLL:
     L.ec_synchronize.restype = st
     L.ec_explicit_step.argtypes = [_vp, _vp, _vp, C.POINTER(Layout), C.c_double, C.POINTER(StepDiag)]
     L.ec_explicit_step.restype = st
+    L.ec_explicit_step_noalloc.argtypes = [C.POINTER(GridDesc), _vp, _vp, C.POINTER(Layout),
+                                           C.c_double, C.POINTER(StepDiag), _vp]
+    L.ec_explicit_step_noalloc.restype = st
     L.ec_explicit_step_inplace.argtypes = [_vp, _vp, C.POINTER(Layout), C.c_double,
                                            C.POINTER(StepDiag)]
     L.ec_explicit_step_inplace.restype = st

@@ -31,7 +31,8 @@ typedef enum {
   EC_ERR_INVALID_ARGUMENT = 1,
   EC_ERR_HIP = 2,
   EC_ERR_OUT_OF_MEMORY = 3,
-  EC_ERR_UNSUPPORTED = 4
+  EC_ERR_UNSUPPORTED = 4,
+  EC_ERR_TOO_LARGE = 5      /* beyond the fixed workspace of ec_explicit_step_noalloc */
 } ec_status;
 
 /* Riemann solver selector: the euler.riemann values of core/solver_types.hpp:11
@@ -131,6 +132,23 @@ ec_status ec_synchronize(ec_ctx* ctx);
  * otherwise the stream is synchronised and diag filled. */
 ec_status ec_explicit_step(ec_ctx* ctx, const double* U_in, double* U_out,
                            const ec_layout* layout, double dt, ec_step_diag* diag);
+
+/* ec_explicit_step without a context and without allocating anything, for
+ * callers that cannot allocate device or host memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc).  No reference counterpart.
+ * desc: as for ec_create (no exact-Dirichlet sides); stream: a hipStream_t or
+ * NULL; the default 16x15 tile shape.  Results equal ec_explicit_step's for
+ * the same grid.  The workspace (run control block and one partial per tile)
+ * is a fixed device array of the library, one per GPU, holding
+ * EC_NOALLOC_MAX_TILES tile partials: grids needing more tiles than that
+ * (ceil(nx/16) * ceil(ny/15) > 65536, e.g. beyond 4096 x 3840) are rejected
+ * with EC_ERR_TOO_LARGE.  Calls that overlap on one GPU race on that array.
+ * diag may be NULL (the call only enqueues work); otherwise the stream is
+ * synchronised and diag filled. */
+#define EC_NOALLOC_MAX_TILES 65536
+ec_status ec_explicit_step_noalloc(const ec_grid_desc* desc, const double* U_in, double* U_out,
+                                   const ec_layout* layout, double dt, ec_step_diag* diag,
+                                   void* stream);
 
 /* The same step in place, as the reference mutates its MultiFab: the result
  * goes through a context-owned buffer and is copied back (the reference's
