@@ -102,7 +102,8 @@ struct ImexWork {
   double* yo = nullptr;
   GmCtl* ctlo = nullptr;
   GmCtl* ctlo_host = nullptr;
-  std::vector<hipGraphExec_t> exec_outer;  // MO step graphs, then the closing graph
+  std::vector<hipGraphExec_t> exec_outer;    // MO step graphs, then the closing graph (FP32 inner)
+  std::vector<hipGraphExec_t> exec_outer64;  // the same with the FP64 inner solver
   // one-reduction cycle (FP32 inner solver)
   double* opart = nullptr;   // tile partials, 2 x otiles x KP (ping-pong by Arnoldi step)
   double* otot = nullptr;    // their totals (KP) on grids of many tiles
@@ -138,6 +139,9 @@ void imex_release(ec_ctx* ctx)
   if (w->ctl_host) (void)hipHostFree(w->ctl_host);
   if (w->ctlo_host) (void)hipHostFree(w->ctlo_host);
   for (hipGraphExec_t e : w->exec_outer) {
+    if (e) (void)hipGraphExecDestroy(e);
+  }
+  for (hipGraphExec_t e : w->exec_outer64) {
     if (e) (void)hipGraphExecDestroy(e);
   }
   if (w->hA64) (void)hipHostFree(w->hA64);
@@ -1312,8 +1316,9 @@ constexpr int MO = 20;  // outer restart length
 // A fresh inner cycle on the unit right-hand side v_j: u_0 = v_j in FP32,
 // g_0 = 1 and the inner control block reset (done at once when the outer
 // cycle is done).
+template <typename T>
 __global__ void __launch_bounds__(RB) inner_start_kernel(const double* __restrict__ v,
-                                                         float* __restrict__ U, long long n, GmCtl* ci,
+                                                         T* __restrict__ U, long long n, GmCtl* ci,
                                                          const GmCtl* co, double* g)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1327,7 +1332,7 @@ __global__ void __launch_bounds__(RB) inner_start_kernel(const double* __restric
     g[0] = 1.0;
   }
   for (long long t = (long long)blockIdx.x * RB + threadIdx.x; t < n; t += (long long)gridDim.x * RB) {
-    U[t] = (float)v[t];
+    U[t] = (T)v[t];
   }
 }
 
@@ -1767,11 +1772,39 @@ static void enqueue_cycle(ImexWork* w, hipStream_t st, const OpArgs<T>* dA, T* V
   (void)hipMemcpyAsync(w->ctl_host, w->ctl, sizeof(GmCtl), hipMemcpyDeviceToHost, st);
 }
 
-// Outer step j of the mixed form: the FP32 inner GMRES(m) solve of
-// A z_j = v_j (one-reduction kernels, unit right-hand side, relative
-// tolerance inner_thr), z_j in FP64, then w = A z_j orthogonalised against
-// V_0..V_j in FP64 (CGS2), column j's Givens step and V_{j+1} = w / |w|.
-// Every kernel returns at once when the outer cycle is done.
+// The one-reduction kernel for inner step k: the smallest instantiation that
+// holds the k basis values in registers (FP64 holds at most 16, which keeps
+// a 1024-thread workgroup within its 128 registers per lane; the rest are
+// read from memory).
+template <typename T>
+using OnesyncFn = void (*)(const OpArgs<T>*, T*, T*, int, int, const double*, double*, int, GmCtl*,
+                           double*, double*, double*, double*, double*, double*);
+
+template <typename T>
+static OnesyncFn<T> onesync_for(int k, bool summed)
+{
+  if constexpr (sizeof(T) == 4) {
+    return summed ? (k <= 8    ? onesync_kernel<float, 8, true>
+                     : k <= 16 ? onesync_kernel<float, 16, true>
+                     : k <= 24 ? onesync_kernel<float, 24, true>
+                               : onesync_kernel<float, 32, true>)
+                  : (k <= 8    ? onesync_kernel<float, 8, false>
+                     : k <= 16 ? onesync_kernel<float, 16, false>
+                     : k <= 24 ? onesync_kernel<float, 24, false>
+                               : onesync_kernel<float, 32, false>);
+  } else {
+    return summed ? (k <= 8 ? onesync_kernel<double, 8, true> : onesync_kernel<double, 16, true>)
+                  : (k <= 8 ? onesync_kernel<double, 8, false> : onesync_kernel<double, 16, false>);
+  }
+}
+
+// Outer step j of the flexible form: the inner GMRES(m) solve of A z_j = v_j
+// in precision T (FP32 for the mixed form, FP64 for its FP64 comparator;
+// one-reduction kernels, unit right-hand side, relative tolerance inner_thr),
+// z_j in FP64, then w = A z_j orthogonalised against V_0..V_j in FP64 (CGS2),
+// column j's Givens step and V_{j+1} = w / |w|.  Every kernel returns at once
+// when the outer cycle is done.
+template <typename T>
 static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
 {
   const long long n = (long long)w->nx * w->ny;
@@ -1779,28 +1812,32 @@ static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
   const int m = w->m;
   double* vj = w->Vo + (long long)j * n;
   double* zj = w->Zo + (long long)j * n;
-  hipLaunchKernelGGL(inner_start_kernel, dim3(nb), dim3(RB), 0, st, vj, w->V32, n, w->ctl, w->ctlo,
+  T* Ub;
+  T* Zb;
+  const OpArgs<T>* dA;
+  if constexpr (sizeof(T) == 4) {
+    Ub = w->V32;
+    Zb = w->w32;
+    dA = w->dA32;
+  } else {
+    Ub = w->V;
+    Zb = w->w;
+    dA = w->dA64;
+  }
+  hipLaunchKernelGGL((inner_start_kernel<T>), dim3(nb), dim3(RB), 0, st, vj, Ub, n, w->ctl, w->ctlo,
                      w->g);
   for (int k = 0; k <= m; ++k) {
-    // The smallest instantiation that holds the k basis values in registers.
     // Up to ONE_SUM_TILES tiles every workgroup sums the previous kernel's
     // tile partials itself; beyond, that redundant sum grows as tiles^2, and
     // a fixed-order finalize after each kernel sums them once instead.
     const bool summed = w->otiles > ONE_SUM_TILES;
-    auto kern = summed ? (k <= 8    ? onesync_kernel<float, 8, true>
-                          : k <= 16 ? onesync_kernel<float, 16, true>
-                          : k <= 24 ? onesync_kernel<float, 24, true>
-                                    : onesync_kernel<float, 32, true>)
-                       : (k <= 8    ? onesync_kernel<float, 8, false>
-                          : k <= 16 ? onesync_kernel<float, 16, false>
-                          : k <= 24 ? onesync_kernel<float, 24, false>
-                                    : onesync_kernel<float, 32, false>);
+    auto kern = onesync_for<T>(k, summed);
     // the partials ping-pong like z: a workgroup that finishes early must not
     // overwrite what a later workgroup of the same launch still has to read
     const size_t ps = (size_t)w->otiles * KP;
     double* pout = w->opart + (k & 1) * ps;
     const double* pin = summed ? w->otot : w->opart + ((k + 1) & 1) * ps;
-    hipLaunchKernelGGL(kern, dim3(w->otiles), dim3(ONT), 0, st, w->dA32, w->V32, w->w32, k, m, pin, pout,
+    hipLaunchKernelGGL(kern, dim3(w->otiles), dim3(ONT), 0, st, dA, Ub, Zb, k, m, pin, pout,
                        w->otiles, w->ctl, w->hraw, w->inv_nu, w->R, w->g, w->cs, w->sn);
     if (summed && k < m) {
       hipLaunchKernelGGL(finalize_dots_kernel, dim3(k + 2), dim3(RB), 0, st, (const double*)pout,
@@ -1810,8 +1847,8 @@ static void enqueue_outer_step(ImexWork* w, hipStream_t st, int j)
   hipLaunchKernelGGL(onesync_close_kernel, dim3(1), dim3(ONT), 0, st,
                      w->opart + (m & 1) * (size_t)w->otiles * KP, w->otiles, m, w->ctl,
                      w->hraw, w->R, w->g, w->cs, w->sn, w->y, (const double*)w->inv_nu);
-  hipLaunchKernelGGL((xupdate_kernel<float, double>), dim3(nb), dim3(RB), 0, st, w->V32, n, w->y, zj,
-                     1.0, w->ctl, 1);
+  hipLaunchKernelGGL((xupdate_kernel<T, double>), dim3(nb), dim3(RB), 0, st, Ub, n, w->y, zj, 1.0,
+                     w->ctl, 1);
   hipLaunchKernelGGL(outer_arnoldi_kernel, dim3(nb), dim3(RB), 0, st, w->dA64, w->Vo, zj, j, w->w,
                      w->partials, w->ctlo);
   hipLaunchKernelGGL(finalize_dots_kernel, dim3(j + 1), dim3(RB), 0, st, w->partials, nb, MAXK,
@@ -1891,10 +1928,12 @@ static ec_status true_residual(ec_ctx* ctx, ImexWork* w, hipStream_t st, double*
   return EC_OK;
 }
 
-// The mixed form's FP64 outer buffers and step graphs (first use only).
-static ec_status outer_setup(ec_ctx* ctx, ImexWork* w)
+// The flexible form's FP64 outer buffers and step graphs for one inner
+// precision (first use only).
+static ec_status outer_setup(ec_ctx* ctx, ImexWork* w, bool fp64_inner)
 {
-  if (w->Vo && (int)w->exec_outer.size() == MO + 1) return EC_OK;
+  std::vector<hipGraphExec_t>& execs = fp64_inner ? w->exec_outer64 : w->exec_outer;
+  if (w->Vo && (int)execs.size() == MO + 1) return EC_OK;
   const size_t n = (size_t)w->nx * w->ny;
   if (!w->Vo) {
     bool ok = true;
@@ -1912,11 +1951,12 @@ static ec_status outer_setup(ec_ctx* ctx, ImexWork* w)
     std::memset(w->ctlo_host, 0, sizeof(GmCtl));
     EC_HIP_TRY(ctx, hipMemcpy(w->ctlo, w->ctlo_host, sizeof(GmCtl), hipMemcpyHostToDevice));
   }
-  w->exec_outer.assign(MO + 1, nullptr);
+  execs.assign(MO + 1, nullptr);
   for (int j = 0; j <= MO; ++j) {
-    const ec_status st = cycle_graph(ctx, w, &w->exec_outer[j], [&](hipStream_t gs) {
+    const ec_status st = cycle_graph(ctx, w, &execs[j], [&](hipStream_t gs) {
       if (j < MO) {
-        enqueue_outer_step(w, gs, j);
+        if (fp64_inner) enqueue_outer_step<double>(w, gs, j);
+        else enqueue_outer_step<float>(w, gs, j);
       } else {
         enqueue_outer_close(w, gs);
       }
@@ -1933,8 +1973,10 @@ static ec_status outer_setup(ec_ctx* ctx, ImexWork* w)
 static ec_status solve_mixed(ec_ctx* ctx, ImexWork* w, double target, double rhs_norm,
                              const ec_imex_opts& o, double rinf, double r2, SolveResult* out)
 {
-  ec_status st = outer_setup(ctx, w);
+  const bool fp64_inner = o.precision == EC_IMEX_PRECISION_FP64_FLEXIBLE;
+  ec_status st = outer_setup(ctx, w, fp64_inner);
   if (st != EC_OK) return st;
+  const std::vector<hipGraphExec_t>& execs = fp64_inner ? w->exec_outer64 : w->exec_outer;
   const long long n = (long long)w->nx * w->ny;
   GmCtl& h = *w->ctlo_host;
   int total = 0;
@@ -1956,7 +1998,7 @@ static ec_status solve_mixed(ec_ctx* ctx, ImexWork* w, double target, double rhs
     hipLaunchKernelGGL((cycle_start_kernel<double>), dim3(grid_blocks(n)), dim3(RB), 0, w->gs, w->r,
                        w->Vo, n, w->ctlo, w->go);
     EC_HIP_TRY(ctx, hipGetLastError());
-    for (int j = 0; j <= MO; ++j) EC_HIP_TRY(ctx, hipGraphLaunch(w->exec_outer[j], w->gs));
+    for (int j = 0; j <= MO; ++j) EC_HIP_TRY(ctx, hipGraphLaunch(execs[j], w->gs));
     EC_HIP_TRY(ctx, hipStreamSynchronize(w->gs));
     const int kk = h.kk;
     total += h.inner_its;
@@ -2006,7 +2048,7 @@ static ec_status solve_pressure(ec_ctx* ctx, ImexWork* w, const OpArgs<double>& 
     out->bad = true;
     return EC_OK;
   }
-  if (o.precision == EC_IMEX_PRECISION_MIXED) {
+  if (o.precision == EC_IMEX_PRECISION_MIXED || o.precision == EC_IMEX_PRECISION_FP64_FLEXIBLE) {
     return solve_mixed(ctx, w, target, rhs_norm, o, rinf, r2, out);
   }
   st = cycle_graph(ctx, w, &w->exec64, [&](hipStream_t gs) {
@@ -2169,7 +2211,8 @@ extern "C" ec_status ec_imex_step(ec_ctx* ctx, double* U, const ec_layout* lay, 
   }
   if (opts->picard_iterations < 1 || opts->restart < 1 || opts->restart > MAXK ||
       !(opts->tol > 0.0) || opts->max_iter < 0 ||
-      (opts->precision != EC_IMEX_PRECISION_FP64 && opts->precision != EC_IMEX_PRECISION_MIXED) ||
+      (opts->precision != EC_IMEX_PRECISION_FP64 && opts->precision != EC_IMEX_PRECISION_MIXED &&
+       opts->precision != EC_IMEX_PRECISION_FP64_FLEXIBLE) ||
       !(dt > 0.0) || !std::isfinite(dt)) {
     return EC_ERR_INVALID_ARGUMENT;
   }

@@ -212,7 +212,8 @@ class ExplicitSolver:
         check(None, lib().ec_imex_default_opts(C.byref(o)), "ec_imex_default_opts")
         prec = kw.pop("precision", None)
         if prec is not None:
-            o.precision = {"fp64": 0, "mixed": 1}[prec] if isinstance(prec, str) else int(prec)
+            o.precision = ({"fp64": 0, "mixed": 1, "fp64_flexible": 2}[prec] if isinstance(prec, str)
+                           else int(prec))
         for k, v in kw.items():
             setattr(o, k, v)
         return o

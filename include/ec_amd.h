@@ -207,7 +207,16 @@ typedef enum {
   EC_IMEX_NOT_RUN = 8               /* not_run */
 } ec_imex_status;
 
-typedef enum { EC_IMEX_PRECISION_FP64 = 0, EC_IMEX_PRECISION_MIXED = 1 } ec_imex_precision;
+/* FP64: restarted GMRES(restart) with CGS2, all in FP64.  MIXED: FP64
+ * flexible GMRES(20) whose preconditioner is an FP32 GMRES(restart) with one
+ * reduction per Arnoldi step.  FP64_FLEXIBLE: the MIXED structure with the
+ * inner solve in FP64 (the same kernels instantiated for double), the
+ * comparator that isolates what the FP32 inner precision contributes. */
+typedef enum {
+  EC_IMEX_PRECISION_FP64 = 0,
+  EC_IMEX_PRECISION_MIXED = 1,
+  EC_IMEX_PRECISION_FP64_FLEXIBLE = 2
+} ec_imex_precision;
 
 typedef struct {
   int picard_iterations;     /* euler.imex_picard_iterations */

@@ -41,7 +41,7 @@ def test_imex_dt_matches_oracle(ecor, ecamd, gpu):
     assert (dt, m, a) == (dto, mo, ao)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp64_flexible"])
 def test_gresho_imex_step_parity(ecor, ecamd, gpu, precision):
     """BASELINE config 3 at 64^2 (M=0.001), one step.  The oracle's assembled
     rows stagnate near 2.5e-10 relative residual at this size, so its tightest
@@ -87,7 +87,7 @@ def test_config3_512_against_oracle(ecor, ecamd, gpu):
     assert r.status in (0, 1) and r.relative_residual_linf <= 1e-8
     s = solver(ecamd, n, gpu)
     states = {}
-    for precision in ("fp64", "mixed"):
+    for precision in ("fp64", "fp64_flexible", "mixed"):
         Ud = torch.from_numpy(U).to(gpu)
         p = torch.empty(n * n, dtype=torch.float64, device=gpu)
         d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-10, max_iter=20000, restart=30,
@@ -102,10 +102,11 @@ def test_config3_512_against_oracle(ecor, ecamd, gpu):
         assert err[1] <= 2e-6 and err[2] <= 2e-6 and err[3] <= 5e-9, err
         assert perr <= 3e-10, perr
         states[precision] = got
-    # mixed against FP64, both at 1e-10
-    e = rel_l1(states["mixed"], states["fp64"])
-    print("mixed vs fp64", e)
-    assert e[0] == 0.0 and np.all(e[1:3] <= 1e-7) and e[3] <= 1e-9, e
+    # the flexible forms against the plain FP64 form, all at 1e-10
+    for k in ("mixed", "fp64_flexible"):
+        e = rel_l1(states[k], states["fp64"])
+        print(k, "vs fp64", e)
+        assert e[0] == 0.0 and np.all(e[1:3] <= 1e-7) and e[3] <= 1e-9, e
 
 
 def test_mixed_long_inner_cycles_match_fp64(ecor, ecamd, gpu):
@@ -242,7 +243,7 @@ def smooth_low_mach(nx, ny, mach=0.01):
     return np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp64_flexible"])
 @pytest.mark.parametrize("bcs", [("outflow", "outflow", "periodic", "periodic"),
                                  ("fixed", "outflow", "reflect", "outflow")])
 def test_non_periodic_pressure_solves_the_reference_rows(ecor, ecamd, gpu, precision, bcs):
