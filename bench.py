@@ -394,21 +394,59 @@ def bench_adjoint(dev, nx: int = 640, ny: int = 160, nsteps: int = 768, K: int =
     rng = np.random.default_rng(20261203)
     errs = []
     eps = fd_eps
+    D = []
     for _ in range(ndirs):
         ph = rng.uniform(0, 2 * np.pi, 2)
         kx, ky = rng.integers(2, 7, 2)
         d = torch.sin(float(kx) * X + ph[0]) * torch.cos(float(ky) * Y + ph[1]) * bubble
+        D.append(d)
         Jp = float(obj(run_with_dts(s, conserved_from_density(rho + eps * d, u, v, p, 1.4), dts)))
         Jm = float(obj(run_with_dts(s, conserved_from_density(rho - eps * d, u, v, p, 1.4), dts)))
         fd = (Jp - Jm) / (2 * eps)
         ad = float((grad * d).sum())
         errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
+    frozen = frozen_branch_fd(rho, u, v, p, Ub, dts, inflow, torch.stack(D), grad, region,
+                              lx / nx, ly / ny)
     s.close()
     return {"workload": f"shock_bubble_{nx}x{ny}_M1.22_hllc_o2_cfl{cfl:g}_{nsteps}steps_K{K}",
             "forward_ms": fwd_ms, "forward_checkpointed_ms": fwd_chk_ms, "adjoint_ms": adj_ms,
             "adjoint_over_forward": adj_ms / fwd_ms, "steps": nsteps, "checkpoint_every": K,
             "fd_directions": ndirs, "fd_eps": eps, "fd_rel_err_max": max(errs),
-            "fd_rel_err_median": sorted(errs)[len(errs) // 2]}
+            "fd_rel_err_median": sorted(errs)[len(errs) // 2],
+            "frozen_branch_fd": frozen}
+
+
+def frozen_branch_fd(rho, u, v, p, Ub, dts, inflow, D, grad, region, dx, dy, eps: float = 1e-5):
+    """Central differences of the frozen-branch forward map on the same
+    directions: tests/torch_step.py (an independent FP64 torch restatement of
+    the step) records every branch decision of the forward trajectory, then
+    replays them for all 2 x ndirs perturbed runs as one batched run.  This is
+    the map the adjoint differentiates (dt and every limiter and Riemann
+    branch frozen), so its differences carry no kink-crossing error."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, str(ROOT / "tests"))
+    import torch_step  # noqa: E402  (the test suite's differentiable restatement of the step)
+    from ecamd.adjoint import conserved_from_density
+    bcs = ("fixed", "outflow", "reflect", "outflow")
+    B = torch_step.Branches("record")
+    with torch.no_grad():
+        V = Ub[:, None].clone()
+        for dt in dts:
+            V = torch_step.step(V, float(dt), dx, dy, riemann="hllc", bcs=bcs,
+                                fixed=(inflow, None, None, None), B=B)
+        nd = D.shape[0]
+        R = torch.cat([rho[None] + eps * D, rho[None] - eps * D])
+        V = conserved_from_density(R, u[None], v[None], p[None], 1.4)
+        B.replay()
+        for dt in dts:
+            V = torch_step.step(V, float(dt), dx, dy, riemann="hllc", bcs=bcs,
+                                fixed=(inflow, None, None, None), B=B)
+        Jv = (0.5 * (V[1] ** 2 + V[2] ** 2) / V[0] * region).sum((-2, -1)) * (dx * dy)
+    fd = (Jv[:nd] - Jv[nd:]) / (2 * eps)
+    ad = (grad[None] * D).sum((-2, -1))
+    err = ((ad - fd).abs() / ad.abs()).cpu().numpy()
+    return {"eps": eps, "rel_err_max": float(err.max()), "rel_err_median": float(np.median(err))}
 
 
 def bench_ensemble(dev, riemann: str, stop_time: float = 0.1, cfl: float = 0.8,
