@@ -47,7 +47,8 @@ struct GmCtl {
   int done, kk, breakdown, bad;
   int mlim;                    // iterations allowed in this cycle
   int inner_its;               // flexible outer solver: FP32 iterations of this cycle
-  int inner_mlim, pad1;        // flexible outer solver: the inner restart length
+  int inner_mlim;              // flexible outer solver: the inner restart length
+  int inner_budget;            // flexible outer solver: inner iterations this cycle may take
   double thr, est, beta, inv_norm;
   double rinf, r2;             // true residual norms
   double inner_thr;            // flexible outer solver: inner relative tolerance
@@ -1319,14 +1320,19 @@ constexpr int MO = 20;  // outer restart length
 template <typename T>
 __global__ void __launch_bounds__(RB) inner_start_kernel(const double* __restrict__ v,
                                                          T* __restrict__ U, long long n, GmCtl* ci,
-                                                         const GmCtl* co, double* g)
+                                                         GmCtl* co, double* g)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the iterations left of euler.imex_solver_max_iter: a spent budget
+    // closes the outer cycle before this step (its kernels read co->done)
+    const int left = co->inner_budget - co->inner_its;
+    if (left <= 0) co->done = 1;
     ci->done = co->done;
     ci->kk = 0;
     ci->bad = 0;
     ci->breakdown = 0;
-    ci->mlim = co->inner_mlim;  // the requested restart (the graph holds the workspace's m)
+    // the requested restart (the graph holds the workspace's m), at most the budget left
+    ci->mlim = left < co->inner_mlim ? left : co->inner_mlim;
     ci->beta = 1.0;
     ci->thr = co->inner_thr;
     g[0] = 1.0;
@@ -1993,6 +1999,7 @@ static ec_status solve_mixed(ec_ctx* ctx, ImexWork* w, double target, double rhs
     h.thr = target * (beta / rinf);  // as in the FP64 form
     h.inner_thr = o.inner_reduction;
     h.inner_mlim = std::min(o.restart, w->m);
+    h.inner_budget = o.max_iter - total;
     EC_HIP_TRY(ctx, hipMemcpyAsync(w->ctlo, w->ctlo_host, offsetof(GmCtl, dots1),
                                    hipMemcpyHostToDevice, w->gs));
     hipLaunchKernelGGL((cycle_start_kernel<double>), dim3(grid_blocks(n)), dim3(RB), 0, w->gs, w->r,

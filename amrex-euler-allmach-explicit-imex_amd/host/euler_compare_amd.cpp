@@ -501,6 +501,25 @@ std::vector<PlotBox> max_size_boxes(int nx, int ny, int chunk)
   return out;
 }
 
+// The cells of the domain in the order the reference's CSV writers visit
+// them: MFIter over the boxes of the amr.max_grid_size layout, in
+// BoxList::maxSize order, then j and i within each box
+// (io/output_cases.hpp:45-50, 89-92, 133-136, 183-188, 466-469;
+// imex/imex_bdltv20_direct_driver.hpp:677).
+struct CellIJ { int i, j; };
+
+std::vector<CellIJ> cells_in_box_order(int nx, int ny, int max_grid_size)
+{
+  std::vector<CellIJ> out;
+  out.reserve((size_t)nx * ny);
+  for (const PlotBox& b : max_size_boxes(nx, ny, max_grid_size > 0 ? max_grid_size : nx + ny)) {
+    for (int j = b.jlo; j <= b.jhi; ++j) {
+      for (int i = b.ilo; i <= b.ihi; ++i) out.push_back(CellIJ{i, j});
+    }
+  }
+  return out;
+}
+
 bool write_plotfile(const Cfg& c, const std::vector<double>& U, int step, double time)
 {
   const int nx = c.n_cell[0], ny = c.n_cell[1];
@@ -674,8 +693,8 @@ bool write_sb_snapshot(const Cfg& c, const std::vector<double>& U, double dx, do
          "geometric_source_form\n";
   const std::string scheme = sb_scheme_name(c);
   const char* rl = c.method == "imex" ? "imex_pressure_split" : riemann_name(c.riemann_kind);
-  for (int j = 0; j < ny; ++j) {
-    for (int i = 0; i < nx; ++i) {
+  for (const CellIJ& cell : cells_in_box_order(nx, ny, c.max_grid_size)) {
+    const int i = cell.i, j = cell.j;
       const size_t k = (size_t)j * nx + i;
       const double r = U[k], mx = U[n + k], my = U[2 * n + k], E = U[3 * n + k];
       const double inv = 1.0 / r, u = mx * inv, v = my * inv;
@@ -689,7 +708,6 @@ bool write_sb_snapshot(const Cfg& c, const std::vector<double>& U, double dx, do
           << ',' << (r < threshold ? 1 : 0) << ',' << threshold << ',' << g << ',' << c.method << ','
           << scheme << ',' << rl << ',' << c.spatial_order << ',' << sb_claim(c) << ",false,false,1,"
           << sb_source_columns(c) << '\n';
-    }
   }
   return static_cast<bool>(out);
 }
@@ -837,15 +855,14 @@ int run_direct(const Cfg& cfg, const ec_problem_desc& pd)
     std::ofstream out(cfg.final_csv);
     out << std::setprecision(17);
     out << "x,y,rho,u,v,p,exact_rho,exact_u,exact_v,exact_p\n";
-    for (int j = 0; j < ny; ++j) {
-      for (int i = 0; i < nx; ++i) {
+    for (const CellIJ& cell : cells_in_box_order(nx, ny, cfg.max_grid_size)) {
+      const int i = cell.i, j = cell.j;
         const size_t k = (size_t)j * nx + i;
         double q[4];
         prim_at(k, q);
         out << cfg.prob_lo[0] + ((double)i + 0.5) * dx << ',' << cfg.prob_lo[1] + ((double)j + 0.5) * dy
             << ',' << q[0] << ',' << q[1] << ',' << q[2] << ',' << q[3] << ',' << exact[4 * k] << ','
             << exact[4 * k + 1] << ',' << exact[4 * k + 2] << ',' << exact[4 * k + 3] << '\n';
-      }
     }
     csv_ok = static_cast<bool>(out);
   }
@@ -1103,8 +1120,10 @@ int main(int argc, char** argv)
     rs.stop_time = cfg.stop_time;
     rs.cfl = cfg.cfl;
     rs.max_step = cfg.max_step;
-    const int chunk = cfg.plot_int > 0 ? std::min(cfg.plot_int, 512) : 512;
     while (true) {
+      // each chunk ends at the next plot step (io/output_cases.hpp:8 plots
+      // when step % plot_int == 0), at most 512 steps long
+      const int chunk = cfg.plot_int > 0 ? std::min(cfg.plot_int - rs.step % cfg.plot_int, 512) : 512;
       int in_b = 0;
       const int before = rs.step;
       EC_OK_OR_DIE(ctx, ec_explicit_advance(ctx, A, B, &lay, chunk, &rs, &in_b));
@@ -1243,13 +1262,12 @@ int main(int argc, char** argv)
     const char* riemann_label_2 = imex_run ? "imex_pressure_split" : riemann_name(cfg.riemann_kind);
     if (cfg.problem_kind == EC_PROBLEM_TORO) {
       out << "i,j,x,y,rho,u,v,p,exact_rho,exact_u,exact_v,exact_p,rho_error,u_error,v_error,p_error\n";
-      for (int j = 0; j < ny; ++j) {
-        for (int i = 0; i < nx; ++i) {
+      for (const CellIJ& cell : cells_in_box_order(nx, ny, cfg.max_grid_size)) {
+        const int i = cell.i, j = cell.j;
           double q[4];
           prim((size_t)j * nx + i, q);
           out << i << ',' << j << ',' << xc(i) << ',' << yc(j) << ',' << q[0] << ',' << q[1] << ',' << q[2]
               << ',' << q[3] << ",nan,nan,nan,nan,nan,nan,nan,nan\n";
-        }
       }
     } else if (cfg.problem_kind == EC_PROBLEM_GRESHO) {
       // the run minima of the IMEX admissibility checks, NaN without IMEX steps
@@ -1261,8 +1279,8 @@ int main(int argc, char** argv)
           << "spatial_order,slope_limiter,"
           << "imex_trial_density_run_min,"
           << "imex_high_trial_pressure_run_min,imex_high_trial_internal_energy_run_min\n";
-      for (int j = 0; j < ny; ++j) {
-        for (int i = 0; i < nx; ++i) {
+      for (const CellIJ& cell : cells_in_box_order(nx, ny, cfg.max_grid_size)) {
+        const int i = cell.i, j = cell.j;
           double q[4], e[4];
           prim((size_t)j * nx + i, q);
           ec_exact_state_at(&pd, cfg.is_periodic, xc(i), yc(j), time, e);
@@ -1278,15 +1296,14 @@ int main(int argc, char** argv)
               << cfg.spatial_order << ',' << limiter_name(cfg.limiter_kind) << ','
               << (have ? adm_rho_min : nan) << ',' << (have ? adm_p_min : nan) << ','
               << (have ? adm_e_min : nan) << '\n';
-        }
       }
     } else if (cfg.problem_kind == EC_PROBLEM_RIEMANN_QUADRANT) {
       out << "i,j,x,y,rho,u,v,p,initial_rho,initial_u,initial_v,initial_p,"
              "time,step,method,riemann,spatial_order,slope_limiter,"
              "field_boundary,geometry_is_periodic_x,geometry_is_periodic_y,"
              "quadrant_case,claim_limit\n";
-      for (int j = 0; j < ny; ++j) {
-        for (int i = 0; i < nx; ++i) {
+      for (const CellIJ& cell : cells_in_box_order(nx, ny, cfg.max_grid_size)) {
+        const int i = cell.i, j = cell.j;
           double q[4], e[4];
           prim((size_t)j * nx + i, q);
           ec_exact_state_at(&pd, cfg.is_periodic, xc(i), yc(j), 0.0, e);
@@ -1297,14 +1314,13 @@ int main(int argc, char** argv)
               << (cfg.is_periodic[0] ? 1 : 0) << ',' << (cfg.is_periodic[1] ? 1 : 0) << ','
               << "liska_wendroff_clawpack_quadrant,"
               << "genuine_2d_visual_finite_smoke_no_exact_solution\n";
-        }
       }
     } else {
       out << "x,y,rho,u,v,pressure,exact_rho,exact_u,exact_v,exact_pressure,"
           << "density_error,velocity_error,pressure_error,time,step,method,riemann,"
           << "spatial_order,slope_limiter,geometry_is_periodic_x,geometry_is_periodic_y\n";
-      for (int j = 0; j < ny; ++j) {
-        for (int i = 0; i < nx; ++i) {
+      for (const CellIJ& cell : cells_in_box_order(nx, ny, cfg.max_grid_size)) {
+        const int i = cell.i, j = cell.j;
           double q[4], e[4];
           prim((size_t)j * nx + i, q);
           ec_exact_state_at(&pd, cfg.is_periodic, xc(i), yc(j), time, e);
@@ -1315,7 +1331,6 @@ int main(int argc, char** argv)
               << cfg.method << ',' << riemann_label_2 << ',' << cfg.spatial_order << ','
               << limiter_name(cfg.limiter_kind) << ',' << (cfg.is_periodic[0] ? 1 : 0) << ','
               << (cfg.is_periodic[1] ? 1 : 0) << '\n';
-        }
       }
     }
   }

@@ -221,8 +221,9 @@ typedef enum {
 typedef struct {
   int picard_iterations;     /* euler.imex_picard_iterations */
   double tol;                /* euler.imex_solver_tol (absolute or relative-to-max(|b|inf,1)) */
-  int max_iter;              /* euler.imex_solver_max_iter: Krylov iterations per solve (mixed:
-                                FP32 inner iterations, checked at outer restarts) */
+  int max_iter;              /* euler.imex_solver_max_iter: Krylov iterations per solve (the
+                                flexible forms: inner iterations; an outer cycle stops once
+                                they are spent) */
   int restart;               /* GMRES restart length, 1..64 (mixed: of the FP32 inner cycles) */
   int precision;             /* ec_imex_precision: FP64 GMRES, or MIXED: FP64 flexible outer
                                 GMRES(20) preconditioned by FP32 GMRES(restart) */

@@ -28,6 +28,22 @@ euler.cfl = 0.4
 """
 
 
+def grid_field(rows, col, nx, ny, dx=None, dy=None, xlo=0.0, ylo=0.0):
+    """A CSV column as an (ny, nx) field, each row placed by its own i, j (or
+    x, y) columns: the writers visit the cells box by box (amr.max_grid_size),
+    as the reference's MFIter loops do."""
+    f = np.full((ny, nx), np.nan)
+    for r in rows:
+        if "i" in r:
+            i, j = int(r["i"]), int(r["j"])
+        else:
+            i = int(round((float(r["x"]) - xlo) / dx - 0.5))
+            j = int(round((float(r["y"]) - ylo) / dy - 0.5))
+        f[j, i] = float(r[col])
+    assert not np.isnan(f).any()
+    return f
+
+
 def run(tmp_path, *overrides):
     inp = tmp_path / "inputs"
     inp.write_text(INPUTS)
@@ -53,8 +69,12 @@ def test_sod_row(tmp_path, ecor):
                                     0.8, 0.2, 100000)
     rows = list(csv.DictReader(open(out_csv)))
     assert len(rows) == 1600
-    rho = np.array([float(r["rho"]) for r in rows]).reshape(4, 400)
+    rho = grid_field(rows, "rho", 400, 4)
     assert np.array_equal(rho, ref[0])
+    # box by box, as the reference's MFIter loop writes them: 400 cells
+    # across with max_grid_size 32 gives 32-wide boxes, so the row after
+    # (31, 0) is (0, 1)
+    assert [(int(r["i"]), int(r["j"])) for r in rows[31:33]] == [(31, 0), (0, 1)]
 
 
 def test_capped_and_failure_rows(tmp_path):
@@ -145,7 +165,7 @@ def test_toro_exact_dirichlet_row(tmp_path, ecor):
     ref, res, _ = ecor.run_explicit(g, ecor.init_toro(nx, ny, L, R), 0.4, 0.25, 100000)
     assert int(kv["completed_steps"]) == res.steps
     rows = list(csv.DictReader(open(out_csv)))
-    rho = np.array([float(r["rho"]) for r in rows]).reshape(ny, nx)
+    rho = grid_field(rows, "rho", nx, ny)
     # the fan's ghost states take the device pow(): relative 1e-12 in L1
     assert np.abs(rho - ref[0]).sum() <= 1e-12 * np.abs(ref[0]).sum()
     # the exact boundary differs from outflow (the y ghost rows)
@@ -188,10 +208,12 @@ def test_direct_bdltv20_toro_row(tmp_path, ecor):
     rows = list(csv.DictReader(open(out_csv)))
     assert list(rows[0].keys()) == ["x", "y", "rho", "u", "v", "p", "exact_rho", "exact_u",
                                     "exact_v", "exact_p"]
-    rho = np.array([float(r["rho"]) for r in rows]).reshape(ny, nx)
+    dx_, dy_ = 1 / nx, 0.078125 / ny
+    rho = grid_field(rows, "rho", nx, ny, dx_, dy_)
     assert np.abs(rho - U[0]).max() <= 1e-9 * np.abs(U[0]).max()
-    ex = np.array([ecor.exact_riemann_sample(float(r["x"]), t, L, R)[0] for r in rows[:nx]])
-    assert np.array_equal(np.array([float(r["exact_rho"]) for r in rows[:nx]]), ex)
+    xs = (np.arange(nx) + 0.5) * dx_
+    ex = np.array([ecor.exact_riemann_sample(x, t, L, R)[0] for x in xs])
+    assert np.array_equal(grid_field(rows, "exact_rho", nx, ny, dx_, dy_)[0], ex)
     l1 = np.abs(rho - ex[None, :]).mean()
     assert float(kv["bdltv20_paper_density_l1_error"]) == pytest.approx(l1, rel=1e-9)
 
@@ -255,7 +277,7 @@ def test_gresho_final_csv_schema(tmp_path, ecor):
         ("explicit", "xie_am_hllc_p", "not_applicable", "minmod")
     assert r0["imex_trial_density_run_min"] == "nan"
     U0 = ecor.init_gresho(n, n, mach=mach, p0=1 / (1.4 * mach * mach))
-    p_ex = np.array([float(r["exact_pressure"]) for r in rows]).reshape(n, n)
+    p_ex = grid_field(rows, "exact_pressure", n, n, 1 / n, 1 / n)
     p_init = 0.4 * (U0[3] - 0.5 * (U0[1] ** 2 + U0[2] ** 2) / U0[0])
     assert np.abs(p_ex - p_init).max() <= 1e-13 * p_init.max()
     # the errors are the differences of the columns
@@ -317,9 +339,24 @@ def test_plotfiles(tmp_path):
     # splits 5 into 2 + 2 + 1: boxes 32, 32 and 16 cells wide
     assert ch[4].startswith("(3 0") and ch[5] == "((0,0) (31,7) (0,0))" and ch[7] == "((64,0) (79,7) (0,0))"
     rows = list(csv.DictReader(open(out_csv)))
-    rho = np.array([float(r["rho"]) for r in rows]).reshape(8, 80)
-    p = np.array([float(r["p"]) for r in rows]).reshape(8, 80)
+    rho = grid_field(rows, "rho", 80, 8)
+    p = grid_field(rows, "p", 80, 8)
     assert np.array_equal(field[0], rho) and np.array_equal(field[4], p)
     nb = 3
     mins = [[float(v) for v in ln.rstrip(",").split(",")] for ln in ch[4 + nb + 3 + nb + 2: 4 + nb + 3 + nb + 2 + nb]]
     assert mins[0][0] == field[0][:, :32].min() and mins[2][4] == field[4][:, 64:].min()
+
+
+def test_plot_interval_longer_than_a_device_chunk(tmp_path):
+    """amr.plot_int beyond the device loop's 512-step chunks and not a
+    multiple of them: a plotfile at every multiple of plot_int (write_plotfile
+    at step % plot_int == 0, io/output_cases.hpp:8), at step 0 and at the
+    end."""
+    rc, kv, err = run(tmp_path, "amr.n_cell=64 4", "amr.plot_int=600", "geometry.prob_hi=1 0.0625",
+                      "geometry.is_periodic=1 1", "euler.problem=gresho_vortex",
+                      "euler.vortex_center=0.5 0.03125", "euler.mach=0.1", "euler.riemann=rusanov",
+                      "euler.spatial_order=1", "stop_time=100.0", "max_step=1300")
+    assert rc == 0, err
+    assert kv["completed_steps"] == "1300"
+    plts = sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("plt"))
+    assert plts == ["plt00000", "plt00600", "plt01200", "plt01300"]

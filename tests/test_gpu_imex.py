@@ -109,6 +109,26 @@ def test_config3_512_against_oracle(ecor, ecamd, gpu):
         assert e[0] == 0.0 and np.all(e[1:3] <= 1e-7) and e[3] <= 1e-9, e
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp64_flexible"])
+def test_iterations_stay_within_max_iter(ecor, ecamd, gpu, precision):
+    """euler.imex_solver_max_iter bounds the Krylov iterations of a solve in
+    every form (the flexible forms count their inner iterations, and stop an
+    outer cycle once the budget is spent): a tolerance out of reach with 50
+    iterations ends not converged after at most 50 per Picard solve."""
+    n = 64
+    U = ecor.init_gresho(n, n, mach=0.001)
+    g = ecor.make_grid(n, n, 1 / n, 1 / n)
+    dt, _, _ = ecor.imex_dt(g, U, 0.8, 0.0, 1.0)
+    s = solver(ecamd, n, gpu)
+    Ud = torch.from_numpy(U).to(gpu)
+    d = s.imex_step(Ud, dt, s.imex_opts(tol=1e-15, max_iter=50, precision=precision,
+                                        picard_iterations=1))
+    print(precision, ecamd.IMEX_STATUS[d.status], d.iterations, d.last_iterations)
+    assert ecamd.IMEX_STATUS[d.status] == "linear_solve_not_converged"
+    assert 0 < d.iterations <= 50 and d.last_iterations <= 50
+    assert np.array_equal(Ud.cpu().numpy(), U)           # not applied
+
+
 def test_mixed_long_inner_cycles_match_fp64(ecor, ecamd, gpu):
     """The mixed form with GMRES(64) inner cycles run to their full length
     (inner tolerance 1e-14): more than 32 Arnoldi steps per inner cycle, so

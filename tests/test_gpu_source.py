@@ -114,7 +114,10 @@ def test_cylindrical_driver_row(tmp_path, ecor, ecamd):
     rows = list(csv.DictReader(open(files[0])))
     assert rows[0]["cylindrical_source_used"] == "true"
     assert rows[0]["geometric_source_form"] == "clawpack_radial_symmetry_fractional_source_step"
-    rho = np.array([float(x["rho"]) for x in rows]).reshape(40, 160)
+    rho = np.full((40, 160), np.nan)
+    for x in rows:                   # box by box (amr.max_grid_size): place by i, j
+        rho[int(x["j"]), int(x["i"])] = float(x["rho"])
+    assert not np.isnan(rho).any()
     s, g, U = sb_setup(ecor, ecamd, 1)
     t, steps = 0.0, 0
     while t < 0.02 - 1e-14:
