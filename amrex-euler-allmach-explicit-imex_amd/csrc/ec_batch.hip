@@ -368,6 +368,7 @@ struct ec_batch {
   int ntiles = 0;
   hipGraphExec_t graph = nullptr;
   int graph_shape = -1;
+  int graph_rounds = 0;      // rounds the instantiated graph holds
   int last_mode = 0, last_shape = 0, last_rounds = 0;
   double last_ms = 0.0;
   unsigned spin_limit = 1u << 20;
@@ -528,7 +529,7 @@ ec_status run_graph(ec_batch* b, int* rounds_out)
   const BatchShape s = kBatchShapes[b->cfg_shape];
   const BatchFns f = batch_fns(b->cfg_shape, b->riemann, b->order);
   const int R = b->rounds_per_graph;
-  if (!b->graph || b->graph_shape != b->cfg_shape) {
+  if (!b->graph || b->graph_shape != b->cfg_shape || b->graph_rounds != R) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     hipGraph_t g = nullptr;
@@ -549,6 +550,7 @@ ec_status run_graph(ec_batch* b, int* rounds_out)
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(b, e, "hipGraphInstantiate");
     b->graph_shape = b->cfg_shape;
+    b->graph_rounds = R;
   }
   // After an even number of rounds the latest run controls are in parity 0;
   // a member is finished when its latest round did not step it.
@@ -895,7 +897,12 @@ extern "C" ec_status ec_batch_autotune(ec_batch* b, ec_batch_timing* table, int 
   BT_TRY(b, hipMemcpyAsync(b->backup, b->ua, sb, hipMemcpyDeviceToDevice, b->stream));
   double best = INFINITY;
   int best_mode = EC_BATCH_GRAPH, best_shape = 0, n = 0;
-  for (int mode = EC_BATCH_GRAPH; mode <= EC_BATCH_PERSISTENT; ++mode) {
+  // Graph mode only: the persistent mode's per-member waits need agent-scope
+  // release/acquire fences, which write back and invalidate the XCD's L2 at
+  // every wait, and it measured 3 to 5 times slower than graph mode on every
+  // tile shape (64 Riemann strips, HLLC-P: 148-239 ms against 49-78 ms).  It
+  // stays selectable through ec_batch_configure.
+  for (int mode = EC_BATCH_GRAPH; mode <= EC_BATCH_GRAPH; ++mode) {
     for (int sh = 0; sh < kNumBatchShapes; ++sh) {
       BT_TRY(b, hipMemcpyAsync(b->ua, b->backup, sb, hipMemcpyDeviceToDevice, b->stream));
       double ms = 0.0;
@@ -910,6 +917,29 @@ extern "C" ec_status ec_batch_autotune(ec_batch* b, ec_batch_timing* table, int 
         best_shape = sh;
       }
     }
+  }
+  // Rounds per graph replay for the best shape: each replay ends in one host
+  // synchronisation, and a run ends up to R - 1 idle rounds late.
+  int best_r = b->rounds_per_graph;
+  for (int R : {32, 64, 128}) {
+    if (R == b->rounds_per_graph) continue;
+    const int keep = b->rounds_per_graph;
+    b->rounds_per_graph = R;
+    BT_TRY(b, hipMemcpyAsync(b->ua, b->backup, sb, hipMemcpyDeviceToDevice, b->stream));
+    double ms = 0.0;
+    st = run_once(b, best_mode, best_shape, &ms);
+    if (st != EC_OK) return st;
+    if (ms < best) {
+      best = ms;
+      best_r = R;
+    }
+    b->rounds_per_graph = keep;
+  }
+  if (best_r != b->rounds_per_graph) {
+    b->rounds_per_graph = best_r;
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    b->graph_shape = -1;
   }
   BT_TRY(b, hipMemcpyAsync(b->ua, b->backup, sb, hipMemcpyDeviceToDevice, b->stream));
   BT_TRY(b, hipStreamSynchronize(b->stream));

@@ -369,9 +369,11 @@ ec_status ec_batch_tile_shape(int tile_id, int* tx, int* ty, int* threads);
 /* mode: ec_batch_mode; tile_id -1 lets the autotuner choose; rounds_per_graph
  * (even, 0 keeps the current value) is the number of rounds per hipGraph. */
 ec_status ec_batch_configure(ec_batch* b, int mode, int tile_id, int rounds_per_graph);
-/* Times one complete run of every (mode, tile shape) pair on a copy of the
- * current states (which are restored) and keeps the fastest.  table may be
- * NULL; *n_entries receives the number of pairs timed. */
+/* Times one complete run of every graph-mode tile shape on a copy of the
+ * current states (which are restored) and keeps the fastest.  Persistent mode
+ * is not timed (it measured 3-5x slower on every shape: its per-member waits
+ * need device-scope fences); ec_batch_configure still selects it.  table may
+ * be NULL; *n_entries receives the number of pairs timed. */
 ec_status ec_batch_autotune(ec_batch* b, ec_batch_timing* table, int table_cap, int* n_entries);
 /* Runs every member from its current state at time 0 until stop_time,
  * max_step or a failure status; the final states replace the current ones.

@@ -7,7 +7,7 @@ strips of tests/golden/ensemble_64.json, O2 HLLC and HLLC-P.
   device loop (ec_explicit_advance) on that member alone, which the
   explicit tests pin to the oracle;
 * per-member statuses: a non-physical member stops alone;
-* the autotuner times every (mode, tile) pair and leaves the states as
+* the autotuner times every graph-mode tile shape and leaves the states as
   they were.
 """
 import json
@@ -135,7 +135,8 @@ def test_autotune_restores_states(gpu):
     ens = build(gpu, "xie_am_hllc_p", 0.01, 100000)
     before = [ens.get_state(k) for k in range(len(table()))]
     timings = ens.autotune()
-    assert len(timings) == 2 * len(tile_shapes()) and all(t.ms > 0 for t in timings)
+    assert len(timings) == len(tile_shapes()) and all(t.ms > 0 for t in timings)
+    assert all(t.mode == "graph" for t in timings)    # graph mode only
     for k in range(len(table())):
         assert np.array_equal(ens.get_state(k), before[k])
     res, used = ens.run()
