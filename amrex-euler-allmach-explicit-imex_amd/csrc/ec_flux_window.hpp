@@ -29,6 +29,16 @@
 
 namespace ec {
 
+// limited_slope with the minmod limiter (reconstruction_fluxes.hpp:2-12) for
+// finite operands: when a * b > 0 both have one sign, and the operand of
+// smaller magnitude (b on a tie, which then equals a) is that sign on
+// min(|a|, |b|).  Three instructions fewer than the ternaries.
+__device__ __forceinline__ double minmod_finite(double a, double b)
+{
+  const double m = copysign(fmin(fabs(a), fabs(b)), a);
+  return a * b <= 0.0 ? 0.0 : m;
+}
+
 struct FaceSide {
   double E;   // total energy of the face state
   double q2;  // u^2 + v^2

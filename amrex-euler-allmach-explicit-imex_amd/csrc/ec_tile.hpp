@@ -13,8 +13,8 @@
 //      y-faces, kept in registers and then written over the dead A..C storage;
 //   E. the conservative update, written to U_out, with an epilogue on U_out:
 //      max CFL spectral rate, min rho, min p, non-finite and degenerate-star
-//      counts, reduced per wavefront with cross-lane shuffles and per
-//      workgroup in LDS.
+//      counts, stored per thread in LDS beside the fluxes and reduced by
+//      wavefront 0 alone (four slots per lane, then one DPP tree).
 // Stage A also checks that every state the tile reads lies in the magnitude
 // window of ec_exact_ops.hpp.  When it does (always, for physical states of
 // moderate size), stages C and D use the window operators of
@@ -149,28 +149,83 @@ __device__ __forceinline__ Red wave_reduce(Red r)
   return r;
 }
 
+// The wavefront tree of rate, rho_min, p_min and a packed count k (result in
+// lane 63).  The quad swaps and mirrors give every lane a lane of its own
+// row, so their moves need no old value; the row broadcasts keep the old
+// value (the identity) in the rows they do not write.
+__device__ __forceinline__ void small_tree(double& rate, double& rmin, double& pmin, int& k)
+{
+  auto full = [&](auto ctrl_c) {
+    constexpr int CTRL = decltype(ctrl_c)::value;
+    rate = fmax(rate, dpp_f64_all<CTRL>(rate));
+    rmin = fmin(rmin, dpp_f64_all<CTRL>(rmin));
+    pmin = fmin(pmin, dpp_f64_all<CTRL>(pmin));
+    k += dpp_i32_all<CTRL>(k);
+  };
+  auto bcast = [&](auto ctrl_c, auto mask_c) {
+    constexpr int CTRL = decltype(ctrl_c)::value, MASK = decltype(mask_c)::value;
+    rate = fmax(rate, dpp_f64_id<CTRL, MASK>(rate, -INFINITY));
+    rmin = fmin(rmin, dpp_f64_id<CTRL, MASK>(rmin, INFINITY));
+    pmin = fmin(pmin, dpp_f64_id<CTRL, MASK>(pmin, INFINITY));
+    k += dpp_i32<CTRL, MASK>(k);
+  };
+  full(std::integral_constant<int, 0xB1>{});
+  full(std::integral_constant<int, 0x4E>{});
+  full(std::integral_constant<int, 0x141>{});
+  full(std::integral_constant<int, 0x140>{});
+  bcast(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{});
+  bcast(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{});
+}
+
 // wave_reduce for per-lane counts small enough to share one 32-bit word
 // (bad < 2^15 and deg < 2^16 summed over the wavefront, as in a tile's
 // epilogue): one integer move per step instead of two.
 __device__ __forceinline__ Red wave_reduce_small(Red r)
 {
   int k = (r.bad << 16) | r.deg;
-  auto step = [&](auto ctrl_c, auto mask_c) {
-    constexpr int CTRL = decltype(ctrl_c)::value, MASK = decltype(mask_c)::value;
-    r.rate = fmax(r.rate, dpp_f64_id<CTRL, MASK>(r.rate, -INFINITY));
-    r.rmin = fmin(r.rmin, dpp_f64_id<CTRL, MASK>(r.rmin, INFINITY));
-    r.pmin = fmin(r.pmin, dpp_f64_id<CTRL, MASK>(r.pmin, INFINITY));
-    k += dpp_i32<CTRL, MASK>(k);
-  };
-  step(std::integral_constant<int, 0xB1>{}, std::integral_constant<int, 0xF>{});
-  step(std::integral_constant<int, 0x4E>{}, std::integral_constant<int, 0xF>{});
-  step(std::integral_constant<int, 0x141>{}, std::integral_constant<int, 0xF>{});
-  step(std::integral_constant<int, 0x140>{}, std::integral_constant<int, 0xF>{});
-  step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{});
-  step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{});
+  small_tree(r.rate, r.rmin, r.pmin, k);
   r.bad = k >> 16;
   r.deg = k & 0xffff;
   return r;
+}
+
+// A tile epilogue's workgroup reduction through LDS: every thread stores its
+// values in `slots` (3 NT doubles, then NT packed counts bad << 16 | deg, with
+// bad < 2^15 and deg < 2^16 over the workgroup), wavefront 0 combines NT / 64
+// slots per lane and finishes with one wavefront tree.  The other wavefronts
+// do no reduction work.  The result is valid in every lane of wavefront 0;
+// the slots must stay untouched until wavefront 0 has returned.
+template <int NT>
+__device__ __forceinline__ Red tile_reduce(const Red& r, double* slots)
+{
+  static_assert(NT % 64 == 0, "whole wavefronts");
+  int* ks = reinterpret_cast<int*>(slots + 3 * NT);
+  const int t = threadIdx.x;
+  slots[t] = r.rate;
+  slots[NT + t] = r.rmin;
+  slots[2 * NT + t] = r.pmin;
+  ks[t] = (r.bad << 16) | r.deg;
+  __syncthreads();
+  Red out = r;
+  if (t < 64) {
+    double rate = slots[t], rmin = slots[NT + t], pmin = slots[2 * NT + t];
+    int k = ks[t];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) {
+      rate = fmax(rate, slots[64 * w + t]);
+      rmin = fmin(rmin, slots[NT + 64 * w + t]);
+      pmin = fmin(pmin, slots[2 * NT + 64 * w + t]);
+      k += ks[64 * w + t];
+    }
+    small_tree(rate, rmin, pmin, k);
+    out.rate = readlane_f64<63>(rate);
+    out.rmin = readlane_f64<63>(rmin);
+    out.pmin = readlane_f64<63>(pmin);
+    k = __builtin_amdgcn_readlane(k, 63);
+    out.bad = k >> 16;
+    out.deg = k & 0xffff;
+  }
+  return out;
 }
 
 // Workgroup reduction; result valid in thread 0.  SMALL: the per-lane counts
@@ -269,7 +324,10 @@ struct TileStep {
   static constexpr int OFF_FX = 0;
   static constexpr int OFF_FY = 4 * NXF;
   static constexpr int END2 = OFF_FY + 4 * NYF;
-  static constexpr int NLDS = END1 > END2 ? END1 : END2;
+  // the epilogue reduction's slots, beside the fluxes stage E reads
+  static constexpr int OFF_RED = END2;
+  static constexpr int END3 = OFF_RED + 3 * NT + (NT + 1) / 2;
+  static constexpr int NLDS = END1 > END3 ? END1 : END3;
 
   struct Smem {
     double lds[NLDS];
@@ -375,16 +433,39 @@ struct TileStep {
 
     // B. limited slopes (reconstruction_fluxes.hpp:2-12).
     if constexpr (O2) {
-      for (int t = tid; t < NS; t += NT) {
-        const int qc = (t / SX + 1) * RX + (t % SX + 1);
+      // each cell's 5-point stencil of the 4 variables is read before any
+      // slope is written, so the reads issue back to back
+      auto slopes = [&](auto fast_c) {
+        constexpr bool FAST = decltype(fast_c)::value;
+        for (int t = tid; t < NS; t += NT) {
+          const int qc = (t / SX + 1) * RX + (t % SX + 1);
+          double st[4][5];
   #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const double* q = lds + OFF_Q + c * NR;
-          const double vc = q[qc];
-          lds[OFF_SX + c * NS + t] = limited_slope(vc - q[qc - 1], q[qc + 1] - vc, g.limiter);
-          lds[OFF_SY + c * NS + t] = limited_slope(vc - q[qc - RX], q[qc + RX] - vc, g.limiter);
+          for (int c = 0; c < 4; ++c) {
+            const double* q = lds + OFF_Q + c * NR;
+            st[c][0] = q[qc];
+            st[c][1] = q[qc - 1];
+            st[c][2] = q[qc + 1];
+            st[c][3] = q[qc - RX];
+            st[c][4] = q[qc + RX];
+          }
+  #pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const double vc = st[c][0];
+            if constexpr (FAST) {
+              lds[OFF_SX + c * NS + t] = minmod_finite(vc - st[c][1], st[c][2] - vc);
+              lds[OFF_SY + c * NS + t] = minmod_finite(vc - st[c][3], st[c][4] - vc);
+            } else {
+              lds[OFF_SX + c * NS + t] = limited_slope(vc - st[c][1], st[c][2] - vc, g.limiter);
+              lds[OFF_SY + c * NS + t] = limited_slope(vc - st[c][3], st[c][4] - vc, g.limiter);
+            }
+          }
         }
-      }
+      };
+      // minmod on finite differences when every state of the tile is inside
+      // the window
+      if (win && g.limiter == 0) slopes(std::true_type{});
+      else slopes(std::false_type{});
       __syncthreads();
     }
 
@@ -414,11 +495,11 @@ struct TileStep {
     };
 
     // C. face pressure ratios for the HLLC-P sensor (reconstruction_fluxes.hpp:390-419).
-    // Inside the window both pressures are finite and positive, so the floor
-    // is a max, and the ratio min/max lies in (0, 1] already.
+    // Inside the window both face pressures lie between neighbouring cell
+    // pressures, so they are finite and at least 2^-40 > 1e-14: the floor is
+    // the identity, and the ratio min/max lies in (0, 1] already.
     auto ratio_w = [](double pl, double pr) {
-      const double a = fmax(pl, 1.0e-14), b = fmax(pr, 1.0e-14);
-      return quot(fmin(a, b), recip(fmax(a, b)));
+      return quot(fmin(pl, pr), recip(fmax(pl, pr)));
     };
     if constexpr (HP) {
       if (win) {
@@ -455,6 +536,7 @@ struct TileStep {
     const int lim_x = g.nx - i0, lim_y = g.ny - j0;  // cells of this tile inside the domain
     constexpr int NPX = (NXF + NT - 1) / NT, NPY = (NYF + NT - 1) / NT;
     // W: the window operators (every state of the tile inside the window).
+    const Recip rg = recip(gamma - 1.0);  // gamma - 1 lies in the window there
     auto face = [&](auto axis_c, auto w_c, int f, Cons& out, unsigned char& dflag) {
       constexpr int AX = decltype(axis_c)::value;
       constexpr bool W = decltype(w_c)::value;
@@ -498,7 +580,7 @@ struct TileStep {
         const Prim R = face_state(ri, rj, AX, -0.5);
         if constexpr (W) {
           bool degen;
-          out = interface_flux_w<AX, RIEMANN>(L, R, gamma, recip(gamma - 1.0), sensor, degen);
+          out = interface_flux_w<AX, RIEMANN>(L, R, gamma, rg, sensor, degen);
           dflag = degen ? 1 : 0;
         } else {
           out = interface_flux(L, R, AX, gamma, RIEMANN, sensor);
@@ -574,7 +656,7 @@ struct TileStep {
       }
     }
     if (!diag) return mine;
-    return block_reduce<NT, true>(mine, sm.red_scratch);
+    return tile_reduce<NT>(mine, lds + OFF_RED);
   }
 };
 

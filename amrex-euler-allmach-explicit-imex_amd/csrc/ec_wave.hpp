@@ -41,6 +41,33 @@ __device__ __forceinline__ int dpp_i32(int v)
   return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
 }
 
+// Moves under a control that gives every lane a lane of its own row (quad
+// permutes, row mirrors) with every lane active: no lane keeps an old value,
+// so none is supplied and no copy is made.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64_all(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int LANE>
+__device__ __forceinline__ double readlane_f64(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffLL), LANE);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), LANE);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32_all(int v)
+{
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+
 __device__ __forceinline__ double wave_sum_lane63(double v)
 {
   v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]: pairs
