@@ -6,10 +6,12 @@ never falls back: a missing library or a failing call raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 PKG_DIR = Path(__file__).resolve().parent.parent
-LIB_PATH = PKG_DIR / "libec_amd.so"
+# EC_AMD_LIB: another build of the same sources (the sanitizer build)
+LIB_PATH = Path(os.environ.get("EC_AMD_LIB", PKG_DIR / "libec_amd.so"))
 
 EC_OK = 0
 STATUS_TEXT = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of device memory",

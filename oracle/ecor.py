@@ -16,7 +16,8 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "libec_oracle.so"
+# EC_ORACLE_LIB: another build of the same sources (the sanitizer build)
+LIB_PATH = Path(os.environ.get("EC_ORACLE_LIB", HERE / "libec_oracle.so"))
 
 RUSANOV, HLLC, LOWMACH_HLLC, XIE_HLLCP = 0, 1, 2, 3
 MINMOD, MC, VANLEER = 0, 1, 2

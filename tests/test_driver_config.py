@@ -1,12 +1,15 @@
 """The drop-in driver's input checks (core/config_reader.hpp:118-322): each
 rejected configuration aborts with the reference's message before any device
 work, so these run on the CPU.  Skipped when the driver has not been built."""
+import os
 import subprocess
 from pathlib import Path
 
 import pytest
 
-EXE = Path(__file__).resolve().parent.parent / "amrex-euler-allmach-explicit-imex_amd" / "euler_compare_amd"
+# EC_AMD_DRIVER: another build of the driver (the sanitizer build)
+EXE = Path(os.environ.get("EC_AMD_DRIVER", Path(__file__).resolve().parent.parent
+                          / "amrex-euler-allmach-explicit-imex_amd" / "euler_compare_amd"))
 
 pytestmark = pytest.mark.skipif(not EXE.exists(), reason="driver not built")
 
