@@ -93,20 +93,113 @@ def cpu_baseline(n: int, mach: float, cfl: float, steps: int) -> dict:
 # tiles), from the committed rocprofv3 SQ counter summary.
 VALU_FILE = ROOT / "profiles" / "explicit_step_xie_hllcp_gresho2048_valu.json"
 
+# The sources the explicit step kernel is compiled from: a counter figure is
+# reported as current only when it was measured on these exact bytes.
+KERNEL_SOURCES = ("ec_explicit.hip", "ec_tile.hpp", "ec_flux_window.hpp", "ec_exact_ops.hpp",
+                  "ec_state.hpp", "ec_ghost.hpp", "ec_wave.hpp", "ec_context.hpp")
 
-def valu_bound(riemann: str, order: int, n: int, kernel_s: float):
+
+def kernel_sources_digest() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        h.update(name.encode() + b"\0" + (PKG / "csrc" / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def profile_figure(path: Path, key: str, digest: str):
+    """A per-launch counter figure from a committed profile summary, with the
+    sources digest and commit it was measured on.  A figure measured on other
+    sources is stale: its value is withheld (None) and reported beside."""
+    if not path.exists():
+        return None, None
+    try:
+        d = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None, None
+    prov = {"file": str(path.relative_to(ROOT)), "kernel_sources": d.get("kernel_sources"),
+            "commit": d.get("commit"), "current": d.get("kernel_sources") == digest}
+    if not prov["current"]:
+        prov["stale_value"] = d.get(key)
+        return None, prov
+    return d.get(key), prov
+
+
+def valu_bound(riemann: str, order: int, n: int, kernel_s: float, digest: str):
     """The FP64 VALU issue bound of the step kernel: counted VALU wave
     instructions x 4 cycles (a wave64 instruction on a 16-lane SIMD) over the
     1024 SIMDs at 2.4 GHz, against the measured kernel time."""
-    if not (VALU_FILE.exists() and n == 2048 and riemann == "xie_am_hllc_p" and order == 2):
+    if not (n == 2048 and riemann == "xie_am_hllc_p" and order == 2):
         return None
-    try:
-        d = json.loads(VALU_FILE.read_text())
-    except (OSError, ValueError):
+    valu, prov = profile_figure(VALU_FILE, "valu_wave_insts_per_launch", digest)
+    if prov is None:
         return None
-    issue_s = d["valu_wave_insts_per_launch"] * 4 / (1024 * 2.4e9)
-    return {"valu_wave_insts_per_launch": d["valu_wave_insts_per_launch"],
-            "issue_bound_us": issue_s * 1e6, "frac_of_issue_bound": issue_s / kernel_s}
+    if valu is None:
+        return {"valu_wave_insts_per_launch": None, "profile": prov}
+    issue_s = valu * 4 / (1024 * 2.4e9)
+    return {"valu_wave_insts_per_launch": valu, "issue_bound_us": issue_s * 1e6,
+            "frac_of_issue_bound": issue_s / kernel_s, "profile": prov}
+
+
+def _counter_means(run_dir: Path, kernel_substr: str) -> dict:
+    """Per-dispatch means of every counter of one rocprofv3 --pmc run over the
+    dispatches whose kernel name contains kernel_substr."""
+    import csv
+    files = sorted(run_dir.rglob("*counter_collection.csv"))
+    if not files:
+        raise FileNotFoundError(f"no counter_collection.csv under {run_dir}")
+    acc = {}
+    for r in csv.DictReader(open(files[0])):
+        if kernel_substr in r["Kernel_Name"]:
+            acc.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def summarize_counters(spec: list, commit: str) -> None:
+    """Writes the two profile summaries bench.py reads (VALU counts; HBM
+    traffic with the copy calibration) from rocprofv3 --pmc run directories:
+    spec holds sq=DIR (SQ_WAVES, SQ_INSTS_VALU, ...), fetch=DIR, write=DIR
+    (the step), copy_fetch=DIR, copy_write=DIR (bench.py --mode copy)."""
+    dirs = dict(item.split("=", 1) for item in spec)
+    digest = kernel_sources_digest()
+    kern = "explicit_step_kernel<16, 15, 256, 3, 2>"
+    sq, passes = {}, []
+    for i, d in enumerate(dirs["sq"].split(",")):   # one directory per counter pass
+        m = _counter_means(Path(d), kern)
+        sq.update(m)
+        passes.append(sorted(m))
+    wl = "gresho_vortex_2048x2048_periodic_M0.01_explicit_o2_minmod_xie_am_hllc_p_cfl0.4"
+    base = "python3 bench.py --steps 10 --warmup 2 --no-cpu-baseline --no-extras"
+    valu = {"kernel": "explicit_step_kernel<16,15,256,3,2>", "workload": wl,
+            "kernel_sources": digest, "commit": commit,
+            "valu_wave_insts_per_launch": round(sq["SQ_INSTS_VALU"]),
+            "waves_per_launch": round(sq["SQ_WAVES"]),
+            "counters_per_dispatch": {k: round(v, 1) for k, v in sorted(sq.items())},
+            "commands": [f"rocprofv3 --pmc {' '.join(c)} -d gpurun_out/pmc_sq{i} -o p "
+                         f"--output-format csv -- {base}" for i, c in enumerate(passes)]}
+    VALU_FILE.write_text(json.dumps(valu, indent=2) + "\n")
+    fetch = _counter_means(Path(dirs["fetch"]), kern)["FETCH_SIZE"]
+    write = _counter_means(Path(dirs["write"]), kern)["WRITE_SIZE"]
+    cf = _counter_means(Path(dirs["copy_fetch"]), "copyBuffer")["FETCH_SIZE"]
+    cw = _counter_means(Path(dirs["copy_write"]), "copyBuffer")["WRITE_SIZE"]
+    copy_kib = 4 * 2048 * 2048 * 8 / 1024
+    ff, wf = copy_kib / cf, copy_kib / cw
+    traffic = {"kernel": valu["kernel"], "workload": wl, "kernel_sources": digest, "commit": commit,
+               "fetch_size_kib_per_dispatch": round(fetch, 1),
+               "write_size_kib_per_dispatch": round(write, 1),
+               "fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
+               "calibration": f"device copy of {int(copy_kib * 1024)} B: FETCH_SIZE {cf:.1f} KiB, "
+                              f"WRITE_SIZE {cw:.1f} KiB per dispatch",
+               "hbm_bytes_per_launch": round((fetch * ff + write * wf) * 1024),
+               "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE * 2048 * 2048,
+               "commands": [f"rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o p --output-format csv -- {base}",
+                            f"rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o p --output-format csv -- {base}",
+                            "rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_copy_fetch -o p --output-format csv "
+                            "-- python3 bench.py --mode copy --steps 5",
+                            "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_copy_write -o p --output-format csv "
+                            "-- python3 bench.py --mode copy --steps 5"]}
+    TRAFFIC_FILE.write_text(json.dumps(traffic, indent=2) + "\n")
+    print(json.dumps({"valu": str(VALU_FILE), "traffic": str(TRAFFIC_FILE), "kernel_sources": digest}))
 
 
 def copy_calibration(dev, n: int, steps: int) -> int:
@@ -511,8 +604,15 @@ def main() -> int:
                     help="print only that measurement")
     ap.add_argument("--ensemble-mode", choices=("auto", "graph", "persistent"), default="auto",
                     help="ensemble rounds: autotuned, or one mode with tile shape 0")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="KIND=DIR",
+                    help="write the profile summaries from rocprofv3 --pmc run directories "
+                         "(sq=, fetch=, write=, copy_fetch=, copy_write=) and exit")
+    ap.add_argument("--commit", default=None, help="commit the summarized counters were measured on")
     args = ap.parse_args()
 
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters, args.commit)
+        return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
         print(json.dumps({"error": "this build runs the euler_compare path on one GPU; "
@@ -581,12 +681,10 @@ def main() -> int:
     kernel_s = k_ms / 1e3 / max(k_n, 1)
     achieved = BYTES_PER_CELL_UPDATE * cells / kernel_s / 1e9
     copy_gbs = stream_copy_gbs(dev)
-    traffic = None
-    if TRAFFIC_FILE.exists() and n == 2048 and args.riemann == "xie_am_hllc_p" and args.order == 2:
-        try:
-            traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
+    digest = kernel_sources_digest()
+    traffic, traffic_prov = None, None
+    if n == 2048 and args.riemann == "xie_am_hllc_p" and args.order == 2:
+        traffic, traffic_prov = profile_figure(TRAFFIC_FILE, "hbm_bytes_per_launch", digest)
     line = {
         "metric": METRIC,
         "value": value,
@@ -605,11 +703,12 @@ def main() -> int:
                    "nx": n, "ny": n, "parallelism": "single_gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_profile": traffic_prov, "kernel_sources": digest,
                      "kernel": f"explicit_step_kernel ({args.riemann}, O{args.order})",
                      "kernel_us_avg": kernel_s * 1e6, "kernel_launches_timed": k_n,
                      "bytes_per_cell_update": BYTES_PER_CELL_UPDATE,
                      "bytes_per_launch": BYTES_PER_CELL_UPDATE * cells,
-                     "fp64_valu": valu_bound(args.riemann, args.order, n, kernel_s),
+                     "fp64_valu": valu_bound(args.riemann, args.order, n, kernel_s, digest),
                      "stream_copy": {"achieved": copy_gbs, "unit": "GB/s",
                                      "frac_of_peak": copy_gbs / HBM_PEAK_GBS,
                                      "kernel_frac_of_copy": achieved / copy_gbs}},
