@@ -58,6 +58,7 @@ struct TileInfo {
 struct BatchArgs {
   const MemberDev* members;
   const TileInfo* tiles;   // per global tile
+  const TileInfo* round_tiles;  // graph mode: the tile of each workgroup of one group's launch
   const int* tile_member;  // global tile -> member
   double* ua;
   double* ub;
@@ -173,14 +174,15 @@ __global__ void __launch_bounds__(NT) batch_init_kernel(BatchArgs A)
   store_partial(A.part + A.ntiles, t, r);
 }
 
-// Graph mode: one launch = one round; workgroup b owns global tile b.
+// Graph mode: one launch = one round of one member group; workgroup b owns
+// the tile round_tiles[b].
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
 __global__ void __launch_bounds__(NT) batch_round_kernel(BatchArgs A, int p)
 {
   using T = TileStep<TX, TY, NT, RIEMANN, ORDER>;
   __shared__ typename T::Smem sm;
   __shared__ RoundShared rsh;
-  const TileInfo& ti = A.tiles[blockIdx.x];
+  const TileInfo& ti = A.round_tiles[blockIdx.x];
   (void)member_round<TX, TY, NT, RIEMANN, ORDER>(sm, rsh, A, ti.md, ti.m, ti.lt, ti.lt + 1, p);
 }
 
@@ -335,6 +337,8 @@ static BatchFns batch_fns(int shape, int riemann, int order)
 
 using namespace ec;
 
+constexpr int kMaxGroups = 4;
+
 struct ec_batch {
   int device = 0;
   hipStream_t user_stream = nullptr;
@@ -366,9 +370,20 @@ struct ec_batch {
   int rounds_per_graph = 32;
   int cfg_shape = -1;        // shape the device tables hold
   int ntiles = 0;
-  hipGraphExec_t graph = nullptr;
   int graph_shape = -1;
   int graph_rounds = 0;      // rounds the instantiated graph holds
+  // Graph mode splits the members into groups (balanced by tile count), each
+  // group's rounds captured in its own graph and replayed on its own stream:
+  // the groups' rounds overlap (one group's loads with another's arithmetic,
+  // one group's launch gap with another's round).  gstream[0] is `stream`.
+  int groups = 2;
+  int cfg_groups = 0;        // groups the device tables hold
+  int graph_groups = 0;      // groups the instantiated graphs hold
+  hipStream_t gstream[kMaxGroups] = {};
+  hipEvent_t gev[kMaxGroups] = {};
+  TileInfo* gtiles[kMaxGroups] = {};
+  int gblocks[kMaxGroups] = {};
+  hipGraphExec_t ggraph[kMaxGroups] = {};
   int last_mode = 0, last_shape = 0, last_rounds = 0;
   double last_ms = 0.0;
   unsigned spin_limit = 1u << 20;
@@ -402,6 +417,15 @@ bool member_ok(const ec_member_desc& d)
   return true;
 }
 
+void drop_graphs(ec_batch* b)
+{
+  for (auto& g : b->ggraph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  b->graph_shape = -1;
+}
+
 int tiles_of(const ec_member_desc& d, int shape, int* tx_out)
 {
   const BatchShape s = kBatchShapes[shape];
@@ -413,7 +437,7 @@ int tiles_of(const ec_member_desc& d, int shape, int* tx_out)
 // Device tables (member descriptors, tile -> member) for one tile shape.
 ec_status configure_tables(ec_batch* b, int shape)
 {
-  if (b->cfg_shape == shape) return EC_OK;
+  if (b->cfg_shape == shape && b->cfg_groups == b->groups) return EC_OK;
   std::vector<MemberDev> md(b->nm);
   std::vector<int> tm;
   for (int m = 0; m < b->nm; ++m) {
@@ -457,6 +481,35 @@ ec_status configure_tables(ec_batch* b, int shape)
     ti[t].m = tm[t];
     ti[t].lt = t - md[tm[t]].tile0;
   }
+  // Member groups of the graph mode: largest member first, to the group with
+  // the fewest tiles; each group's workgroups take its tiles member by member.
+  const int G = std::min(b->groups, b->nm);
+  std::vector<int> order(b->nm);
+  for (int m = 0; m < b->nm; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return md[x].ntiles > md[y].ntiles; });
+  std::vector<std::vector<int>> members_of(G);
+  std::vector<int> load(G, 0);
+  for (int m : order) {
+    const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    members_of[g].push_back(m);
+    load[g] += md[m].ntiles;
+  }
+  for (int g = 0; g < kMaxGroups; ++g) {
+    if (b->gtiles[g]) (void)hipFree(b->gtiles[g]);
+    b->gtiles[g] = nullptr;
+    b->gblocks[g] = 0;
+  }
+  for (int g = 0; g < G; ++g) {
+    std::sort(members_of[g].begin(), members_of[g].end());
+    std::vector<TileInfo> gt;
+    for (int m : members_of[g])
+      for (int t = 0; t < md[m].ntiles; ++t) gt.push_back(ti[md[m].tile0 + t]);
+    BT_TRY(b, hipMalloc(&b->gtiles[g], sizeof(TileInfo) * gt.size()));
+    BT_TRY(b, hipMemcpyAsync(b->gtiles[g], gt.data(), sizeof(TileInfo) * gt.size(),
+                             hipMemcpyHostToDevice, b->stream));
+    b->gblocks[g] = (int)gt.size();
+  }
   BT_TRY(b, hipMemcpyAsync(b->members, md.data(), sizeof(MemberDev) * (size_t)b->nm,
                            hipMemcpyHostToDevice, b->stream));
   BT_TRY(b, hipMemcpyAsync(b->tile_member, tm.data(), sizeof(int) * (size_t)nt,
@@ -466,11 +519,8 @@ ec_status configure_tables(ec_batch* b, int shape)
   BT_TRY(b, hipStreamSynchronize(b->stream));
   b->ntiles = nt;
   b->cfg_shape = shape;
-  if (b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-    b->graph_shape = -1;
-  }
+  b->cfg_groups = b->groups;
+  drop_graphs(b);
   return EC_OK;
 }
 
@@ -479,6 +529,7 @@ BatchArgs make_args(ec_batch* b)
   BatchArgs a;
   a.members = b->members;
   a.tiles = b->tiles;
+  a.round_tiles = b->gtiles[0];
   a.tile_member = b->tile_member;
   a.ua = b->ua;
   a.ub = b->ub;
@@ -529,35 +580,50 @@ ec_status run_graph(ec_batch* b, int* rounds_out)
   const BatchShape s = kBatchShapes[b->cfg_shape];
   const BatchFns f = batch_fns(b->cfg_shape, b->riemann, b->order);
   const int R = b->rounds_per_graph;
-  if (!b->graph || b->graph_shape != b->cfg_shape || b->graph_rounds != R) {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-    hipGraph_t g = nullptr;
-    BatchArgs a = make_args(b);
-    BT_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < R; ++k) {
-      int p = k & 1;
-      void* args[] = {&a, &p};
-      const hipError_t e = hipLaunchKernel(f.round, dim3(b->ntiles), dim3(s.nt), args, 0, b->stream);
-      if (e != hipSuccess) {
-        (void)hipStreamEndCapture(b->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        return fail(b, e, "hipLaunchKernel(batch_round_kernel) under capture");
+  if (b->graph_shape != b->cfg_shape || b->graph_rounds != R || b->graph_groups != b->cfg_groups) {
+    drop_graphs(b);
+    for (int gi = 0; gi < kMaxGroups; ++gi) {
+      if (b->gblocks[gi] == 0) continue;
+      hipStream_t st = b->gstream[gi];
+      hipGraph_t g = nullptr;
+      BatchArgs a = make_args(b);
+      a.round_tiles = b->gtiles[gi];
+      BT_TRY(b, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < R; ++k) {
+        int p = k & 1;
+        void* args[] = {&a, &p};
+        const hipError_t e = hipLaunchKernel(f.round, dim3(b->gblocks[gi]), dim3(s.nt), args, 0, st);
+        if (e != hipSuccess) {
+          (void)hipStreamEndCapture(st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          return fail(b, e, "hipLaunchKernel(batch_round_kernel) under capture");
+        }
       }
+      BT_TRY(b, hipStreamEndCapture(st, &g));
+      const hipError_t e = hipGraphInstantiate(&b->ggraph[gi], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(b, e, "hipGraphInstantiate");
     }
-    BT_TRY(b, hipStreamEndCapture(b->stream, &g));
-    const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(b, e, "hipGraphInstantiate");
     b->graph_shape = b->cfg_shape;
     b->graph_rounds = R;
+    b->graph_groups = b->cfg_groups;
   }
   // After an even number of rounds the latest run controls are in parity 0;
   // a member is finished when its latest round did not step it.
   const long long bound = round_bound(b);
   long long rounds = 0;
   for (;;) {
-    BT_TRY(b, hipGraphLaunch(b->graph, b->stream));
+    // the groups' streams start after `stream` and `stream` waits for them
+    BT_TRY(b, hipEventRecord(b->gev[0], b->stream));
+    for (int gi = 1; gi < kMaxGroups; ++gi)
+      if (b->ggraph[gi]) BT_TRY(b, hipStreamWaitEvent(b->gstream[gi], b->gev[0], 0));
+    for (int gi = 0; gi < kMaxGroups; ++gi)
+      if (b->ggraph[gi]) BT_TRY(b, hipGraphLaunch(b->ggraph[gi], b->gstream[gi]));
+    for (int gi = 1; gi < kMaxGroups; ++gi) {
+      if (!b->ggraph[gi]) continue;
+      BT_TRY(b, hipEventRecord(b->gev[gi], b->gstream[gi]));
+      BT_TRY(b, hipStreamWaitEvent(b->stream, b->gev[gi], 0));
+    }
     rounds += R;
     BT_TRY(b, hipMemcpyAsync(b->ctrl_host, b->ctrl, sizeof(DevCtrl) * (size_t)b->nm,
                              hipMemcpyDeviceToHost, b->stream));
@@ -735,6 +801,12 @@ extern "C" ec_status ec_batch_create(const ec_member_desc* members, int n_member
       hipEventCreate(&b->ev_b) != hipSuccess) {
     return bad(EC_ERR_HIP);
   }
+  b->gstream[0] = b->stream;
+  for (int g = 0; g < kMaxGroups; ++g) {
+    if (g > 0 && hipStreamCreateWithFlags(&b->gstream[g], hipStreamNonBlocking) != hipSuccess)
+      return bad(EC_ERR_HIP);
+    if (hipEventCreateWithFlags(&b->gev[g], hipEventDisableTiming) != hipSuccess) return bad(EC_ERR_HIP);
+  }
   const size_t sb = sizeof(double) * (size_t)tot;
   if (hipMalloc(&b->ua, sb) != hipSuccess || hipMalloc(&b->ub, sb) != hipSuccess ||
       hipMalloc(&b->ctrl, sizeof(DevCtrl) * 2 * (size_t)n_members) != hipSuccess ||
@@ -761,7 +833,12 @@ extern "C" ec_status ec_batch_destroy(ec_batch* b)
   if (!b) return EC_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  drop_graphs(b);
+  for (int g = 0; g < kMaxGroups; ++g) {
+    if (b->gtiles[g]) (void)hipFree(b->gtiles[g]);
+    if (b->gev[g]) (void)hipEventDestroy(b->gev[g]);
+    if (g > 0 && b->gstream[g]) (void)hipStreamDestroy(b->gstream[g]);
+  }
   double* bufs[] = {b->ua, b->ub, b->backup};
   for (double* p : bufs)
     if (p) (void)hipFree(p);
@@ -879,12 +956,19 @@ extern "C" ec_status ec_batch_configure(ec_batch* b, int mode, int tile_id, int 
   b->shape = tile_id;
   if (rounds_per_graph > 0 && rounds_per_graph != b->rounds_per_graph) {
     b->rounds_per_graph = rounds_per_graph;
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-    b->graph_shape = -1;
+    drop_graphs(b);
   }
   return EC_OK;
 }
+
+extern "C" ec_status ec_batch_set_groups(ec_batch* b, int groups)
+{
+  if (!b || groups < 1 || groups > kMaxGroups) return EC_ERR_INVALID_ARGUMENT;
+  b->groups = groups;   // the tables and graphs follow on the next run
+  return EC_OK;
+}
+
+extern "C" int ec_batch_groups(const ec_batch* b) { return b ? b->groups : 0; }
 
 extern "C" ec_status ec_batch_autotune(ec_batch* b, ec_batch_timing* table, int table_cap,
                                        int* n_entries)
@@ -937,10 +1021,25 @@ extern "C" ec_status ec_batch_autotune(ec_batch* b, ec_batch_timing* table, int 
   }
   if (best_r != b->rounds_per_graph) {
     b->rounds_per_graph = best_r;
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-    b->graph_shape = -1;
+    drop_graphs(b);
   }
+  // Member groups (streams) for the best shape and rounds per replay.
+  int best_g = b->groups;
+  for (int G = 1; G <= kMaxGroups; ++G) {
+    if (G == b->groups) continue;
+    const int keep = b->groups;
+    b->groups = G;
+    BT_TRY(b, hipMemcpyAsync(b->ua, b->backup, sb, hipMemcpyDeviceToDevice, b->stream));
+    double ms = 0.0;
+    st = run_once(b, best_mode, best_shape, &ms);
+    if (st != EC_OK) return st;
+    if (ms < best) {
+      best = ms;
+      best_g = G;
+    }
+    b->groups = keep;
+  }
+  b->groups = best_g;
   BT_TRY(b, hipMemcpyAsync(b->ua, b->backup, sb, hipMemcpyDeviceToDevice, b->stream));
   BT_TRY(b, hipStreamSynchronize(b->stream));
   b->mode = best_mode;

@@ -134,6 +134,60 @@ EC_HD Cons fixed_state(const GridParams& g, int side)
   return Cons{g.fixed[side][0], g.fixed[side][1], g.fixed[side][2], g.fixed[side][3]};
 }
 
+// load_cell in two phases, so that a tile can issue all of its loads before
+// it converts any cell: cell_src says which valid cell the ghost rule reads
+// (with the momentum flips) or that the value is not read at all (an exact or
+// fixed side); cell_value then forms the state from what was loaded.
+struct CellSrc {
+  long long k;   // offset of the valid cell read (kind 0)
+  int kind;      // 0 read, 1 fixed side `side`, 2 exact
+  int side;
+  bool rx, ry;   // normal momentum flips of a reflecting side
+};
+
+EC_HD CellSrc cell_src(const GridParams& g, long long rs, int i, int j)
+{
+  CellSrc c{0, 0, 0, false, false};
+  if (g.exact && exact_cell(g, i, j)) {
+    c.kind = 2;
+    return c;
+  }
+  const int side = (i < 0 && g.bc[0] == BC_FIXED)       ? 0
+                   : (i >= g.nx && g.bc[1] == BC_FIXED) ? 1
+                   : (j < 0 && g.bc[2] == BC_FIXED)     ? 2
+                   : (j >= g.ny && g.bc[3] == BC_FIXED) ? 3
+                                                        : -1;
+  if (side >= 0) {
+    c.kind = 1;
+    c.side = side;
+    return c;
+  }
+  const int ii = map_axis(i, g.nx, g.bc[0], g.bc[1], c.rx);
+  const int jj = map_axis(j, g.ny, g.bc[2], g.bc[3], c.ry);
+  c.k = (long long)jj * rs + ii;
+  return c;
+}
+
+// The loaded values of a kind-0 source (unused otherwise).
+EC_HD Cons cell_read(const double* __restrict__ u, long long cs, const CellSrc& c)
+{
+  if (c.kind != 0) return Cons{0.0, 0.0, 0.0, 0.0};
+  return Cons{u[c.k], u[cs + c.k], u[2 * cs + c.k], u[3 * cs + c.k]};
+}
+
+EC_HD Cons cell_value(const CellSrc& c, Cons v, const GridParams& g, int i, int j,
+                      const double* time_src)
+{
+  if (c.kind == 2) {
+    const double t = time_src ? *time_src : (g.time_ptr ? *g.time_ptr : g.time);
+    return exact_state(*g.exact, t, i, j);
+  }
+  if (c.kind == 1) return fixed_state(g, c.side);
+  if (c.rx) v.mx = -v.mx;
+  if (c.ry) v.my = -v.my;
+  return v;
+}
+
 // time_src: where exact ghost states read the step's time, if not from g
 // (the ensembles keep each member's step time in LDS).
 EC_HD Cons load_cell(const double* __restrict__ u, long long rs, long long cs,

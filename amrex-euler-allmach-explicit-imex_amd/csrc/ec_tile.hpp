@@ -412,14 +412,30 @@ struct TileStep {
         }
       }
     } else {
-      for (int t = tid; t < NR; t += NT) {
-        const int li = t % RX, lj = t / RX;
-        const Prim q = to_prim(load_cell(uin, rs, cs, g, i0 + li - 2, j0 + lj - 2, time_src), gamma);
-        lds[OFF_Q + t] = q.r;
-        lds[OFF_Q + NR + t] = q.u;
-        lds[OFF_Q + 2 * NR + t] = q.v;
-        lds[OFF_Q + 3 * NR + t] = q.p;
-        ok = ok & pos_in_window(q.r) & pos_in_window(q.p) & vel_in_window(q.u) & vel_in_window(q.v);
+      // through the ghost rule: every load first, then the values
+      constexpr int NA = (NR + NT - 1) / NT;
+      CellSrc src[NA];
+      Cons raw[NA];
+  #pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const int t = tid + a * NT;
+        if (t < NR) {
+          src[a] = cell_src(g, rs, i0 + t % RX - 2, j0 + t / RX - 2);
+          raw[a] = cell_read(uin, cs, src[a]);
+        }
+      }
+  #pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const int t = tid + a * NT;
+        if (t < NR) {
+          const Cons c = cell_value(src[a], raw[a], g, i0 + t % RX - 2, j0 + t / RX - 2, time_src);
+          const Prim q = to_prim(c, gamma);
+          lds[OFF_Q + t] = q.r;
+          lds[OFF_Q + NR + t] = q.u;
+          lds[OFF_Q + 2 * NR + t] = q.v;
+          lds[OFF_Q + 3 * NR + t] = q.p;
+          ok = ok & pos_in_window(q.r) & pos_in_window(q.p) & vel_in_window(q.u) & vel_in_window(q.v);
+        }
       }
     }
     {

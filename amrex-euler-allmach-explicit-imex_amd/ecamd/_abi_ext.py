@@ -164,6 +164,10 @@ def declare(L) -> None:
     L.ec_batch_tile_shape.restype = st
     L.ec_batch_configure.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.ec_batch_configure.restype = st
+    L.ec_batch_set_groups.argtypes = [vp, C.c_int]
+    L.ec_batch_set_groups.restype = st
+    L.ec_batch_groups.argtypes = [vp]
+    L.ec_batch_groups.restype = C.c_int
     L.ec_batch_autotune.argtypes = [vp, C.POINTER(BatchTiming), C.c_int, ip]
     L.ec_batch_autotune.restype = st
     L.ec_batch_run.argtypes = [vp, C.POINTER(RunState), C.POINTER(BatchTiming)]

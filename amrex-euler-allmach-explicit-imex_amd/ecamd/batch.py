@@ -138,6 +138,14 @@ class Ensemble:
         self._check(lib().ec_batch_configure(self._b, MODES[mode], tile_id, rounds_per_graph),
                     "ec_batch_configure")
 
+    def set_groups(self, groups: int) -> None:
+        """Graph mode: members split into `groups` streams (1..4)."""
+        self._check(lib().ec_batch_set_groups(self._b, groups), "ec_batch_set_groups")
+
+    @property
+    def groups(self) -> int:
+        return lib().ec_batch_groups(self._b)
+
     def autotune(self) -> list[Timing]:
         n = 2 * lib().ec_batch_tile_count()
         table = (BatchTiming * n)()

@@ -572,10 +572,11 @@ def bench_ensemble(dev, riemann: str, stop_time: float = 0.1, cfl: float = 0.8,
     res, used = ens.run()
     cell_updates = sum(r.step * m["nx"] * 4 for r, m in zip(res, members))
     shapes = tile_shapes()
+    groups = ens.groups
     out = {"workload": f"riemann_strips_64_nx100-1600x4_o2_minmod_{riemann}_cfl{cfl:g}_t{stop_time:g}",
            "members": len(members), "cell_updates": cell_updates,
            "max_member_steps": max(r.step for r in res), "best": {"mode": used.mode,
-           "tile": shapes[used.tile_id], "ms": used.ms},
+           "tile": shapes[used.tile_id], "ms": used.ms, "stream_groups": groups},
            "cell_updates_per_s": cell_updates / (used.ms / 1e3),
            "autotune": [{"mode": t.mode, "tile": shapes[t.tile_id], "ms": t.ms} for t in tuned],
            "statuses_ok": all(r.status == 0 for r in res)}

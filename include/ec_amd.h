@@ -369,6 +369,12 @@ ec_status ec_batch_tile_shape(int tile_id, int* tx, int* ty, int* threads);
 /* mode: ec_batch_mode; tile_id -1 lets the autotuner choose; rounds_per_graph
  * (even, 0 keeps the current value) is the number of rounds per hipGraph. */
 ec_status ec_batch_configure(ec_batch* b, int mode, int tile_id, int rounds_per_graph);
+/* Graph mode: the members split into `groups` (1..4, balanced by tile count),
+ * each group's rounds replayed on a stream of its own so that the groups'
+ * rounds overlap; the autotuner picks the count (no reference counterpart:
+ * the reference runs one grid at a time). */
+ec_status ec_batch_set_groups(ec_batch* b, int groups);
+int ec_batch_groups(const ec_batch* b);
 /* Times one complete run of every graph-mode tile shape on a copy of the
  * current states (which are restored) and keeps the fastest.  Persistent mode
  * is not timed (it measured 3-5x slower on every shape: its per-member waits
