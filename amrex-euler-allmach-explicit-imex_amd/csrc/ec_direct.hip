@@ -563,7 +563,8 @@ extern "C" ec_status ec_direct_step(ec_ctx* ctx, double* U, const ec_layout* lay
   }
   if (opts->picard_iterations < 1 || opts->restart < 1 || opts->restart > 64 || !(opts->tol > 0.0) ||
       opts->max_iter < 0 ||
-      (opts->precision != EC_IMEX_PRECISION_FP64 && opts->precision != EC_IMEX_PRECISION_MIXED) ||
+      (opts->precision != EC_IMEX_PRECISION_FP64 && opts->precision != EC_IMEX_PRECISION_MIXED &&
+       opts->precision != EC_IMEX_PRECISION_FP64_FLEXIBLE) ||
       !(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(time)) {
     return EC_ERR_INVALID_ARGUMENT;
   }

@@ -87,6 +87,21 @@ std::string lower(std::string s)
   return s;
 }
 
+// euler.imex_gmres_precision: fp64 (default), mixed (FP32 inner solves),
+// fp64_flexible (the mixed form's structure in FP64)
+int gmres_precision(const std::string& name)
+{
+  const std::string v = lower(name);
+  if (v == "mixed") return EC_IMEX_PRECISION_MIXED;
+  if (v == "fp64_flexible") return EC_IMEX_PRECISION_FP64_FLEXIBLE;
+  return EC_IMEX_PRECISION_FP64;
+}
+
+const char* gmres_precision_name(int p)
+{
+  return p == EC_IMEX_PRECISION_MIXED ? "mixed" : p == EC_IMEX_PRECISION_FP64_FLEXIBLE ? "fp64_flexible" : "fp64";
+}
+
 // euler.bdltv20_paper_t1_s2 -> ec_direct_case, -1 if unknown
 int direct_case(const std::string& m)
 {
@@ -782,7 +797,7 @@ int run_direct(const Cfg& cfg, const ec_problem_desc& pd)
   io.tol = cfg.imex_solver_tol;
   io.max_iter = cfg.imex_solver_max_iter;
   io.restart = cfg.imex_gmres_restart;
-  io.precision = lower(cfg.imex_gmres_precision) == "mixed" ? EC_IMEX_PRECISION_MIXED : EC_IMEX_PRECISION_FP64;
+  io.precision = gmres_precision(cfg.imex_gmres_precision);
 
   int steps = 0, solver_failures = 0, nonfinite = 0, nonpositive = 0, guards = 0, its_max = 0;
   double res_max = 0.0, rel_max = 0.0, time = 0.0;
@@ -924,7 +939,7 @@ int run_direct(const Cfg& cfg, const ec_problem_desc& pd)
   std::cout << "bdltv20_paper_final_csv_written=" << (csv_ok ? 1 : 0) << "\n";
   std::cout << "bdltv20_paper_nonclaims="
                "no_full_3d_no_second_order_time_no_ap_proof_no_production_no_gpu_cpu_efficiency\n";
-  std::cout << "bdltv20_paper_gmres_precision=" << (io.precision ? "mixed" : "fp64") << "\n";
+  std::cout << "bdltv20_paper_gmres_precision=" << gmres_precision_name(io.precision) << "\n";
   std::cout << "bdltv20_paper_wall_time_sec=" << wall << "\n";
   std::cout << "bdltv20_paper_t1_s2_status=" << (passed ? "passed" : "failed") << "\n";
   return passed ? 0 : 2;
@@ -1110,7 +1125,7 @@ int main(int argc, char** argv)
   io.tol = cfg.imex_solver_tol;
   io.max_iter = cfg.imex_solver_max_iter;
   io.restart = cfg.imex_gmres_restart;
-  io.precision = lower(cfg.imex_gmres_precision) == "mixed" ? EC_IMEX_PRECISION_MIXED : EC_IMEX_PRECISION_FP64;
+  io.precision = gmres_precision(cfg.imex_gmres_precision);
 
   // the cylindrical source runs between updates: host-dt loop
   const bool device_loop = cfg.method == "explicit" && snaps.empty() && !cfg.cylindrical;
@@ -1420,7 +1435,7 @@ int main(int argc, char** argv)
     std::cout << "imex_trial_pressure_min=" << last_imex.trial_pressure_min << "\n";
     std::cout << "imex_trial_internal_energy_min=" << last_imex.trial_internal_energy_min << "\n";
     std::cout << "imex_nonfinite_count_last=" << last_imex.nonfinite_count << "\n";
-    std::cout << "imex_gmres_precision=" << (io.precision ? "mixed" : "fp64") << "\n";
+    std::cout << "imex_gmres_precision=" << gmres_precision_name(io.precision) << "\n";
   }
   // app/driver_main.hpp:550-571 and :609-620: the problem metrics of the final state
   if (cfg.problem_kind == EC_PROBLEM_GRESHO) {

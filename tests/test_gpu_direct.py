@@ -89,7 +89,7 @@ def close(a, b, rel):
     ("isentropic_vortex_periodic_2d", 1.0, 24, 24, (1.0, 1.0), 2, None),
     ("advection_blob_periodic_2d", 0.5, 24, 16, (1.0, 1.0), 2, None),
 ])
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp64_flexible"])
 def test_direct_steps_match_oracle(ecor, ecamd, gpu, case, eps, nx, ny, hi, steps, dtf, precision):
     s, g, d, U = setup(ecor, ecamd, case, eps, nx, ny, hi)
     Ud = torch.from_numpy(U.copy()).to(gpu)
@@ -110,7 +110,7 @@ def test_direct_steps_match_oracle(ecor, ecamd, gpu, case, eps, nx, ny, hi, step
         assert dd.nonfinite_count == r.nonfinite_count
         assert dd.nonpositive_count == r.nonpositive_count
         # guards test |m_l + m_r| <= 1e-12 max(1, |m_l| + |m_r|): at rest the
-        # mixed solve's last digits can move a face across the threshold
+        # flexible solves' last digits can move a face across the threshold
         if precision == "fp64":
             assert dd.guard_count == r.guard_count
         else:
