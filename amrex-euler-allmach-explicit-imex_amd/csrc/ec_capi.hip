@@ -96,6 +96,8 @@ extern "C" ec_status ec_create(const ec_grid_desc* desc, void* stream, ec_ctx** 
     const int n = step_partials_needed(ctx, t);
     if (n > need) need = n;
   }
+  // room for the second-level partials of a large grid's reduction
+  need += kPartialGroupsMax;
   ctx->max_partials = need;
   const size_t state_bytes = sizeof(double) * 4 * (size_t)desc->nx * desc->ny;
   if (hipMalloc(&ctx->ctrl, sizeof(DevCtrl)) != hipSuccess ||

@@ -50,6 +50,12 @@ struct DevCtrl {
 };
 
 // Per-workgroup partial of the fused epilogue reductions.
+// Partials of a step above kTwoLevelMin tiles are reduced in two levels, by
+// at most kPartialGroupsMax workgroups first; workspaces hold that many more
+// slots after the tile partials.
+constexpr int kTwoLevelMin = 1024;
+constexpr int kPartialGroupsMax = 128;
+
 struct Partial {
   double rate, rho_min, p_min;
   int bad, degenerate;

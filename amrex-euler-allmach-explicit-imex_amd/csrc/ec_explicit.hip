@@ -30,6 +30,26 @@ struct StepArgs {
   int tiles_x, nblocks;
 };
 
+// First level of the partials reduction of a large grid: workgroup b
+// reduces partials [b * chunk, min((b + 1) * chunk, nparts)) into out[b], one
+// load per thread in flight, so that the one-workgroup finalize below reads a
+// few dozen partials instead of thousands.  Order independent, like every
+// combine here.
+template <int NT>
+__global__ void __launch_bounds__(NT) partials_reduce_kernel(const Partial* __restrict__ partials,
+                                                             int nparts, int chunk, Partial* __restrict__ out)
+{
+  __shared__ Red scratch[NT / 64];
+  Red r = red_identity();
+  const int lo = blockIdx.x * chunk, hi = min(lo + chunk, nparts);
+  for (int p = lo + (int)threadIdx.x; p < hi; p += NT) {
+    const Partial q = partials[p];
+    red_combine(r, Red{q.rate, q.rho_min, q.p_min, q.bad, q.degenerate});
+  }
+  r = block_reduce<NT>(r, scratch);
+  store_partial(out, blockIdx.x, r);
+}
+
 // Reduces the per-workgroup partials of a step (or of a state reduction) and
 // updates the run control.  One workgroup; launched after the producer, so
 // the kernel boundary orders the partial stores before these loads and no
@@ -191,8 +211,25 @@ ec_status launch_step_raw(const GridParams& grid, int tile_id, hipStream_t strea
   step_fn(tile_id, grid.riemann, grid.order)(a, dim3(a.nblocks), dim3(s.nt), stream);
   if (ev1 && hipEventRecord(ev1, stream) != hipSuccess) return EC_ERR_HIP;
   if (a.diag) {
-    hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0, stream,
-                       partials, a.nblocks, ctrl, device_dt ? 1 : 0, a.dt_log);
+    // beyond kTwoLevelMin tiles the partials are first reduced by up to
+    // kPartialGroupsMax workgroups into the slots after the tile partials
+    const Partial* fin = partials;
+    int nfin = a.nblocks;
+    if (a.nblocks >= kTwoLevelMin && a.nblocks + kPartialGroupsMax <= max_partials) {
+      const int chunk = (a.nblocks + kPartialGroupsMax - 1) / kPartialGroupsMax;
+      const int groups = (a.nblocks + chunk - 1) / chunk;
+      hipLaunchKernelGGL((partials_reduce_kernel<256>), dim3(groups), dim3(256), 0, stream, partials,
+                         a.nblocks, chunk, partials + a.nblocks);
+      fin = partials + a.nblocks;
+      nfin = groups;
+    }
+    if (nfin <= 256) {
+      hipLaunchKernelGGL((finalize_kernel<256>), dim3(1), dim3(256), 0, stream, fin, nfin, ctrl,
+                         device_dt ? 1 : 0, a.dt_log);
+    } else {
+      hipLaunchKernelGGL((finalize_kernel<kFinalizeThreads>), dim3(1), dim3(kFinalizeThreads), 0, stream,
+                         fin, nfin, ctrl, device_dt ? 1 : 0, a.dt_log);
+    }
   }
   return hipGetLastError() == hipSuccess ? EC_OK : EC_ERR_HIP;
 }
