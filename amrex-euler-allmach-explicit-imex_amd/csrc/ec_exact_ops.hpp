@@ -17,6 +17,8 @@
 //
 //   recip(b)      = v_rcp + the same 2 Newton steps            (5 instructions)
 //   quot(a, R)    = v_mul, v_fma, v_fma, v_div_fixup            (4 instructions)
+//   quot_pos(a, R) = the same without v_div_fixup, for a numerator that is
+//                   nonzero, or +0 over a positive denominator    (3 instructions)
 //   sqrt_core(x)  = v_rsq + the same refinement                 (10 instructions)
 //
 // give the bits of a / b and sqrt(x) exactly, and one recip() serves every
@@ -51,6 +53,20 @@ __device__ __forceinline__ double quot(double a, const Recip& r)
   const double q = a * r.y;
   const double e = fma(-r.b, q, a);
   return __builtin_amdgcn_div_fixup(fma(e, r.y, q), r.b, a);
+}
+
+// a / r.b, bit for bit, for r.b inside the window and a numerator that is
+// inside the window and nonzero, or +0 over a positive r.b.  Beyond the
+// special operands (0, inf, NaN) v_div_fixup only gives the result the sign
+// of a / b, and the refined quotient already carries that sign here: for
+// a != 0 it is a * y corrected by a relative O(2^-52) term, and for a = +0,
+// b > 0 every step yields +0 (the residual fma adds -0 and +0, which round
+// to +0).  So v_div_fixup is the identity and is left out.
+__device__ __forceinline__ double quot_pos(double a, const Recip& r)
+{
+  const double q = a * r.y;
+  const double e = fma(-r.b, q, a);
+  return fma(e, r.y, q);
 }
 
 // sqrt(x), bit for bit, for finite x >= 2^-767 (and NaN or negative x, which

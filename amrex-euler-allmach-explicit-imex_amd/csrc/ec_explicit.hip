@@ -81,8 +81,19 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const Partial* partials, i
 }
 
 
+// Waves per SIMD the step kernel is compiled for: the 256-thread tiles (LDS
+// below 32 KiB with the split slopes) run five workgroups per CU when their
+// registers fit 96 VGPRs; the 512-thread tile is held to two workgroups by its
+// LDS anyway.
+template <int NT>
+constexpr int step_waves_per_eu()
+{
+  return NT == 256 ? 5 : 1;
+}
+
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
-__global__ void __launch_bounds__(NT) explicit_step_kernel(StepArgs A)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<NT>())))
+explicit_step_kernel(StepArgs A)
 {
   using T = TileStep<TX, TY, NT, RIEMANN, ORDER>;
   __shared__ typename T::Smem sm;
@@ -130,12 +141,15 @@ struct TileShape {
   int tx, ty, nt;
 };
 
-using StepFn = void (*)(StepArgs, dim3, dim3, hipStream_t);
+// Launches the step kernel and returns its workgroup count (the number of
+// partials it writes), or 0 on a HIP error.
+using StepFn = int (*)(const StepArgs&, hipStream_t);
 
 template <int TX, int TY, int NT, int R, int O>
-void launch_step(StepArgs a, dim3 grid, dim3 block, hipStream_t s)
+int launch_step(const StepArgs& a, hipStream_t s)
 {
-  hipLaunchKernelGGL((explicit_step_kernel<TX, TY, NT, R, O>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((explicit_step_kernel<TX, TY, NT, R, O>), dim3(a.nblocks), dim3(NT), 0, s, a);
+  return a.nblocks;
 }
 
 template <int TX, int TY, int NT>
@@ -208,19 +222,20 @@ ec_status launch_step_raw(const GridParams& grid, int tile_id, hipStream_t strea
   a.nblocks = a.tiles_x * tiles_y;
   if (a.nblocks > max_partials) return EC_ERR_INVALID_ARGUMENT;
   if (ev0 && hipEventRecord(ev0, stream) != hipSuccess) return EC_ERR_HIP;
-  step_fn(tile_id, grid.riemann, grid.order)(a, dim3(a.nblocks), dim3(s.nt), stream);
+  const int nparts = step_fn(tile_id, grid.riemann, grid.order)(a, stream);
+  if (nparts <= 0) return EC_ERR_HIP;
   if (ev1 && hipEventRecord(ev1, stream) != hipSuccess) return EC_ERR_HIP;
   if (a.diag) {
-    // beyond kTwoLevelMin tiles the partials are first reduced by up to
-    // kPartialGroupsMax workgroups into the slots after the tile partials
+    // beyond kTwoLevelMin partials they are first reduced by up to
+    // kPartialGroupsMax workgroups into the slots after them
     const Partial* fin = partials;
-    int nfin = a.nblocks;
-    if (a.nblocks >= kTwoLevelMin && a.nblocks + kPartialGroupsMax <= max_partials) {
-      const int chunk = (a.nblocks + kPartialGroupsMax - 1) / kPartialGroupsMax;
-      const int groups = (a.nblocks + chunk - 1) / chunk;
+    int nfin = nparts;
+    if (nparts >= kTwoLevelMin && nparts + kPartialGroupsMax <= max_partials) {
+      const int chunk = (nparts + kPartialGroupsMax - 1) / kPartialGroupsMax;
+      const int groups = (nparts + chunk - 1) / chunk;
       hipLaunchKernelGGL((partials_reduce_kernel<256>), dim3(groups), dim3(256), 0, stream, partials,
-                         a.nblocks, chunk, partials + a.nblocks);
-      fin = partials + a.nblocks;
+                         nparts, chunk, partials + nparts);
+      fin = partials + nparts;
       nfin = groups;
     }
     if (nfin <= 256) {

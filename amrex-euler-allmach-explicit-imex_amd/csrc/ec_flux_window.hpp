@@ -52,9 +52,9 @@ __device__ __forceinline__ FaceSide face_side(const Prim& q, double gamma, const
 {
   FaceSide s;
   s.q2 = q.u * q.u + q.v * q.v;
-  s.E = quot(q.p, rg) + 0.5 * q.r * s.q2;
+  s.E = quot_pos(q.p, rg) + 0.5 * q.r * s.q2;
   s.rr = recip(q.r);
-  s.c = sqrt_core(quot(gamma * q.p, s.rr));
+  s.c = sqrt_core(quot_pos(gamma * q.p, s.rr));
   return s;
 }
 
@@ -138,8 +138,8 @@ __device__ __forceinline__ void low_mach_w(const Prim& L, const Prim& R, const F
 {
   const double sp_l = sqrt_core0(a.q2);
   const double sp_r = sqrt_core0(b.q2);
-  const double m_l = quot(sp_l, recip(fmax(a.c, 1.0e-12)));
-  const double m_r = quot(sp_r, recip(fmax(b.c, 1.0e-12)));
+  const double m_l = quot_pos(sp_l, recip(fmax(a.c, 1.0e-12)));
+  const double m_r = quot_pos(sp_r, recip(fmax(b.c, 1.0e-12)));
   const double z = fmin(fmax(fmax(m_l, m_r), 0.0), 1.0);
   const double u_avg = 0.5 * (L.u + R.u);
   const double v_avg = 0.5 * (L.v + R.v);
@@ -178,7 +178,8 @@ __device__ __forceinline__ Cons xie_hllcp_w(const Prim& L, const Prim& R, const 
     const Recip ra = recip(alpha_denom);
     // hllc_star_state_is_degenerate: its denominator rho_l (s_l - u_l) -
     // rho_r (s_r - u_r) is -alpha_denom exactly, and a / -b == -(a / b).
-    const double s_m = -quot(R.p - L.p + L.r * q_l * dl - R.r * q_r * dr, ra);
+    // (only |s_l - s_m| and |s_r - s_m| are used: a zero's sign cannot matter)
+    const double s_m = -quot_pos(R.p - L.p + L.r * q_l * dl - R.r * q_r * dr, ra);
     degen = fabs(s_l - s_m) <= 1.0e-12 || fabs(s_r - s_m) <= 1.0e-12;
     const double s_star = quot(alpha_r * q_r - alpha_l * q_l + L.p - R.p, ra);
     const double rld = s_l - s_star;
@@ -194,9 +195,9 @@ __device__ __forceinline__ Cons xie_hllcp_w(const Prim& L, const Prim& R, const 
     const double u_hat = quot(w_l * L.u + w_r * R.u, rroe);
     const double v_hat = quot(w_l * L.v + w_r * R.v, rroe);
     const double q_hat = AX == 0 ? u_hat : v_hat;
-    const double h_l = quot(a.E + L.p, a.rr);
-    const double h_r = quot(b.E + R.p, b.rr);
-    const double h_hat = quot(w_l * h_l + w_r * h_r, rroe);
+    const double h_l = quot_pos(a.E + L.p, a.rr);
+    const double h_r = quot_pos(b.E + R.p, b.rr);
+    const double h_hat = quot_pos(w_l * h_l + w_r * h_r, rroe);
     const double uv2 = u_hat * u_hat + v_hat * v_hat;
     const double c_hat_sq = (gamma - 1.0) * (h_hat - 0.5 * uv2);
     if (!(c_hat_sq > 0.0)) break;         // finite inside the window
@@ -204,8 +205,8 @@ __device__ __forceinline__ Cons xie_hllcp_w(const Prim& L, const Prim& R, const 
     const double m_hat = quot(q_hat, recip(fmax(c_hat, 1.0e-12)));
     const double sp_l = sqrt_core0(a.q2);
     const double sp_r = sqrt_core0(b.q2);
-    const double theta = fmin(fmax(fmax(quot(sp_l, recip(fmax(c_l, 1.0e-12))),
-                                        quot(sp_r, recip(fmax(c_r, 1.0e-12)))), 0.0), 1.0);
+    const double theta = fmin(fmax(fmax(quot_pos(sp_l, recip(fmax(c_l, 1.0e-12))),
+                                        quot_pos(sp_r, recip(fmax(c_r, 1.0e-12)))), 0.0), 1.0);
     const double f = fmin(fmax(sensor_f, 0.0), 1.0);
     const double pss = theta * p_star + (1.0 - theta) * 0.5 * (L.p + R.p);
     const double psss = f * pss + (1.0 - f) * p_star;
@@ -217,9 +218,9 @@ __device__ __forceinline__ Cons xie_hllcp_w(const Prim& L, const Prim& R, const 
     const double alpha = left ? alpha_l : alpha_r;
     const double EK = left ? a.E : b.E;
     const Recip rk = left ? a.rr : b.rr;
-    const double rs = quot(alpha, recip(left ? rld : rrd));
-    const double ek = quot(EK, rk);
-    const double es = ek + (s_star - qn) * (s_star + quot(K.p, recip(alpha)));
+    const double rs = quot_pos(alpha, recip(left ? rld : rrd));   // |alpha| > 1e-12
+    const double ek = quot_pos(EK, rk);
+    const double es = ek + (s_star - qn) * (s_star + quot_pos(K.p, recip(alpha)));
     constexpr double nx = AX == 0 ? 1.0 : 0.0;
     constexpr double ny = AX == 1 ? 1.0 : 0.0;
     const double rss = rs * s_star;
@@ -248,8 +249,8 @@ __device__ __forceinline__ Cons interface_flux_w(const Prim& L, const Prim& R, d
     FaceSide ca = a, cb = b;
     ca.q2 = cl.u * cl.u + cl.v * cl.v;
     cb.q2 = cr.u * cr.u + cr.v * cr.v;
-    ca.E = quot(cl.p, rg) + 0.5 * cl.r * ca.q2;
-    cb.E = quot(cr.p, rg) + 0.5 * cr.r * cb.q2;
+    ca.E = quot_pos(cl.p, rg) + 0.5 * cl.r * ca.q2;
+    cb.E = quot_pos(cr.p, rg) + 0.5 * cr.r * cb.q2;
     return hllc_w<AX>(cl, cr, ca, cb, gamma, degen);
   } else if constexpr (RIEMANN == 3) {
     return xie_hllcp_w<AX>(L, R, a, b, gamma, sensor_f, degen);

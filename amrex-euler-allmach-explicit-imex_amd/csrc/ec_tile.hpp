@@ -104,7 +104,7 @@ __device__ __forceinline__ void red_cell_w(Red& r, const Cons& u, const GridPara
     return;
   }
   const Recip rr = recip(u.r);
-  const double inv_rho = quot(1.0, rr);
+  const double inv_rho = quot_pos(1.0, rr);
   const double ux = u.mx * inv_rho;
   const double uy = u.my * inv_rho;
   const double kinetic = 0.5 * u.r * (ux * ux + uy * uy);
@@ -115,7 +115,7 @@ __device__ __forceinline__ void red_cell_w(Red& r, const Cons& u, const GridPara
   r.pmin = fmin(r.pmin, bad ? -INFINITY : p);
   r.bad += bad ? 1 : 0;
   double c;
-  if (__all(pos_in_window(p) ? 1 : 0)) c = sqrt_core(quot(g.gamma * p, rr));
+  if (__all(pos_in_window(p) ? 1 : 0)) c = sqrt_core(quot_pos(g.gamma * p, rr));
   else c = sqrt(g.gamma * p / u.r);
   r.rate = fmax(r.rate, cell_rate(ux, uy, c, g));
 }
@@ -305,7 +305,13 @@ __device__ __forceinline__ void store_partial(Partial* partials, int slot, const
   if (threadIdx.x == 0) partials[slot] = Partial{r.rate, r.rmin, r.pmin, r.bad, r.deg};
 }
 
-template <int TX, int TY, int NT, int RIEMANN, int ORDER>
+// SPLIT_SLOPES: see SPLIT below.  PREFETCH_OWN: each thread's own cell of
+// stage E (one cell per thread when the tile has no more cells than threads)
+// is loaded with stage A's loads, so a latency-bound caller (one tile per
+// round) does not wait on a second round trip in stage E; it costs 8 VGPRs
+// through stages A-D.
+template <int TX, int TY, int NT, int RIEMANN, int ORDER, bool SPLIT_SLOPES = true,
+          bool PREFETCH_OWN = false>
 struct TileStep {
   static constexpr int RX = TX + 4, RY = TY + 4, NR = RX * RY;        // q: [-2,TX+1]x[-2,TY+1]
   static constexpr int SX = TX + 2, SY = TY + 2, NS = SX * SY;        // slopes: [-1,TX]x[-1,TY]
@@ -315,10 +321,17 @@ struct TileStep {
   static constexpr int YRX = TX + 2, NYR = YRX * (TY + 1);            // y ratios [-1,TX]x[0,TY]
   static constexpr bool HP = RIEMANN == 3;
   static constexpr bool O2 = ORDER == 2;
+  // SPLIT: the slopes of rho, u and v are held for one axis at a time (the
+  // x-slopes until the x-faces are done, then the y-slopes in their place);
+  // both pressure slopes stay, for the sensor's ratios.  Five slope planes
+  // instead of eight.
+  static constexpr bool SPLIT = O2 && SPLIT_SLOPES;
   static constexpr int OFF_Q = 0;
-  static constexpr int OFF_SX = OFF_Q + 4 * NR;
-  static constexpr int OFF_SY = OFF_SX + (O2 ? 4 * NS : 0);
-  static constexpr int OFF_RX = OFF_SY + (O2 ? 4 * NS : 0);
+  static constexpr int OFF_S = OFF_Q + 4 * NR;
+  static constexpr int NSP = O2 ? (SPLIT ? 5 : 8) : 0;
+  static constexpr int OFF_SX = OFF_S;                      // unsplit: x-slopes, 4 planes
+  static constexpr int OFF_SY = OFF_S + 4 * NS;             // unsplit: y-slopes, 4 planes
+  static constexpr int OFF_RX = OFF_S + NSP * NS;
   static constexpr int OFF_RY = OFF_RX + (HP ? NXR : 0);
   static constexpr int END1 = OFF_RY + (HP ? NYR : 0);
   static constexpr int OFF_FX = 0;
@@ -363,79 +376,55 @@ struct TileStep {
     double* lds = sm.lds;
     unsigned char* degx = sm.degx;
     unsigned char* degy = sm.degy;
-    const int tid = threadIdx.x;
+    // The thread index passes through an empty asm statement: a caller that
+    // runs several tiles in a loop cannot then hoist this tile's index
+    // arithmetic out of the loop and keep it live across every stage.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const double gamma = g.gamma;
     Red mine = red_identity();
 
-    // Each thread's own cell of stage E (one cell per thread when the tile
-    // has no more cells than threads), loaded now so that its latency hides
-    // behind stages A-D.
-    constexpr bool ONE_CELL = TX * TY <= NT;
+    constexpr int NA = (NR + NT - 1) / NT;
+    constexpr bool OWN = PREFETCH_OWN && TX * TY <= NT;
     double own[4] = {0.0, 0.0, 0.0, 0.0};
-    if constexpr (ONE_CELL) {
-      const int ci = tid % TX, cj = tid / TX;
-      if (tid < TX * TY && ci < g.nx - i0 && cj < g.ny - j0) {
-        const long long k = (long long)(j0 + cj) * rs + (i0 + ci);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) own[c] = uin[c * cs + k];
-      }
-    }
-
-    // A. primitive variables of the tile and its 2-cell halo.  A tile whose
-    // halo lies inside the domain reads it directly (the ghost rule is the
-    // identity there); boundary tiles go through the rule.
     const bool interior = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= g.nx && j0 + TY + 2 <= g.ny;
     bool ok = pos_in_window(gamma) && pos_in_window(gamma - 1.0);
-    if (interior) {
-      // every load of the thread first, then the conversions
-      constexpr int NA = (NR + NT - 1) / NT;
-      Cons raw[NA];
-      const double* tb = uin + (long long)(j0 - 2) * rs + (i0 - 2);
-  #pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        const int t = tid + a * NT;
-        if (t < NR) {
-          const long long k = (long long)(t / RX) * rs + t % RX;
-          raw[a] = Cons{tb[k], tb[cs + k], tb[2 * cs + k], tb[3 * cs + k]};
-        }
-      }
-  #pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        const int t = tid + a * NT;
-        if (t < NR) {
-          const Prim q = to_prim(raw[a], gamma);
-          lds[OFF_Q + t] = q.r;
-          lds[OFF_Q + NR + t] = q.u;
-          lds[OFF_Q + 2 * NR + t] = q.v;
-          lds[OFF_Q + 3 * NR + t] = q.p;
-          ok = ok & pos_in_window(q.r) & pos_in_window(q.p) & vel_in_window(q.u) & vel_in_window(q.v);
-        }
-      }
-    } else {
-      // through the ghost rule: every load first, then the values
-      constexpr int NA = (NR + NT - 1) / NT;
-      CellSrc src[NA];
-      Cons raw[NA];
-  #pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        const int t = tid + a * NT;
-        if (t < NR) {
-          src[a] = cell_src(g, rs, i0 + t % RX - 2, j0 + t / RX - 2);
-          raw[a] = cell_read(uin, cs, src[a]);
-        }
-      }
-  #pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        const int t = tid + a * NT;
-        if (t < NR) {
-          const Cons c = cell_value(src[a], raw[a], g, i0 + t % RX - 2, j0 + t / RX - 2, time_src);
-          const Prim q = to_prim(c, gamma);
-          lds[OFF_Q + t] = q.r;
-          lds[OFF_Q + NR + t] = q.u;
-          lds[OFF_Q + 2 * NR + t] = q.v;
-          lds[OFF_Q + 3 * NR + t] = q.p;
-          ok = ok & pos_in_window(q.r) & pos_in_window(q.p) & vel_in_window(q.u) & vel_in_window(q.v);
-        }
+    // A. primitive variables of the tile and its 2-cell halo.  Each lane
+    // first finds where its cells come from (directly on a tile whose halo
+    // lies inside the domain, where the ghost rule is the identity; through
+    // the ghost rule otherwise), then every load of stage A is issued in
+    // straight-line code: loads under per-lane branches would each wait for
+    // the loads before them.  Lanes without a cell read a clamped one and
+    // ignore it; fixed and exact ghost cells read cell 0 and ignore it.
+    Cons raw[NA];
+    CellSrc src[NA];
+    if constexpr (OWN) {
+      const int ci = min(tid % TX, g.nx - i0 - 1), cj = min(tid / TX, min(TY, g.ny - j0) - 1);
+      const long long k = (long long)(j0 + cj) * rs + (i0 + ci);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) own[c] = uin[c * cs + k];
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int t = min(tid + a * NT, NR - 1);
+      const int i = i0 + t % RX - 2, j = j0 + t / RX - 2;
+      if (interior) src[a] = CellSrc{(long long)j * rs + i, 0, 0, false, false};
+      else src[a] = cell_src(g, rs, i, j);
+      const long long k = src[a].kind == 0 ? src[a].k : 0;
+      raw[a] = Cons{uin[k], uin[cs + k], uin[2 * cs + k], uin[3 * cs + k]};
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int t = tid + a * NT;
+      if (t < NR) {
+        const Cons c = interior ? raw[a]
+                                : cell_value(src[a], raw[a], g, i0 + t % RX - 2, j0 + t / RX - 2, time_src);
+        const Prim q = to_prim(c, gamma);
+        lds[OFF_Q + t] = q.r;
+        lds[OFF_Q + NR + t] = q.u;
+        lds[OFF_Q + 2 * NR + t] = q.v;
+        lds[OFF_Q + 3 * NR + t] = q.p;
+        ok = ok & pos_in_window(q.r) & pos_in_window(q.p) & vel_in_window(q.u) & vel_in_window(q.v);
       }
     }
     {
@@ -447,12 +436,58 @@ struct TileStep {
   #pragma unroll
     for (int w = 0; w < NT / 64; ++w) win = win && sm.wave_ok[w] != 0;
 
+    // Slope planes: component c (0..3 = rho, u, v, p) along axis.
+    auto slope_plane = [&](int axis, int c) -> int {
+      if constexpr (SPLIT) return c < 3 ? OFF_S + c * NS : OFF_S + (3 + axis) * NS;
+      return (axis == 0 ? OFF_SX : OFF_SY) + c * NS;
+    };
     // B. limited slopes (reconstruction_fluxes.hpp:2-12).
-    if constexpr (O2) {
+    if constexpr (SPLIT) {
+      // phase 1: every x-slope, and the y-slope of p
+      auto slopes1 = [&](auto fast_c, auto half_c) {
+        constexpr bool FAST = decltype(fast_c)::value;
+        constexpr bool HALF = decltype(half_c)::value;
+        constexpr double SC = HALF ? 0.5 : 1.0;
+        for (int t = tid; t < NS; t += NT) {
+          const int qc = (t / SX + 1) * RX + (t % SX + 1);
+          double st[4][3], pd, pu;
+  #pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const double* q = lds + OFF_Q + c * NR;
+            st[c][0] = q[qc];
+            st[c][1] = q[qc - 1];
+            st[c][2] = q[qc + 1];
+          }
+          pd = lds[OFF_Q + 3 * NR + qc - RX];
+          pu = lds[OFF_Q + 3 * NR + qc + RX];
+  #pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const double vc = st[c][0];
+            const double sx = FAST ? minmod_finite(vc - st[c][1], st[c][2] - vc)
+                                   : limited_slope(vc - st[c][1], st[c][2] - vc, g.limiter);
+            lds[OFF_S + c * NS + t] = SC * sx;
+          }
+          const double vc = st[3][0];
+          const double sy = FAST ? minmod_finite(vc - pd, pu - vc) : limited_slope(vc - pd, pu - vc, g.limiter);
+          lds[OFF_S + 4 * NS + t] = SC * sy;
+        }
+      };
+      constexpr bool HW = true;
+      if (win && g.limiter == 0) slopes1(std::true_type{}, std::bool_constant<HW>{});
+      else if (win) slopes1(std::false_type{}, std::bool_constant<HW>{});
+      else slopes1(std::false_type{}, std::false_type{});
+      __syncthreads();
+    } else if constexpr (O2) {
       // each cell's 5-point stencil of the 4 variables is read before any
       // slope is written, so the reads issue back to back
-      auto slopes = [&](auto fast_c) {
+      // HALF: the window path stores 0.5 * slope, which is exact there
+      // (every nonzero slope is at least 2^-153 in magnitude: a difference of
+      // window values, or a limiter combination of such differences), so a
+      // face state is q + h or q - h, one operation per component.
+      auto slopes = [&](auto fast_c, auto half_c) {
         constexpr bool FAST = decltype(fast_c)::value;
+        constexpr bool HALF = decltype(half_c)::value;
+        constexpr double SC = HALF ? 0.5 : 1.0;
         for (int t = tid; t < NS; t += NT) {
           const int qc = (t / SX + 1) * RX + (t % SX + 1);
           double st[4][5];
@@ -469,75 +504,101 @@ struct TileStep {
           for (int c = 0; c < 4; ++c) {
             const double vc = st[c][0];
             if constexpr (FAST) {
-              lds[OFF_SX + c * NS + t] = minmod_finite(vc - st[c][1], st[c][2] - vc);
-              lds[OFF_SY + c * NS + t] = minmod_finite(vc - st[c][3], st[c][4] - vc);
+              lds[OFF_SX + c * NS + t] = SC * minmod_finite(vc - st[c][1], st[c][2] - vc);
+              lds[OFF_SY + c * NS + t] = SC * minmod_finite(vc - st[c][3], st[c][4] - vc);
             } else {
-              lds[OFF_SX + c * NS + t] = limited_slope(vc - st[c][1], st[c][2] - vc, g.limiter);
-              lds[OFF_SY + c * NS + t] = limited_slope(vc - st[c][3], st[c][4] - vc, g.limiter);
+              lds[OFF_SX + c * NS + t] = SC * limited_slope(vc - st[c][1], st[c][2] - vc, g.limiter);
+              lds[OFF_SY + c * NS + t] = SC * limited_slope(vc - st[c][3], st[c][4] - vc, g.limiter);
             }
           }
         }
       };
       // minmod on finite differences when every state of the tile is inside
       // the window
-      if (win && g.limiter == 0) slopes(std::true_type{});
-      else slopes(std::false_type{});
+      constexpr bool HW = true;
+      if (win && g.limiter == 0) slopes(std::true_type{}, std::bool_constant<HW>{});
+      else if (win) slopes(std::false_type{}, std::bool_constant<HW>{});
+      else slopes(std::false_type{}, std::false_type{});
       __syncthreads();
     }
 
     // Face state of cell (ci,cj) (tile coordinates) on one side of an axis:
     // q + (0.5*side)*slope (reconstruction_fluxes.hpp:14-24).
-    auto face_state = [&](int ci, int cj, int axis, double half_side) -> Prim {
+    // HS: the slopes in LDS are halved (the window path), so half_side only
+    // chooses between adding and subtracting them; (-0.5) * s == -(0.5 * s).
+    auto face_state = [&](auto hs_c, int ci, int cj, int axis, double half_side) -> Prim {
+      constexpr bool HS = decltype(hs_c)::value;
       const int qi = (cj + 2) * RX + (ci + 2);
       Prim q{lds[OFF_Q + qi], lds[OFF_Q + NR + qi], lds[OFF_Q + 2 * NR + qi], lds[OFF_Q + 3 * NR + qi]};
       if constexpr (O2) {
         const int si = (cj + 1) * SX + (ci + 1);
-        const double* s = lds + (axis == 0 ? OFF_SX : OFF_SY) + si;
-        q.r = q.r + half_side * s[0];
-        q.u = q.u + half_side * s[NS];
-        q.v = q.v + half_side * s[2 * NS];
-        q.p = q.p + half_side * s[3 * NS];
+        const double sr = lds[slope_plane(axis, 0) + si], su = lds[slope_plane(axis, 1) + si];
+        const double sv = lds[slope_plane(axis, 2) + si], sp = lds[slope_plane(axis, 3) + si];
+        if constexpr (HS) {
+          if (half_side > 0.0) {
+            q.r = q.r + sr;
+            q.u = q.u + su;
+            q.v = q.v + sv;
+            q.p = q.p + sp;
+          } else {
+            q.r = q.r - sr;
+            q.u = q.u - su;
+            q.v = q.v - sv;
+            q.p = q.p - sp;
+          }
+        } else {
+          q.r = q.r + half_side * sr;
+          q.u = q.u + half_side * su;
+          q.v = q.v + half_side * sv;
+          q.p = q.p + half_side * sp;
+        }
       }
       return q;
     };
-    auto face_p = [&](int ci, int cj, int axis, double half_side) -> double {
+    auto face_p = [&](auto hs_c, int ci, int cj, int axis, double half_side) -> double {
+      constexpr bool HS = decltype(hs_c)::value;
       const int qi = (cj + 2) * RX + (ci + 2);
       double p = lds[OFF_Q + 3 * NR + qi];
       if constexpr (O2) {
         const int si = (cj + 1) * SX + (ci + 1);
-        p = p + half_side * lds[(axis == 0 ? OFF_SX : OFF_SY) + 3 * NS + si];
+        const double s = lds[slope_plane(axis, 3) + si];
+        if constexpr (HS) p = half_side > 0.0 ? p + s : p - s;
+        else p = p + half_side * s;
       }
       return p;
     };
+    using HalfW = std::true_type;
 
     // C. face pressure ratios for the HLLC-P sensor (reconstruction_fluxes.hpp:390-419).
     // Inside the window both face pressures lie between neighbouring cell
     // pressures, so they are finite and at least 2^-40 > 1e-14: the floor is
     // the identity, and the ratio min/max lies in (0, 1] already.
     auto ratio_w = [](double pl, double pr) {
-      return quot(fmin(pl, pr), recip(fmax(pl, pr)));
+      return quot_pos(fmin(pl, pr), recip(fmax(pl, pr)));
     };
     if constexpr (HP) {
       if (win) {
         for (int t = tid; t < NXR + NYR; t += NT) {
           if (t < NXR) {
             const int fi = t % XRX, j = t / XRX - 1;
-            lds[OFF_RX + t] = ratio_w(face_p(fi - 1, j, 0, 0.5), face_p(fi, j, 0, -0.5));
+            lds[OFF_RX + t] = ratio_w(face_p(HalfW{}, fi - 1, j, 0, 0.5), face_p(HalfW{}, fi, j, 0, -0.5));
           } else {
             const int s = t - NXR;
             const int i = s % YRX - 1, fj = s / YRX;
-            lds[OFF_RY + s] = ratio_w(face_p(i, fj - 1, 1, 0.5), face_p(i, fj, 1, -0.5));
+            lds[OFF_RY + s] = ratio_w(face_p(HalfW{}, i, fj - 1, 1, 0.5), face_p(HalfW{}, i, fj, 1, -0.5));
           }
         }
       } else {
         for (int t = tid; t < NXR + NYR; t += NT) {
           if (t < NXR) {
             const int fi = t % XRX, j = t / XRX - 1;
-            lds[OFF_RX + t] = pressure_ratio(face_p(fi - 1, j, 0, 0.5), face_p(fi, j, 0, -0.5));
+            lds[OFF_RX + t] = pressure_ratio(face_p(std::false_type{}, fi - 1, j, 0, 0.5),
+                                            face_p(std::false_type{}, fi, j, 0, -0.5));
           } else {
             const int s = t - NXR;
             const int i = s % YRX - 1, fj = s / YRX;
-            lds[OFF_RY + s] = pressure_ratio(face_p(i, fj - 1, 1, 0.5), face_p(i, fj, 1, -0.5));
+            lds[OFF_RY + s] = pressure_ratio(face_p(std::false_type{}, i, fj - 1, 1, 0.5),
+                                            face_p(std::false_type{}, i, fj, 1, -0.5));
           }
         }
       }
@@ -592,8 +653,9 @@ struct TileStep {
       dflag = 0;
       out = Cons{0.0, 0.0, 0.0, 0.0};
       if (needed) {
-        const Prim L = face_state(li, lj, AX, 0.5);
-        const Prim R = face_state(ri, rj, AX, -0.5);
+        using HS = std::bool_constant<W>;
+        const Prim L = face_state(HS{}, li, lj, AX, 0.5);
+        const Prim R = face_state(HS{}, ri, rj, AX, -0.5);
         if constexpr (W) {
           bool degen;
           out = interface_flux_w<AX, RIEMANN>(L, R, gamma, rg, sensor, degen);
@@ -608,11 +670,43 @@ struct TileStep {
     };
     Cons fxv[NPX], fyv[NPY];
     unsigned char dfx[NPX], dfy[NPY];
+    // phase 2 of the split slopes: the y-slopes of rho, u and v, over the
+    // x-slopes the x-faces no longer need
+    auto slopes2 = [&](auto fast_c, auto half_c) {
+      constexpr bool FAST = decltype(fast_c)::value;
+      constexpr bool HALF = decltype(half_c)::value;
+      constexpr double SC = HALF ? 0.5 : 1.0;
+      for (int t = tid; t < NS; t += NT) {
+        const int qc = (t / SX + 1) * RX + (t % SX + 1);
+        double st[3][3];
+  #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double* q = lds + OFF_Q + c * NR;
+          st[c][0] = q[qc];
+          st[c][1] = q[qc - RX];
+          st[c][2] = q[qc + RX];
+        }
+  #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double vc = st[c][0];
+          const double sy = FAST ? minmod_finite(vc - st[c][1], st[c][2] - vc)
+                                 : limited_slope(vc - st[c][1], st[c][2] - vc, g.limiter);
+          lds[OFF_S + c * NS + t] = SC * sy;
+        }
+      }
+    };
     auto all_faces = [&](auto w_c) {
+      constexpr bool W = decltype(w_c)::value;
   #pragma unroll
       for (int p = 0; p < NPX; ++p) {
         const int f = tid + p * NT;
         if (f < NXF) face(std::integral_constant<int, 0>{}, w_c, f, fxv[p], dfx[p]);
+      }
+      if constexpr (SPLIT) {
+        __syncthreads();  // every x-face has read the x-slopes
+        if (W && g.limiter == 0) slopes2(std::true_type{}, std::bool_constant<W>{});
+        else slopes2(std::false_type{}, std::bool_constant<W>{});
+        __syncthreads();
       }
   #pragma unroll
       for (int p = 0; p < NPY; ++p) {
@@ -661,7 +755,7 @@ struct TileStep {
         for (int c = 0; c < 4; ++c) {
           const double fxl = lds[OFF_FX + c * NXF + xl], fxr = lds[OFF_FX + c * NXF + xl + 1];
           const double fyb = lds[OFF_FY + c * NYF + yb], fyt = lds[OFF_FY + c * NYF + yb + TX];
-          const double old = ONE_CELL ? own[c] : uin[c * cs + k];
+          const double old = OWN ? own[c] : uin[c * cs + k];
           nu[c] = old - dt * ((fxr - fxl) * inv_dx + (fyt - fyb) * inv_dy);
           uout[c * cs + k] = nu[c];
         }
