@@ -37,14 +37,13 @@
 #include "ec_exact_api.hpp"
 #include "ec_tile.hpp"
 
-#ifndef EC_BATCH_SPLIT
-#define EC_BATCH_SPLIT 0
-#endif
-#ifndef EC_BATCH_OWN
-#define EC_BATCH_OWN 1
-#endif
-
 namespace ec {
+
+// A round of an ensemble is one tile per workgroup and waits on latency, not
+// on issue: the tile keeps all eight slope planes (the split form adds two
+// barriers) and loads the cells of its stage E with those of stage A.
+template <int TX, int TY, int NT, int RIEMANN, int ORDER>
+using BatchTile = TileStep<TX, TY, NT, RIEMANN, ORDER, /*SPLIT_SLOPES=*/false, /*PREFETCH_OWN=*/true>;
 
 struct MemberDev {
   GridParams g;
@@ -104,11 +103,11 @@ __device__ Red tile_state_reduce(Red* scratch, const double* __restrict__ u, lon
 // by this workgroup.  Returns whether the member stepped (so another round
 // of it follows).
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
-__device__ bool member_round(typename TileStep<TX, TY, NT, RIEMANN, ORDER, EC_BATCH_SPLIT != 0, EC_BATCH_OWN != 0>::Smem& sm,
+__device__ bool member_round(typename BatchTile<TX, TY, NT, RIEMANN, ORDER>::Smem& sm,
                              RoundShared& rsh, const BatchArgs& A, const MemberDev& md, int m, int ta,
                              int tb, int p)
 {
-  using T = TileStep<TX, TY, NT, RIEMANN, ORDER, EC_BATCH_SPLIT != 0, EC_BATCH_OWN != 0>;
+  using T = BatchTile<TX, TY, NT, RIEMANN, ORDER>;
   const int tid = threadIdx.x;
   const DevCtrl* cin = A.ctrl + (long long)p * A.nm + m;
 
@@ -186,7 +185,7 @@ __global__ void __launch_bounds__(NT) batch_init_kernel(BatchArgs A)
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
 __global__ void __launch_bounds__(NT) batch_round_kernel(BatchArgs A, int p)
 {
-  using T = TileStep<TX, TY, NT, RIEMANN, ORDER, EC_BATCH_SPLIT != 0, EC_BATCH_OWN != 0>;
+  using T = BatchTile<TX, TY, NT, RIEMANN, ORDER>;
   __shared__ typename T::Smem sm;
   __shared__ RoundShared rsh;
   const TileInfo& ti = A.round_tiles[blockIdx.x];
@@ -227,7 +226,7 @@ __device__ bool member_barrier(const BatchArgs& A, RoundShared& rsh, int m, unsi
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
 __global__ void __launch_bounds__(NT) batch_persistent_kernel(BatchArgs A, const int2* ranges)
 {
-  using T = TileStep<TX, TY, NT, RIEMANN, ORDER, EC_BATCH_SPLIT != 0, EC_BATCH_OWN != 0>;
+  using T = BatchTile<TX, TY, NT, RIEMANN, ORDER>;
   __shared__ typename T::Smem sm;
   __shared__ RoundShared rsh;
   constexpr int kMaxOwned = 64;   // members one workgroup may hold (host-checked)
