@@ -110,6 +110,16 @@ __device__ bool member_round(typename BatchTile<TX, TY, NT, RIEMANN, ORDER>::Sme
   using T = BatchTile<TX, TY, NT, RIEMANN, ORDER>;
   const int tid = threadIdx.x;
   const DevCtrl* cin = A.ctrl + (long long)p * A.nm + m;
+  const long long rs = md.g.nx, cs = (long long)md.g.nx * md.g.ny;
+
+  // 0. stage A's loads of the first tile, issued before the fold so that the
+  // two wait as one: every member steps once per round from step 0 while it
+  // runs, so a running member's state at the round of parity p is in buffer p
+  // (step & 1 == p).  The step is checked after the fold, and a tile whose
+  // state is elsewhere is loaded again.
+  typename T::Loaded ld;
+  const double* guess = (p ? A.ub : A.ua) + md.off;
+  if (ta < tb) T::load(ld, guess, rs, cs, md.g, (ta % md.tiles_x) * TX, (ta / md.tiles_x) * TY);
 
   // 1. fold the previous round's partials of every tile of the member.  The
   // partials are read whether or not a fold is pending (the buffer always
@@ -145,14 +155,14 @@ __device__ bool member_round(typename BatchTile<TX, TY, NT, RIEMANN, ORDER>::Sme
 
   // 2. step the owned tiles
   if (active) {
-    const long long rs = md.g.nx, cs = (long long)md.g.nx * md.g.ny;
     const double* uin = ((step & 1) ? A.ub : A.ua) + md.off;
     double* uout = ((step & 1) ? A.ua : A.ub) + md.off;
     Partial* out = A.part + (long long)p * A.ntiles + md.tile0;
     for (int t = ta; t < tb; ++t) {
       const int i0 = (t % md.tiles_x) * TX, j0 = (t / md.tiles_x) * TY;
+      if (t != ta || uin != guess) T::load(ld, uin, rs, cs, md.g, i0, j0);
       // exact-Dirichlet ghosts take the step's start time, held in LDS
-      const Red rr = T::run(sm, uin, uout, rs, cs, md.g, i0, j0, dt, true, &rsh.c.time);
+      const Red rr = T::run_loaded(sm, ld, uin, uout, rs, cs, md.g, i0, j0, dt, true, &rsh.c.time);
       store_partial(out, t, rr);
       __syncthreads();
     }
@@ -181,7 +191,8 @@ __global__ void __launch_bounds__(NT) batch_init_kernel(BatchArgs A)
 }
 
 // Graph mode: one launch = one round of one member group; workgroup b owns
-// the tile round_tiles[b].
+// the tile round_tiles[b].  Between replays the host moves the tiles of
+// finished members to the end of the table, marked m = -1.
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
 __global__ void __launch_bounds__(NT) batch_round_kernel(BatchArgs A, int p)
 {
@@ -189,6 +200,7 @@ __global__ void __launch_bounds__(NT) batch_round_kernel(BatchArgs A, int p)
   __shared__ typename T::Smem sm;
   __shared__ RoundShared rsh;
   const TileInfo& ti = A.round_tiles[blockIdx.x];
+  if (ti.m < 0) return;  // a member that had finished when this replay began
   (void)member_round<TX, TY, NT, RIEMANN, ORDER>(sm, rsh, A, ti.md, ti.m, ti.lt, ti.lt + 1, p);
 }
 
@@ -389,6 +401,11 @@ struct ec_batch {
   hipEvent_t gev[kMaxGroups] = {};
   TileInfo* gtiles[kMaxGroups] = {};
   int gblocks[kMaxGroups] = {};
+  // host copy of each group's full tile table (member order), and the table
+  // last uploaded (kept until the copy has completed: every replay ends with
+  // a stream synchronisation)
+  std::vector<TileInfo> gtiles_host[kMaxGroups];
+  std::vector<TileInfo> gtiles_up[kMaxGroups];
   hipGraphExec_t ggraph[kMaxGroups] = {};
   int last_mode = 0, last_shape = 0, last_rounds = 0;
   double last_ms = 0.0;
@@ -505,10 +522,12 @@ ec_status configure_tables(ec_batch* b, int shape)
     if (b->gtiles[g]) (void)hipFree(b->gtiles[g]);
     b->gtiles[g] = nullptr;
     b->gblocks[g] = 0;
+    b->gtiles_host[g].clear();
   }
   for (int g = 0; g < G; ++g) {
     std::sort(members_of[g].begin(), members_of[g].end());
-    std::vector<TileInfo> gt;
+    std::vector<TileInfo>& gt = b->gtiles_host[g];
+    gt.clear();
     for (int m : members_of[g])
       for (int t = 0; t < md[m].ntiles; ++t) gt.push_back(ti[md[m].tile0 + t]);
     BT_TRY(b, hipMalloc(&b->gtiles[g], sizeof(TileInfo) * gt.size()));
@@ -581,6 +600,31 @@ long long round_bound(const ec_batch* b)
   return mx + 2;
 }
 
+// Uploads each group's tile table with the tiles of members marked in
+// `finished` moved to the end as m = -1 (their workgroups exit at once); the
+// order of the other tiles is kept.  The copies are ordered before the next
+// replay on `stream`, which every group's stream waits for.
+static ec_status upload_round_tables(ec_batch* b, const std::vector<char>& finished)
+{
+  for (int g = 0; g < kMaxGroups; ++g) {
+    const std::vector<TileInfo>& full = b->gtiles_host[g];
+    if (full.empty() || !b->gtiles[g]) continue;
+    std::vector<TileInfo>& t = b->gtiles_up[g];
+    t.clear();
+    t.reserve(full.size());
+    for (const TileInfo& x : full)
+      if (!finished[x.m]) t.push_back(x);
+    while (t.size() < full.size()) {
+      TileInfo x = full[0];
+      x.m = -1;
+      t.push_back(x);
+    }
+    BT_TRY(b, hipMemcpyAsync(b->gtiles[g], t.data(), sizeof(TileInfo) * t.size(), hipMemcpyHostToDevice,
+                             b->stream));
+  }
+  return EC_OK;
+}
+
 ec_status run_graph(ec_batch* b, int* rounds_out)
 {
   const BatchShape s = kBatchShapes[b->cfg_shape];
@@ -615,9 +659,14 @@ ec_status run_graph(ec_batch* b, int* rounds_out)
     b->graph_groups = b->cfg_groups;
   }
   // After an even number of rounds the latest run controls are in parity 0;
-  // a member is finished when its latest round did not step it.
+  // a member is finished when its latest round did not step it.  Finished
+  // members leave the round tables at the next replay (their final control
+  // stays in parity 0, which no later round writes).
   const long long bound = round_bound(b);
   long long rounds = 0;
+  std::vector<char> finished(b->nm, 0);
+  ec_status st = upload_round_tables(b, finished);
+  if (st != EC_OK) return st;
   for (;;) {
     // the groups' streams start after `stream` and `stream` waits for them
     BT_TRY(b, hipEventRecord(b->gev[0], b->stream));
@@ -634,9 +683,17 @@ ec_status run_graph(ec_batch* b, int* rounds_out)
     BT_TRY(b, hipMemcpyAsync(b->ctrl_host, b->ctrl, sizeof(DevCtrl) * (size_t)b->nm,
                              hipMemcpyDeviceToHost, b->stream));
     BT_TRY(b, hipStreamSynchronize(b->stream));
-    bool done = true;
-    for (int m = 0; m < b->nm && done; ++m) done = b->ctrl_host[m].counter == 0;
+    bool done = true, changed = false;
+    for (int m = 0; m < b->nm; ++m) {
+      const bool fin = b->ctrl_host[m].counter == 0;
+      done = done && fin;
+      if (fin && !finished[m]) {
+        finished[m] = 1;
+        changed = true;
+      }
+    }
     if (done) break;
+    if (changed && (st = upload_round_tables(b, finished)) != EC_OK) return st;
     if (rounds > bound + R) {
       b->last_error = "ensemble members still running past their max_step";
       return EC_ERR_HIP;

@@ -93,18 +93,18 @@ __device__ __forceinline__ Cons hllc_w(const Prim& L, const Prim& R, const FaceS
   const double s_r = fmax(un_l + a.c, un_r + b.c);
   bool fallback = !(s_l < s_r);
   degen = fallback;
-  if (!fallback) {
-    if (s_l >= 0.0) return phys_w<AX>(L, a.E);
-    if (s_r <= 0.0) return phys_w<AX>(R, b.E);
+  if (EC_LIKELY(!fallback)) {
+    if (EC_UNLIKELY(s_l >= 0.0)) return phys_w<AX>(L, a.E);
+    if (EC_UNLIKELY(s_r <= 0.0)) return phys_w<AX>(R, b.E);
     const double dl = s_l - un_l, dr = s_r - un_r;
     const double den = L.r * dl - R.r * dr;
     fallback = fabs(den) <= 1.0e-12;
-    if (!fallback) {
+    if (EC_LIKELY(!fallback)) {
       const double s_m = quot(R.p - L.p + L.r * un_l * dl - R.r * un_r * dr, recip(den));
       const double lsd = s_l - s_m;
       const double rsd = s_r - s_m;
       fallback = fabs(lsd) <= 1.0e-12 || fabs(rsd) <= 1.0e-12;
-      if (!fallback) {
+      if (EC_LIKELY(!fallback)) {
         const double p_star_l = L.p + L.r * dl * (s_m - un_l);
         const double p_star_r = R.p + R.r * dr * (s_m - un_r);
         const double p_star = 0.5 * (p_star_l + p_star_r);
@@ -165,16 +165,16 @@ __device__ __forceinline__ Cons xie_hllcp_w(const Prim& L, const Prim& R, const 
   const double s_r = fmax(q_l + c_l, q_r + c_r);
   degen = true;
   do {
-    if (!(s_l < s_r)) break;
+    if (EC_UNLIKELY(!(s_l < s_r))) break;
     degen = false;
-    if (s_l >= 0.0) return phys_w<AX>(L, a.E);
-    if (s_r <= 0.0) return phys_w<AX>(R, b.E);
+    if (EC_UNLIKELY(s_l >= 0.0)) return phys_w<AX>(L, a.E);
+    if (EC_UNLIKELY(s_r <= 0.0)) return phys_w<AX>(R, b.E);
     const double dl = s_l - q_l, dr = s_r - q_r;
     const double alpha_l = L.r * dl;
     const double alpha_r = R.r * dr;
     const double alpha_denom = alpha_r - alpha_l;
     degen = true;
-    if (fabs(alpha_denom) <= 1.0e-12) break;
+    if (EC_UNLIKELY(fabs(alpha_denom) <= 1.0e-12)) break;
     const Recip ra = recip(alpha_denom);
     // hllc_star_state_is_degenerate: its denominator rho_l (s_l - u_l) -
     // rho_r (s_r - u_r) is -alpha_denom exactly, and a / -b == -(a / b).
@@ -184,8 +184,8 @@ __device__ __forceinline__ Cons xie_hllcp_w(const Prim& L, const Prim& R, const 
     const double s_star = quot(alpha_r * q_r - alpha_l * q_l + L.p - R.p, ra);
     const double rld = s_l - s_star;
     const double rrd = s_r - s_star;
-    if (fabs(rld) <= 1.0e-12 || fabs(rrd) <= 1.0e-12 || fabs(alpha_l) <= 1.0e-12 ||
-        fabs(alpha_r) <= 1.0e-12) {
+    if (EC_UNLIKELY(fabs(rld) <= 1.0e-12 || fabs(rrd) <= 1.0e-12 || fabs(alpha_l) <= 1.0e-12 ||
+                    fabs(alpha_r) <= 1.0e-12)) {
       break;
     }
     const double p_star = quot(alpha_r * L.p - alpha_l * R.p - alpha_l * alpha_r * (q_l - q_r), ra);
@@ -200,7 +200,7 @@ __device__ __forceinline__ Cons xie_hllcp_w(const Prim& L, const Prim& R, const 
     const double h_hat = quot_pos(w_l * h_l + w_r * h_r, rroe);
     const double uv2 = u_hat * u_hat + v_hat * v_hat;
     const double c_hat_sq = (gamma - 1.0) * (h_hat - 0.5 * uv2);
-    if (!(c_hat_sq > 0.0)) break;         // finite inside the window
+    if (EC_UNLIKELY(!(c_hat_sq > 0.0))) break;         // finite inside the window
     const double c_hat = sqrt_core(c_hat_sq);
     const double m_hat = quot(q_hat, recip(fmax(c_hat, 1.0e-12)));
     const double sp_l = sqrt_core0(a.q2);

@@ -178,11 +178,11 @@ EC_HD Cons cell_read(const double* __restrict__ u, long long cs, const CellSrc& 
 EC_HD Cons cell_value(const CellSrc& c, Cons v, const GridParams& g, int i, int j,
                       const double* time_src)
 {
-  if (c.kind == 2) {
+  if (EC_UNLIKELY(c.kind == 2)) {
     const double t = time_src ? *time_src : (g.time_ptr ? *g.time_ptr : g.time);
     return exact_state(*g.exact, t, i, j);
   }
-  if (c.kind == 1) return fixed_state(g, c.side);
+  if (EC_UNLIKELY(c.kind == 1)) return fixed_state(g, c.side);
   if (c.rx) v.mx = -v.mx;
   if (c.ry) v.my = -v.my;
   return v;

@@ -21,6 +21,12 @@ namespace ec {
 
 #define EC_HD __host__ __device__ __forceinline__
 
+// Branch hints for the code layout: the fallback and boundary paths are
+// placed out of line, so that the instruction fetch of the ordinary path
+// stays sequential.  They change no result.
+#define EC_LIKELY(x) __builtin_expect(!!(x), 1)
+#define EC_UNLIKELY(x) __builtin_expect(!!(x), 0)
+
 struct Prim { double r, u, v, p; };
 struct Cons { double r, mx, my, E; };
 

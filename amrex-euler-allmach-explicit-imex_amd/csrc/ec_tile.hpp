@@ -99,7 +99,7 @@ __device__ __forceinline__ void red_cell(Red& r, const Cons& u, const GridParams
 // results, bit for bit.  Both choices are wavefront-uniform.
 __device__ __forceinline__ void red_cell_w(Red& r, const Cons& u, const GridParams& g)
 {
-  if (!__all(pos_in_window(u.r) ? 1 : 0)) {
+  if (EC_UNLIKELY(!__all(pos_in_window(u.r) ? 1 : 0))) {
     red_cell(r, u, g);
     return;
   }
@@ -115,7 +115,7 @@ __device__ __forceinline__ void red_cell_w(Red& r, const Cons& u, const GridPara
   r.pmin = fmin(r.pmin, bad ? -INFINITY : p);
   r.bad += bad ? 1 : 0;
   double c;
-  if (__all(pos_in_window(p) ? 1 : 0)) c = sqrt_core(quot_pos(g.gamma * p, rr));
+  if (EC_LIKELY(__all(pos_in_window(p) ? 1 : 0))) c = sqrt_core(quot_pos(g.gamma * p, rr));
   else c = sqrt(g.gamma * p / u.r);
   r.rate = fmax(r.rate, cell_rate(ux, uy, c, g));
 }
@@ -369,56 +369,77 @@ struct TileStep {
   // returns the tile's epilogue reduction (valid in thread 0).
   // diag = false skips the epilogue (CFL rate, diagnostics, degenerate
   // count): a replay with known dts needs only the state.
-  __device__ static Red run(Smem& sm, const double* __restrict__ uin, double* __restrict__ uout,
-                            long long rs, long long cs, const GridParams& g, int i0, int j0,
-                            double dt, bool diag = true, const double* time_src = nullptr)
-  {
-    double* lds = sm.lds;
-    unsigned char* degx = sm.degx;
-    unsigned char* degy = sm.degy;
-    // The thread index passes through an empty asm statement: a caller that
-    // runs several tiles in a loop cannot then hoist this tile's index
-    // arithmetic out of the loop and keep it live across every stage.
-    int tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    const double gamma = g.gamma;
-    Red mine = red_identity();
+  static constexpr int NA = (NR + NT - 1) / NT;  // stage-A cells per thread
+  static constexpr bool OWN = PREFETCH_OWN && TX * TY <= NT;
 
-    constexpr int NA = (NR + NT - 1) / NT;
-    constexpr bool OWN = PREFETCH_OWN && TX * TY <= NT;
-    double own[4] = {0.0, 0.0, 0.0, 0.0};
-    const bool interior = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= g.nx && j0 + TY + 2 <= g.ny;
-    bool ok = pos_in_window(gamma) && pos_in_window(gamma - 1.0);
-    // A. primitive variables of the tile and its 2-cell halo.  Each lane
-    // first finds where its cells come from (directly on a tile whose halo
-    // lies inside the domain, where the ghost rule is the identity; through
-    // the ghost rule otherwise), then every load of stage A is issued in
-    // straight-line code: loads under per-lane branches would each wait for
-    // the loads before them.  Lanes without a cell read a clamped one and
-    // ignore it; fixed and exact ghost cells read cell 0 and ignore it.
+  // What stage A loads, held in registers between load() and run_loaded().
+  struct Loaded {
     Cons raw[NA];
     CellSrc src[NA];
+    double own[4];
+    bool interior;
+  };
+
+  // Stage A's loads.  Each lane first finds where its cells come from
+  // (directly on a tile whose halo lies inside the domain, where the ghost
+  // rule is the identity; through the ghost rule otherwise), then every load
+  // is issued in straight-line code: loads under per-lane branches would each
+  // wait for the loads before them.  Lanes without a cell read a clamped one
+  // and ignore it; fixed and exact ghost cells read cell 0 and ignore it.
+  // Nothing waits for the loads here, so a caller can issue them early.
+  __device__ static void load(Loaded& ld, const double* __restrict__ uin, long long rs, long long cs,
+                              const GridParams& g, int i0, int j0)
+  {
+    const int tid = threadIdx.x;
+    ld.interior = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= g.nx && j0 + TY + 2 <= g.ny;
     if constexpr (OWN) {
       const int ci = min(tid % TX, g.nx - i0 - 1), cj = min(tid / TX, min(TY, g.ny - j0) - 1);
       const long long k = (long long)(j0 + cj) * rs + (i0 + ci);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) own[c] = uin[c * cs + k];
+      for (int c = 0; c < 4; ++c) ld.own[c] = uin[c * cs + k];
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
       const int t = min(tid + a * NT, NR - 1);
       const int i = i0 + t % RX - 2, j = j0 + t / RX - 2;
-      if (interior) src[a] = CellSrc{(long long)j * rs + i, 0, 0, false, false};
-      else src[a] = cell_src(g, rs, i, j);
-      const long long k = src[a].kind == 0 ? src[a].k : 0;
-      raw[a] = Cons{uin[k], uin[cs + k], uin[2 * cs + k], uin[3 * cs + k]};
+      if (ld.interior) ld.src[a] = CellSrc{(long long)j * rs + i, 0, 0, false, false};
+      else ld.src[a] = cell_src(g, rs, i, j);
+      const long long k = ld.src[a].kind == 0 ? ld.src[a].k : 0;
+      ld.raw[a] = Cons{uin[k], uin[cs + k], uin[2 * cs + k], uin[3 * cs + k]};
     }
+  }
+
+  __device__ static Red run(Smem& sm, const double* __restrict__ uin, double* __restrict__ uout,
+                            long long rs, long long cs, const GridParams& g, int i0, int j0,
+                            double dt, bool diag = true, const double* time_src = nullptr)
+  {
+    Loaded ld;
+    load(ld, uin, rs, cs, g, i0, j0);
+    return run_loaded(sm, ld, uin, uout, rs, cs, g, i0, j0, dt, diag, time_src);
+  }
+
+  // The step of the tile from the values load() fetched from uin.
+  __device__ static Red run_loaded(Smem& sm, const Loaded& ld, const double* __restrict__ uin,
+                                   double* __restrict__ uout, long long rs, long long cs,
+                                   const GridParams& g, int i0, int j0, double dt, bool diag = true,
+                                   const double* time_src = nullptr)
+  {
+    double* lds = sm.lds;
+    unsigned char* degx = sm.degx;
+    unsigned char* degy = sm.degy;
+    const int tid = threadIdx.x;
+    const double gamma = g.gamma;
+    Red mine = red_identity();
+    const bool interior = ld.interior;
+    bool ok = pos_in_window(gamma) && pos_in_window(gamma - 1.0);
+
+    // A. primitive variables of the tile and its 2-cell halo, from the loads.
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
       const int t = tid + a * NT;
       if (t < NR) {
-        const Cons c = interior ? raw[a]
-                                : cell_value(src[a], raw[a], g, i0 + t % RX - 2, j0 + t / RX - 2, time_src);
+        const Cons c = interior ? ld.raw[a]
+                                : cell_value(ld.src[a], ld.raw[a], g, i0 + t % RX - 2, j0 + t / RX - 2, time_src);
         const Prim q = to_prim(c, gamma);
         lds[OFF_Q + t] = q.r;
         lds[OFF_Q + NR + t] = q.u;
@@ -473,7 +494,7 @@ struct TileStep {
         }
       };
       constexpr bool HW = true;
-      if (win && g.limiter == 0) slopes1(std::true_type{}, std::bool_constant<HW>{});
+      if (EC_LIKELY(win && g.limiter == 0)) slopes1(std::true_type{}, std::bool_constant<HW>{});
       else if (win) slopes1(std::false_type{}, std::bool_constant<HW>{});
       else slopes1(std::false_type{}, std::false_type{});
       __syncthreads();
@@ -516,7 +537,7 @@ struct TileStep {
       // minmod on finite differences when every state of the tile is inside
       // the window
       constexpr bool HW = true;
-      if (win && g.limiter == 0) slopes(std::true_type{}, std::bool_constant<HW>{});
+      if (EC_LIKELY(win && g.limiter == 0)) slopes(std::true_type{}, std::bool_constant<HW>{});
       else if (win) slopes(std::false_type{}, std::bool_constant<HW>{});
       else slopes(std::false_type{}, std::false_type{});
       __syncthreads();
@@ -577,7 +598,7 @@ struct TileStep {
       return quot_pos(fmin(pl, pr), recip(fmax(pl, pr)));
     };
     if constexpr (HP) {
-      if (win) {
+      if (EC_LIKELY(win)) {
         for (int t = tid; t < NXR + NYR; t += NT) {
           if (t < NXR) {
             const int fi = t % XRX, j = t / XRX - 1;
@@ -704,7 +725,7 @@ struct TileStep {
       }
       if constexpr (SPLIT) {
         __syncthreads();  // every x-face has read the x-slopes
-        if (W && g.limiter == 0) slopes2(std::true_type{}, std::bool_constant<W>{});
+        if (EC_LIKELY(W && g.limiter == 0)) slopes2(std::true_type{}, std::bool_constant<W>{});
         else slopes2(std::false_type{}, std::bool_constant<W>{});
         __syncthreads();
       }
@@ -714,7 +735,7 @@ struct TileStep {
         if (f < NYF) face(std::integral_constant<int, 1>{}, w_c, f, fyv[p], dfy[p]);
       }
     };
-    if (win) all_faces(std::true_type{});
+    if (EC_LIKELY(win)) all_faces(std::true_type{});
     else all_faces(std::false_type{});
     __syncthreads();  // stage A-C storage is dead from here on
   #pragma unroll
@@ -755,7 +776,7 @@ struct TileStep {
         for (int c = 0; c < 4; ++c) {
           const double fxl = lds[OFF_FX + c * NXF + xl], fxr = lds[OFF_FX + c * NXF + xl + 1];
           const double fyb = lds[OFF_FY + c * NYF + yb], fyt = lds[OFF_FY + c * NYF + yb + TX];
-          const double old = OWN ? own[c] : uin[c * cs + k];
+          const double old = OWN ? ld.own[c] : uin[c * cs + k];
           nu[c] = old - dt * ((fxr - fxl) * inv_dx + (fyt - fyb) * inv_dy);
           uout[c * cs + k] = nu[c];
         }
