@@ -50,6 +50,16 @@ __device__ __forceinline__ int xcd_banded(int b, int nb)
   return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// Wave priority inside the tile step.  The stages between barriers (loads
+// and conversion, slopes, pressure ratios, the flux stores, the update) run
+// at a raised priority and the two flux passes at the base priority: when a
+// SIMD has a wave of each kind ready, it issues the barrier-bound stage first,
+// so that workgroup reaches its own flux passes sooner and the flux work of
+// the other workgroups fills the issue slots that the barriers leave
+// (measured on the 2048^2 HLLC-P step: 197 -> 187 us per launch).
+__device__ __forceinline__ void prio_raised() { __builtin_amdgcn_s_setprio(1); }
+__device__ __forceinline__ void prio_base() { __builtin_amdgcn_s_setprio(0); }
+
 struct Red {
   double rate, rmin, pmin;
   int bad, deg;
@@ -414,6 +424,7 @@ struct TileStep {
                             double dt, bool diag = true, const double* time_src = nullptr)
   {
     Loaded ld;
+    prio_raised();
     load(ld, uin, rs, cs, g, i0, j0);
     return run_loaded(sm, ld, uin, uout, rs, cs, g, i0, j0, dt, diag, time_src);
   }
@@ -424,6 +435,7 @@ struct TileStep {
                                    const GridParams& g, int i0, int j0, double dt, bool diag = true,
                                    const double* time_src = nullptr)
   {
+    prio_raised();
     double* lds = sm.lds;
     unsigned char* degx = sm.degx;
     unsigned char* degy = sm.degy;
@@ -718,22 +730,26 @@ struct TileStep {
     };
     auto all_faces = [&](auto w_c) {
       constexpr bool W = decltype(w_c)::value;
+      prio_base();
   #pragma unroll
       for (int p = 0; p < NPX; ++p) {
         const int f = tid + p * NT;
         if (f < NXF) face(std::integral_constant<int, 0>{}, w_c, f, fxv[p], dfx[p]);
       }
+      prio_raised();
       if constexpr (SPLIT) {
         __syncthreads();  // every x-face has read the x-slopes
         if (EC_LIKELY(W && g.limiter == 0)) slopes2(std::true_type{}, std::bool_constant<W>{});
         else slopes2(std::false_type{}, std::bool_constant<W>{});
         __syncthreads();
       }
+      prio_base();
   #pragma unroll
       for (int p = 0; p < NPY; ++p) {
         const int f = tid + p * NT;
         if (f < NYF) face(std::integral_constant<int, 1>{}, w_c, f, fyv[p], dfy[p]);
       }
+      prio_raised();
     };
     if (EC_LIKELY(win)) all_faces(std::true_type{});
     else all_faces(std::false_type{});
