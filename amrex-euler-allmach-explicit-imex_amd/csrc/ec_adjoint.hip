@@ -509,17 +509,18 @@ EC_HD void adj_cell_item(const AdjArgs& A, long long t)
 }
 
 // The cell pass as LDS tiles of CTX x CTY grown cells (one per thread): the
-// primitives of the tile plus a 2-cell ring and the slope bars of the tile
-// plus a 1-cell ring are staged once, so each limiter gradient and slot is
-// read from LDS rather than recomputed per neighbour.  Same arithmetic, in
-// the same order, as adj_cell_item.
+// primitives of the tile plus a 2-cell ring, and the slope bars and summed
+// face-state bars of the tile plus a 1-cell ring, are staged once, so each
+// limiter gradient and slot is read from LDS rather than recomputed per
+// neighbour, and every face-state bar slot is read from memory once.  Same
+// arithmetic, in the same order, as adj_cell_item.
 constexpr int CTX = 32, CTY = 8, CNT = CTX * CTY;
 __global__ void __launch_bounds__(CNT) adj_cells_tile_kernel(AdjArgs A)
 {
   constexpr int QX = CTX + 4, QY = CTY + 4, NQ = QX * QY;  // cells [-2, CT+1]
   constexpr int BX = CTX + 2, BY = CTY + 2, NB = BX * BY;  // cells [-1, CT]
   __shared__ double qs[4][NQ];
-  __shared__ double sbs[8][NB];
+  __shared__ double sbs[12][NB];  // x slope bars, y slope bars, summed face-state bars
   const GridParams& g = A.g;
   const int GX = g.nx + 4, GY = g.ny + 4;
   const int tiles_x = (GX + CTX - 1) / CTX;
@@ -539,9 +540,19 @@ __global__ void __launch_bounds__(CNT) adj_cells_tile_kernel(AdjArgs A)
     }
     for (int t = threadIdx.x; t < NB; t += CNT) {
       const int i = gi0 - 1 + t % BX, j = gj0 - 1 + t / BX;
-      const Prim sx = slope_bar(A, i, j, 0), sy = slope_bar(A, i, j, 1);
-      sbs[0][t] = sx.r; sbs[1][t] = sx.u; sbs[2][t] = sx.v; sbs[3][t] = sx.p;
-      sbs[4][t] = sy.r; sbs[5][t] = sy.u; sbs[6][t] = sy.v; sbs[7][t] = sy.p;
+      const Prim e = qb_load(A, 0, i, j), w = qb_load(A, 1, i, j);
+      const Prim n = qb_load(A, 2, i, j), so = qb_load(A, 3, i, j);
+      // slope_bar of each axis, and the sum the cell's own bar starts from
+      sbs[0][t] = 0.5 * (e.r - w.r); sbs[1][t] = 0.5 * (e.u - w.u);
+      sbs[2][t] = 0.5 * (e.v - w.v); sbs[3][t] = 0.5 * (e.p - w.p);
+      sbs[4][t] = 0.5 * (n.r - so.r); sbs[5][t] = 0.5 * (n.u - so.u);
+      sbs[6][t] = 0.5 * (n.v - so.v); sbs[7][t] = 0.5 * (n.p - so.p);
+      Prim sum{0.0, 0.0, 0.0, 0.0};
+      add(sum, e);
+      add(sum, w);
+      add(sum, n);
+      add(sum, so);
+      sbs[8][t] = sum.r; sbs[9][t] = sum.u; sbs[10][t] = sum.v; sbs[11][t] = sum.p;
     }
     __syncthreads();
   }
@@ -553,7 +564,12 @@ __global__ void __launch_bounds__(CNT) adj_cells_tile_kernel(AdjArgs A)
     return Prim{qs[0][t], qs[1][t], qs[2][t], qs[3][t]};
   };
   Prim qbar{0.0, 0.0, 0.0, 0.0};
-  for (int side = 0; side < 4; ++side) add(qbar, qb_load(A, side, i, j));
+  if (o2) {
+    const int ot = (lj + 1) * BX + (li + 1);
+    qbar = Prim{sbs[8][ot], sbs[9][ot], sbs[10][ot], sbs[11][ot]};
+  } else {
+    for (int side = 0; side < 4; ++side) add(qbar, qb_load(A, side, i, j));
+  }
   if (o2) {
     for (int axis = 0; axis < 2; ++axis) {
       const int di = axis == 0 ? 1 : 0, dj = axis == 0 ? 0 : 1;
