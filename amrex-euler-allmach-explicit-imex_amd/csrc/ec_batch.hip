@@ -74,6 +74,7 @@ struct BatchArgs {
   int* error;              // persistent mode: set by a timed-out wait
   unsigned int* arrive;    // persistent mode: per-member arrival counters
   unsigned int spin_limit;
+  long long max_rounds;    // persistent mode: rounds no run can exceed (largest max_step + 1)
 };
 
 struct RoundShared {
@@ -206,14 +207,18 @@ __global__ void __launch_bounds__(NT) batch_round_kernel(BatchArgs A, int p)
 
 // Waits until the member's arrival counter reaches target.  Bounded: after
 // spin_limit polls, or when another workgroup has raised the error flag, it
-// raises the flag and reports failure.  Every thread fences (release) before
-// the arrival and (acquire) after the wait.
+// raises the flag and reports failure.  The workgroup barrier orders the
+// other threads' stores before thread 0's agent-scope release fence and
+// arrival, and thread 0's acquire fence after the wait before every
+// thread's loads (one fence pair per workgroup: each agent-scope release
+// writes the XCD's L2 back, and a fence in every wavefront cost 1.5x the
+// round time).
 __device__ bool member_barrier(const BatchArgs& A, RoundShared& rsh, int m, unsigned add,
                                unsigned target)
 {
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
+    __threadfence();
     rsh.abort = 0;
     atomicAdd(A.arrive + m, add);
     unsigned spins = 0;
@@ -226,9 +231,9 @@ __device__ bool member_barrier(const BatchArgs& A, RoundShared& rsh, int m, unsi
       }
       __builtin_amdgcn_s_sleep(2);
     }
+    __threadfence();
   }
   __syncthreads();
-  __threadfence();
   return rsh.abort == 0;
 }
 
@@ -248,8 +253,14 @@ __global__ void __launch_bounds__(NT) batch_persistent_kernel(BatchArgs A, const
   const int first_member = A.tile_member[range.x];
   for (int k = threadIdx.x; k < kMaxOwned; k += NT) done[k] = 0;
   __syncthreads();
-  for (int round = 0;; ++round) {
-    const int p = round & 1;
+  for (long long round = 0;; ++round) {
+    // every member has stopped by then: a run control that still reads
+    // "running" is an error, and the workgroup leaves (so the launch drains)
+    if (round > A.max_rounds) {
+      if (threadIdx.x == 0) atomicOr(A.error, 1);
+      return;
+    }
+    const int p = (int)(round & 1);
     bool any = false;
     int t = range.x;
     while (t < range.y) {
@@ -565,6 +576,9 @@ BatchArgs make_args(ec_batch* b)
   a.error = b->error;
   a.arrive = b->arrive;
   a.spin_limit = b->spin_limit;
+  long long mx = 0;
+  for (const auto& d : b->desc) mx = std::max<long long>(mx, d.max_step);
+  a.max_rounds = mx + 1;
   return a;
 }
 
