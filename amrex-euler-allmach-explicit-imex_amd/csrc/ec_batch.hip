@@ -768,7 +768,7 @@ ec_status run_persistent(ec_batch* b, int* rounds_out)
   BT_TRY(b, hipStreamSynchronize(b->stream));
   err = (int)b->ctrl_host[0].counter;
   if (err != 0) {
-    b->last_error = "persistent ensemble: a member wait timed out";
+    b->last_error = "persistent ensemble: a member wait timed out, or a member ran past every max_step";
     return EC_ERR_HIP;
   }
   *rounds_out = 0;
@@ -1060,9 +1060,9 @@ extern "C" ec_status ec_batch_autotune(ec_batch* b, ec_batch_timing* table, int 
   int best_mode = EC_BATCH_GRAPH, best_shape = 0, n = 0;
   // Graph mode only: the persistent mode's per-member waits need agent-scope
   // release/acquire fences, which write back and invalidate the XCD's L2 at
-  // every wait, and it measured 3 to 5 times slower than graph mode on every
-  // tile shape (64 Riemann strips, HLLC-P: 148-239 ms against 49-78 ms).  It
-  // stays selectable through ec_batch_configure.
+  // every wait, and it measured about twice graph mode's time on every tile
+  // shape (64 Riemann strips, HLLC-P: 88-182 ms against 47-60 ms).  It stays
+  // selectable through ec_batch_configure.
   for (int mode = EC_BATCH_GRAPH; mode <= EC_BATCH_GRAPH; ++mode) {
     for (int sh = 0; sh < kNumBatchShapes; ++sh) {
       BT_TRY(b, hipMemcpyAsync(b->ua, b->backup, sb, hipMemcpyDeviceToDevice, b->stream));

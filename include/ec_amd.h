@@ -377,14 +377,18 @@ ec_status ec_batch_set_groups(ec_batch* b, int groups);
 int ec_batch_groups(const ec_batch* b);
 /* Times one complete run of every graph-mode tile shape on a copy of the
  * current states (which are restored) and keeps the fastest.  Persistent mode
- * is not timed (it measured 3-5x slower on every shape: its per-member waits
- * need device-scope fences); ec_batch_configure still selects it.  table may
- * be NULL; *n_entries receives the number of pairs timed. */
+ * is not timed (it measured about twice graph mode's time on every shape: its
+ * per-member waits need device-scope fences); ec_batch_configure still
+ * selects it.  table may be NULL; *n_entries receives the number of pairs
+ * timed. */
 ec_status ec_batch_autotune(ec_batch* b, ec_batch_timing* table, int table_cap, int* n_entries);
 /* Runs every member from its current state at time 0 until stop_time,
  * max_step or a failure status; the final states replace the current ones.
  * results (n_members entries, may be NULL) receive the members' run states;
- * used (may be NULL) the mode, tile shape and wall time of the run. */
+ * used (may be NULL) the mode, tile shape and wall time of the run.
+ * Persistent mode returns EC_ERR_HIP if a member wait timed out or a member
+ * still ran after the largest max_step of the batch (the launch ends either
+ * way). */
 ec_status ec_batch_run(ec_batch* b, ec_run_state* results, ec_batch_timing* used);
 
 /* ---------------------------------------------------- host initial data */
