@@ -101,17 +101,39 @@ explicit_step_kernel(StepArgs A)
   const int i0 = (tile % A.tiles_x) * TX;
   const int j0 = (tile / A.tiles_x) * TY;
 
-  // Run control: every workgroup derives the same dt and activity.
+  // Run control: every workgroup derives the same dt and activity.  In the
+  // device loop the activity test is cheap and every wavefront makes it; dt
+  // (a division) is formed by wavefront 0 alone into LDS and read by the
+  // others in stage E, and a bad dt (which ends the run) is read after
+  // stage A's barrier, where the tile is then carried through.
   double dt = A.dt;
-  bool active = true, dt_bad = false;
-  if (A.device_dt) loop_control(A.ctrl, dt, active, dt_bad);
-  (void)dt_bad;
+  bool active = true;
+  __shared__ double s_dt;
+  __shared__ int s_skip;
+  const double* dt_src = nullptr;
+  const int* skip_src = nullptr;
+  if (A.device_dt) {
+    const DevCtrl* c = A.ctrl;
+    active = loop_active(c);
+    if (active) {
+      if (threadIdx.x < 64) {
+        const double d = euler_dt(c->max_rate, c->cfl, c->time, c->stop_time);
+        if (threadIdx.x == 0) {
+          s_dt = d;
+          s_skip = dt_valid(d) ? 0 : 1;
+        }
+      }
+      dt_src = &s_dt;
+      skip_src = &s_skip;
+    }
+  }
   if (!active) {
     T::carry(A.uin, A.uout, A.rs, A.cs, A.g, i0, j0);
     store_partial(A.partials, tile, red_identity());
     return;
   }
-  const Red r = T::run(sm, A.uin, A.uout, A.rs, A.cs, A.g, i0, j0, dt, A.diag != 0);
+  const Red r = T::run(sm, A.uin, A.uout, A.rs, A.cs, A.g, i0, j0, dt, A.diag != 0, nullptr, dt_src,
+                       skip_src);
   if (A.diag) store_partial(A.partials, tile, r);
 }
 

@@ -255,15 +255,24 @@ __device__ __forceinline__ Red block_reduce(Red r, Red* scratch)
 }
 
 // Run control of a device-loop step, as every workgroup of the step kernel
-// and its finalize kernel derive it from the control block.
+// and its finalize kernel derive it from the control block: whether the
+// loop takes another step, and that step's dt (a dt that is not finite and
+// positive ends the run instead).
+__device__ __forceinline__ bool loop_active(const DevCtrl* c)
+{
+  return c->status == 0 && c->step < c->max_step && c->time < c->stop_time - 1.0e-14;
+}
+
+__device__ __forceinline__ bool dt_valid(double dt) { return isfinite(dt) && dt > 0.0; }
+
 __device__ __forceinline__ void loop_control(const DevCtrl* c, double& dt, bool& active, bool& dt_bad)
 {
-  active = c->status == 0 && c->step < c->max_step && c->time < c->stop_time - 1.0e-14;
+  active = loop_active(c);
   dt_bad = false;
   dt = 0.0;
   if (active) {
     dt = euler_dt(c->max_rate, c->cfl, c->time, c->stop_time);
-    if (!isfinite(dt) || dt <= 0.0) {
+    if (!dt_valid(dt)) {
       active = false;
       dt_bad = true;
     }
@@ -421,19 +430,24 @@ struct TileStep {
 
   __device__ static Red run(Smem& sm, const double* __restrict__ uin, double* __restrict__ uout,
                             long long rs, long long cs, const GridParams& g, int i0, int j0,
-                            double dt, bool diag = true, const double* time_src = nullptr)
+                            double dt, bool diag = true, const double* time_src = nullptr,
+                            const double* dt_src = nullptr, const int* skip_src = nullptr)
   {
     Loaded ld;
     prio_raised();
     load(ld, uin, rs, cs, g, i0, j0);
-    return run_loaded(sm, ld, uin, uout, rs, cs, g, i0, j0, dt, diag, time_src);
+    return run_loaded(sm, ld, uin, uout, rs, cs, g, i0, j0, dt, diag, time_src, dt_src, skip_src);
   }
 
   // The step of the tile from the values load() fetched from uin.
+  // dt_src, skip_src (optional, LDS): a dt formed by one wavefront before
+  // stage A's barrier and read in stage E, and a flag, read after that
+  // barrier, that the step is not taken (the tile is carried through).
   __device__ static Red run_loaded(Smem& sm, const Loaded& ld, const double* __restrict__ uin,
                                    double* __restrict__ uout, long long rs, long long cs,
                                    const GridParams& g, int i0, int j0, double dt, bool diag = true,
-                                   const double* time_src = nullptr)
+                                   const double* time_src = nullptr, const double* dt_src = nullptr,
+                                   const int* skip_src = nullptr)
   {
     prio_raised();
     double* lds = sm.lds;
@@ -465,6 +479,10 @@ struct TileStep {
       if ((tid & 63) == 0) sm.wave_ok[tid >> 6] = wave_all;
     }
     __syncthreads();
+    if (skip_src && EC_UNLIKELY(*skip_src != 0)) {
+      carry(uin, uout, rs, cs, g, i0, j0);
+      return red_identity();
+    }
     bool win = true;
   #pragma unroll
     for (int w = 0; w < NT / 64; ++w) win = win && sm.wave_ok[w] != 0;
@@ -781,6 +799,7 @@ struct TileStep {
     __syncthreads();
 
     // E. conservative update (explicit_update.hpp:90-96) and the epilogue.
+    if (dt_src) dt = *dt_src;
     const double inv_dx = g.inv_dx, inv_dy = g.inv_dy;
     for (int t = tid; t < TX * TY; t += NT) {
       const int ci = t % TX, cj = t / TX;

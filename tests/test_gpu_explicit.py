@@ -186,6 +186,12 @@ def test_run_status_paths(ecor, ecamd, gpu):
     got, run = ecamd.run_explicit(s, to_dev(bad, gpu), 0.4, 1.0, 5)
     assert run.status == 1 and run.step == 0
     assert np.array_equal(got.cpu().numpy(), bad)
+    # a dt that is not finite (infinite CFL number and stop time) ends the run
+    # with invalid_time_step before any step (app/driver_main.hpp:242-247),
+    # every tile carried through unchanged
+    got, run = ecamd.run_explicit(s, to_dev(U, gpu), float("inf"), float("inf"), 5)
+    assert run.status == 2 and run.step == 0 and s.status_name(run) == "invalid_time_step"
+    assert np.array_equal(got.cpu().numpy(), U)
     # standalone diagnostics agree with the oracle
     d = s.diagnostics(to_dev(bad, gpu))
     r, p, nf = ecor.diagnostics(g, bad)
