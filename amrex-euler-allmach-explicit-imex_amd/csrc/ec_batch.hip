@@ -318,9 +318,13 @@ struct BatchShape {
 };
 // 48x4: both face sets of a 4-row strip tile in one pass of 256 threads
 // (196 x-faces, 240 y-faces); 60x4 needs a second y pass but has fewer halo
-// columns per cell; 16x15 suits 2D members.
-static const BatchShape kBatchShapes[] = {{60, 4, 256}, {124, 4, 512}, {16, 15, 256}, {48, 4, 256}};
+// columns per cell; 16x15 suits 2D members.  24x4: the y-faces get wavefronts
+// of their own (TileStep::YOFF; 100 x-faces on wavefronts 0-1, 120 y-faces on
+// 2-3), so each wavefront computes one face, and stage A has one cell per
+// thread: the shortest chain through a round that waits on its slowest tile.
+static const BatchShape kBatchShapes[] = {{60, 4, 256}, {124, 4, 512}, {16, 15, 256}, {48, 4, 256}, {24, 4, 256}};
 constexpr int kNumBatchShapes = sizeof(kBatchShapes) / sizeof(kBatchShapes[0]);
+static_assert(BatchTile<24, 4, 256, 3, 2>::YOFF == 128, "24x4/256: y-faces on wavefronts 2-3");
 
 struct BatchFns {
   const void* round;       // batch_round_kernel
@@ -358,6 +362,7 @@ static BatchFns batch_fns(int shape, int riemann, int order)
     case 1: return fns_for_shape<124, 4, 512>(riemann, order);
     case 2: return fns_for_shape<16, 15, 256>(riemann, order);
     case 3: return fns_for_shape<48, 4, 256>(riemann, order);
+    case 4: return fns_for_shape<24, 4, 256>(riemann, order);
     default: return fns_for_shape<60, 4, 256>(riemann, order);
   }
 }

@@ -345,6 +345,13 @@ struct TileStep {
   // both pressure slopes stay, for the sensor's ratios.  Five slope planes
   // instead of eight.
   static constexpr bool SPLIT = O2 && SPLIT_SLOPES;
+  // YOFF: with every slope plane held (no barrier between the two face
+  // passes) and room for both face sets, the y-faces go to the wavefronts
+  // after the x-faces' (y-face f on thread YOFF + f), so that the two passes
+  // run side by side on different wavefronts instead of one after the other
+  // in the same threads; 0 otherwise.
+  static constexpr int XW = (NXF + 63) / 64 * 64;
+  static constexpr int YOFF = (!SPLIT && XW + NYF <= NT) ? XW : 0;
   static constexpr int OFF_Q = 0;
   static constexpr int OFF_S = OFF_Q + 4 * NR;
   static constexpr int NSP = O2 ? (SPLIT ? 5 : 8) : 0;
@@ -764,8 +771,8 @@ struct TileStep {
       prio_base();
   #pragma unroll
       for (int p = 0; p < NPY; ++p) {
-        const int f = tid + p * NT;
-        if (f < NYF) face(std::integral_constant<int, 1>{}, w_c, f, fyv[p], dfy[p]);
+        const int f = tid + p * NT - YOFF;
+        if ((YOFF == 0 || f >= 0) && f < NYF) face(std::integral_constant<int, 1>{}, w_c, f, fyv[p], dfy[p]);
       }
       prio_raised();
     };
@@ -786,8 +793,8 @@ struct TileStep {
     }
   #pragma unroll
     for (int p = 0; p < NPY; ++p) {
-      const int f = tid + p * NT;
-      if (f < NYF) {
+      const int f = tid + p * NT - YOFF;
+      if ((YOFF == 0 || f >= 0) && f < NYF) {
         double* base = lds + OFF_FY + f;
         base[0] = fyv[p].r;
         base[NYF] = fyv[p].mx;

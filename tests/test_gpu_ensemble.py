@@ -75,7 +75,7 @@ def oracle_capped(ecor):
 
 @pytest.mark.parametrize("riemann", list(RIEMANN_ID))
 @pytest.mark.parametrize("mode", ["graph", "persistent"])
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 def test_capped_members_match_oracle(gpu, oracle_capped, riemann, mode, tile):
     print(f"ensemble capped {riemann} {mode} tile {tile}", flush=True)
     ens = build(gpu, riemann, 0.1, 10)
@@ -153,7 +153,7 @@ def test_autotune_restores_states(gpu):
 
 
 @pytest.mark.parametrize("mode", ["graph", "persistent"])
-@pytest.mark.parametrize("tile", [0, 2])
+@pytest.mark.parametrize("tile", [0, 2, 4])
 def test_mixed_2d_and_strip_members(ecor, gpu, mode, tile):
     """Members of different kinds in one batch: Gresho boxes of several sizes
     (periodic), a strip and a shock-bubble-rule box (fixed / outflow /
