@@ -543,10 +543,10 @@ def frozen_branch_fd(rho, u, v, p, Ub, dts, inflow, D, grad, region, dx, dy, eps
 
 
 def bench_ensemble(dev, riemann: str, stop_time: float = 0.1, cfl: float = 0.8,
-                   mode: str = "auto") -> dict:
+                   mode: str = "auto", tile: int = 0) -> dict:
     """The 64 seeded Riemann strips (tests/golden/ensemble_64.json), each run
     to stop_time: autotuned (mode, tile) table and the best run's throughput
-    (mode "graph" or "persistent": that mode with tile shape 0, no autotune)."""
+    (mode "graph" or "persistent": that mode with tile shape `tile`, no autotune)."""
     import numpy as np
     from ecamd.batch import Ensemble, riemann_strip_members, tile_shapes
     table = json.loads((ROOT / "tests" / "golden" / "ensemble_64.json").read_text())
@@ -568,7 +568,7 @@ def bench_ensemble(dev, riemann: str, stop_time: float = 0.1, cfl: float = 0.8,
         tuned = ens.autotune()
     else:
         tuned = []
-        ens.configure(mode, 0)
+        ens.configure(mode, tile)
     res, used = ens.run()
     cell_updates = sum(r.step * m["nx"] * 4 for r, m in zip(res, members))
     shapes = tile_shapes()
@@ -604,7 +604,9 @@ def main() -> int:
     ap.add_argument("--only", choices=("imex", "adjoint", "ensemble", "direct"),
                     help="print only that measurement")
     ap.add_argument("--ensemble-mode", choices=("auto", "graph", "persistent"), default="auto",
-                    help="ensemble rounds: autotuned, or one mode with tile shape 0")
+                    help="ensemble rounds: autotuned, or one mode with the tile shape of --ensemble-tile")
+    ap.add_argument("--ensemble-tile", type=int, default=0,
+                    help="ensemble tile shape id for --ensemble-mode graph or persistent")
     ap.add_argument("--summarize-counters", nargs="+", metavar="KIND=DIR",
                     help="write the profile summaries from rocprofv3 --pmc run directories "
                          "(sq=, fetch=, write=, copy_fetch=, copy_write=) and exit")
@@ -630,7 +632,7 @@ def main() -> int:
     if args.only:
         fn = {"imex": lambda: bench_imex(dev), "adjoint": lambda: bench_adjoint(dev),
               "direct": lambda: bench_direct(dev),
-              "ensemble": lambda: {r: bench_ensemble(dev, r, mode=args.ensemble_mode)
+              "ensemble": lambda: {r: bench_ensemble(dev, r, mode=args.ensemble_mode, tile=args.ensemble_tile)
                                    for r in ("xie_am_hllc_p", "hllc")}}
         print(json.dumps({args.only: fn[args.only]()}))
         return 0
