@@ -8,6 +8,7 @@ namespace ec {
 
 int num_tile_shapes();
 int step_partials_needed(const ec_ctx* ctx, int tile_id);
+int default_tile_id(int nx, int ny, int device);
 int reduce_blocks(const ec_ctx* ctx);
 void imex_release(ec_ctx* ctx);
 void adjoint_release(ec_ctx* ctx);
@@ -99,6 +100,7 @@ extern "C" ec_status ec_create(const ec_grid_desc* desc, void* stream, ec_ctx** 
   // room for the second-level partials of a large grid's reduction
   need += kPartialGroupsMax;
   ctx->max_partials = need;
+  ctx->tile_id = default_tile_id(desc->nx, desc->ny, ctx->device);
   const size_t state_bytes = sizeof(double) * 4 * (size_t)desc->nx * desc->ny;
   if (hipMalloc(&ctx->ctrl, sizeof(DevCtrl)) != hipSuccess ||
       hipHostMalloc(&ctx->ctrl_host, sizeof(DevCtrl)) != hipSuccess ||

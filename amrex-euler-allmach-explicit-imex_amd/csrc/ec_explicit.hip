@@ -84,18 +84,26 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const Partial* partials, i
 // Waves per SIMD the step kernel is compiled for: the 256-thread tiles (LDS
 // below 32 KiB with the split slopes) run five workgroups per CU when their
 // registers fit 96 VGPRs; the 512-thread tile is held to two workgroups by its
-// LDS anyway.
-template <int NT>
+// LDS anyway.  The strip tile (TY = 4) serves grids too small to fill the chip,
+// where the length of one tile's chain matters more than occupancy: it keeps
+// every slope plane, so that its y-faces run on wavefronts of their own
+// (TileStep::YOFF), with the registers of four workgroups per CU.
+template <int TY>
+constexpr bool step_split_slopes()
+{
+  return TY > 4;
+}
+template <int TY, int NT>
 constexpr int step_waves_per_eu()
 {
-  return NT == 256 ? 5 : 1;
+  return NT == 256 ? (step_split_slopes<TY>() ? 5 : 4) : 1;
 }
 
 template <int TX, int TY, int NT, int RIEMANN, int ORDER>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<NT>())))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<TY, NT>())))
 explicit_step_kernel(StepArgs A)
 {
-  using T = TileStep<TX, TY, NT, RIEMANN, ORDER>;
+  using T = TileStep<TX, TY, NT, RIEMANN, ORDER, step_split_slopes<TY>()>;
   __shared__ typename T::Smem sm;
   const int tile = xcd_banded(blockIdx.x, A.nblocks);
   const int i0 = (tile % A.tiles_x) * TX;
@@ -189,14 +197,16 @@ struct TileFns {
   }
 };
 
-static const TileShape kShapes[] = {{16, 15, 256}, {32, 15, 512}, {32, 7, 256}};
+static const TileShape kShapes[] = {{16, 15, 256}, {32, 15, 512}, {32, 7, 256}, {24, 4, 256}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+static_assert(TileStep<24, 4, 256, 3, 2, step_split_slopes<4>()>::YOFF == 128, "24x4: y-faces on wavefronts 2-3");
 
 static StepFn step_fn(int tile_id, int riemann, int order)
 {
   switch (tile_id) {
     case 1: return TileFns<32, 15, 512>::get(riemann, order);
     case 2: return TileFns<32, 7, 256>::get(riemann, order);
+    case 3: return TileFns<24, 4, 256>::get(riemann, order);
     default: return TileFns<16, 15, 256>::get(riemann, order);
   }
 }
@@ -213,6 +223,23 @@ int tiles_needed(int nx, int ny, int tile_id)
 {
   const TileShape s = kShapes[tile_id];
   return ((nx + s.tx - 1) / s.tx) * ((ny + s.ty - 1) / s.ty);
+}
+
+// The tile a new context starts with: the 16x15 tile, or the 24x4 strip tile
+// (id 3) for a grid whose 16x15 tiles number fewer than 1.5 per CU.  On such a
+// grid the step waits on one tile's chain of dependent work, not on the issue
+// rate, and the strip tile's chain is shorter (its y-faces run beside its
+// x-faces, and stage A has one cell per thread).  Measured with the device
+// loop: 0.76-0.84 of the 16x15 tile's step time on grids up to 256^2 (288
+// tiles of 16x15) and on 4-row strips; 1.04-1.21 from 102,400 cells (440
+// tiles) up.
+int default_tile_id(int nx, int ny, int device)
+{
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) {
+    return 0;
+  }
+  return 2 * tiles_needed(nx, ny, 0) < 3 * cus ? 3 : 0;
 }
 
 // The step kernel and its finalize on a caller-supplied workspace (run

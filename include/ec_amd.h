@@ -188,7 +188,11 @@ ec_status ec_step_kernel_time(ec_ctx* ctx, double* total_ms, int* launches);
 
 /* Kernel tile shape of the explicit step (autotuner hook).  tile_id indexes
  * the compiled shapes; ec_explicit_tile_count() says how many exist and
- * ec_explicit_tile_shape() describes one. */
+ * ec_explicit_tile_shape() describes one.  A new context starts with the
+ * 16x15 tile (id 0), or with the 24x4 strip tile (id 3) when the grid's 16x15
+ * tiles number fewer than 1.5 per CU (up to about 80,000 cells on 256 CUs):
+ * a grid that small leaves the chip waiting on one tile's latency, which the
+ * strip tile shortens.  Every shape gives the same results bit for bit. */
 int ec_explicit_tile_count(void);
 ec_status ec_explicit_tile_shape(int tile_id, int* tx, int* ty, int* threads);
 ec_status ec_set_explicit_tile(ec_ctx* ctx, int tile_id);
